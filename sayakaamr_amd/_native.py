@@ -1,0 +1,66 @@
+"""Loader of the native libraries built in-tree under sayakaamr_amd/lib/.
+
+libsayaka_host.so  host tree, tables and the solver driver (C ABI: include/sayaka_host.h)
+libsayaka_dev.so   HIP kernels for gfx950 and the device C ABI (include/sayaka_amd.h)
+
+There is no fallback: a missing library raises, so a solve can never silently run
+anywhere but on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+HOST_LIB = os.path.join(LIBDIR, "libsayaka_host.so")
+DEV_LIB = os.path.join(LIBDIR, "libsayaka_dev.so")
+
+i32, f64, vp = C.c_int, C.c_double, C.c_void_p
+_host = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+_HOST_SIG = {
+    "sy_tree_create": ([i32, i32, i32, i32, i32, i32, vp, vp, vp, vp], vp),
+    "sy_tree_destroy": ([vp], None),
+    "sy_tree_numblocks": ([vp], i32),
+    "sy_tree_finest": ([vp], i32),
+    "sy_tree_set_flags": ([vp, vp, vp, i32], i32),
+    "sy_tree_regrid": ([vp], i32),
+    "sy_tree_export": ([vp] + [vp] * 14, None),
+    "sy_tree_level_count": ([vp, i32], i32),
+    "sy_tree_level_blocks": ([vp, i32, vp], None),
+}
+
+
+def host():
+    """The host library (tree, tables, solver driver); raises if it was not built."""
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(HOST_LIB):
+        raise NativeLibraryMissing(f"{HOST_LIB} not built; run __graft_entry__.build()")
+    lib = C.CDLL(HOST_LIB)
+    for name, (args, res) in _HOST_SIG.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = res
+    _host = lib
+    return lib
+
+
+def register(lib, sig: dict):
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res

@@ -1,0 +1,106 @@
+// amr_tree.h -- host-side block-structured quadtree/octree (the caller-owned tree).
+//
+// Same call surface and same node numbering as the reference AmrTree (SayakaTree.h:195-333):
+// a fixed pool of nodes, children appended at the end of the pool on refinement, survivors
+// packed on coarsening, iterative 2:1 balance, a 3^d surrounding-block list per node and a
+// per-level block cache.  The dimension is a run-time member (2 or 3) instead of the
+// reference's compile-time SAYAKA_SPACEDIM.
+#pragma once
+
+#include <vector>
+
+#include "sayaka_types.h"
+
+namespace sayaka {
+inline namespace amd {
+
+struct RealBox {
+    Vector3d vlo, vhi;
+    double center(int d) const { return 0.5 * (vlo(d) + vhi(d)); }
+    double length(int d) const { return vhi(d) - vlo(d); }
+};
+
+struct AmrTreeNode {
+    int refineLevel;
+    int neighbor[MAX_FACE];
+    int child[MAX_CHILD];
+    int parent;
+    int whichChild;
+    int nodeType;
+    int nodeType2[MAX_FACE];
+    int blockFlags;
+    Vector3d blockCenter;
+    Vector3d blockLength;
+    RealBox boundBox;
+
+    int getLevel() const { return refineLevel; }
+    bool isLeaf() const { return nodeType == NodeIsLeaf; }
+    bool isNotLeaf() const { return !isLeaf(); }
+    bool isParentOfLeafChild() const { return nodeType == NodeHasLeafChild; }
+    bool hasChildren() const { return child[0] >= 0; }
+    bool hasParent() const { return parent >= 0; }
+    void setUnused();
+};
+
+// 3^d neighbourhood of a node (SayakaTree.h:336-526): ids, node types and the surrounding
+// position each entry was reached through.  Stored [i+1][j+1][k+1] like the reference.
+struct SurroundingBlocks {
+    int blocks[3][3][3];
+    int types[3][3][3];
+    int bcrefs[3][3][3];
+    int& at(int i, int j, int k) { return blocks[i + 1][j + 1][k + 1]; }
+    int at(int i, int j, int k) const { return blocks[i + 1][j + 1][k + 1]; }
+    void setUnused();
+};
+
+struct AmrTree {
+    // configuration (set before init())
+    int ndim = 3;
+    int nb = 8;            // cells per block edge
+    int blockNumGrow = 1;  // ghost layers of user data
+    int minAmrLevel = 1;
+    int maxAmrLevel = 1;
+    int maxBlockNum = 40000;  // pool size (the reference hard-codes 40000, SayakaTree.cpp:43)
+
+    int maxTreeBlockNum = 0;
+    int numBlocks = 0;
+    int gridChanged = 0;
+
+    std::vector<AmrTreeNode> blocks;
+    std::vector<int> refineFlag, coarsenFlag, newChildFlag, stayFlag;
+    std::vector<SurroundingBlocks> surrBlocks;
+    std::vector<std::vector<int>> levels;  // levels[l] = ids of level-l blocks, ascending
+
+    int numFace() const { return 2 * ndim; }
+    int numChild() const { return 1 << ndim; }
+    int cellsPerBlock() const { return ndim == 3 ? nb * nb * nb : nb * nb; }
+
+    void init();
+    void initUniformRootLevel(const int ndiv[3], const RealBox& probbox, const int bc_type[MAX_FACE],
+                              const bool is_periodic[MAX_FACE]);
+    void initRefineToMinLevel();
+
+    // Returns 0 on success, -1 when the node pool overflows (the reference exits).
+    int regrid();
+
+    const AmrTreeNode& operator[](int i) const { return blocks[i]; }
+    AmrTreeNode& operator[](int i) { return blocks[i]; }
+
+    Vector3d getBlockCellSize(int b) const;
+    double getBlockCellVolume(int b) const;
+    Vector3d getBlockCellArea(int b) const;
+    int currentFinestLevel() const;
+    const std::vector<int>& getTreeLevel(int l) const { return levels[l]; }
+
+   private:
+    void checkCoarsen();
+    int refineBlocks();
+    int coarsenBlocks(int oldBlockNum);
+    void setNodeType();
+    void buildSurroundingBlocks();
+    void cacheTreeLevels();
+    int surrIndex(int i, int j, int k) const;
+};
+
+}  // namespace amd
+}  // namespace sayaka

@@ -1,0 +1,94 @@
+// capi_tree.cpp -- extern "C" entry points of the host tree (for ctypes and other FFIs).
+#include <cstring>
+#include <new>
+
+#include "amr_tree.h"
+#include "sayaka_host.h"
+
+using namespace sayaka;
+
+extern "C" {
+
+sy_tree* sy_tree_create(int ndim, int nb, int ngrow, int minlevel, int maxlevel, int max_blocks,
+                        const int* ndiv, const double* lo, const double* hi, const int* bc) {
+    if ((ndim != 2 && ndim != 3) || nb < 2 || nb % 2 || minlevel < 1 || maxlevel < minlevel ||
+        maxlevel >= MAX_LEVEL || max_blocks < 1)
+        return nullptr;
+    AmrTree* t = new (std::nothrow) AmrTree;
+    if (!t) return nullptr;
+    t->ndim = ndim;
+    t->nb = nb;
+    t->blockNumGrow = ngrow;
+    t->minAmrLevel = minlevel;
+    t->maxAmrLevel = maxlevel;
+    t->maxBlockNum = max_blocks;
+    t->init();
+    RealBox pb;
+    pb.vlo = Vector3d::VecMake(lo[0], lo[1], lo[2]);
+    pb.vhi = Vector3d::VecMake(hi[0], hi[1], hi[2]);
+    bool per[MAX_FACE] = {false, false, false, false, false, false};
+    int bct[MAX_FACE];
+    for (int f = 0; f < MAX_FACE; f++) bct[f] = bc[f];
+    const int nd[3] = {ndiv[0], ndiv[1], ndim == 3 ? ndiv[2] : 1};
+    t->initUniformRootLevel(nd, pb, bct, per);
+    t->initRefineToMinLevel();
+    return reinterpret_cast<sy_tree*>(t);
+}
+
+void sy_tree_destroy(sy_tree* h) { delete reinterpret_cast<AmrTree*>(h); }
+
+int sy_tree_numblocks(const sy_tree* h) { return reinterpret_cast<const AmrTree*>(h)->numBlocks; }
+int sy_tree_finest(const sy_tree* h) { return reinterpret_cast<const AmrTree*>(h)->currentFinestLevel(); }
+
+int sy_tree_set_flags(sy_tree* h, const int* refine, const int* coarsen, int n) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    if (n < 0 || n > t->maxTreeBlockNum) return -1;
+    for (int i = 0; i < n; i++) {
+        if (refine) t->refineFlag[i] = refine[i];
+        if (coarsen) t->coarsenFlag[i] = coarsen[i];
+    }
+    return 0;
+}
+
+int sy_tree_regrid(sy_tree* h) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    return t->regrid() == 0 ? t->numBlocks : -1;
+}
+
+void sy_tree_export(const sy_tree* h, int* level, int* parent, int* which_child, int* child8, int* nbr6,
+                    int* node_type, int* node_type2, double* lo3, double* hi3, double* center3,
+                    double* length3, int* surr27, int* surrtype27, int* surrbcref27) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    for (int b = 0; b < t->numBlocks; b++) {
+        const AmrTreeNode& n = t->blocks[b];
+        level[b] = n.refineLevel;
+        parent[b] = n.parent;
+        which_child[b] = n.whichChild;
+        node_type[b] = n.nodeType;
+        std::memcpy(child8 + 8 * b, n.child, 8 * sizeof(int));
+        std::memcpy(nbr6 + 6 * b, n.neighbor, 6 * sizeof(int));
+        std::memcpy(node_type2 + 6 * b, n.nodeType2, 6 * sizeof(int));
+        for (int d = 0; d < 3; d++) {
+            lo3[3 * b + d] = n.boundBox.vlo(d);
+            hi3[3 * b + d] = n.boundBox.vhi(d);
+            center3[3 * b + d] = n.blockCenter(d);
+            length3[3 * b + d] = n.blockLength(d);
+        }
+        const SurroundingBlocks& s = t->surrBlocks[b];
+        std::memcpy(surr27 + 27 * b, &s.blocks[0][0][0], 27 * sizeof(int));
+        std::memcpy(surrtype27 + 27 * b, &s.types[0][0][0], 27 * sizeof(int));
+        std::memcpy(surrbcref27 + 27 * b, &s.bcrefs[0][0][0], 27 * sizeof(int));
+    }
+}
+
+int sy_tree_level_count(const sy_tree* h, int l) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    return (l >= 1 && l <= t->maxAmrLevel) ? (int)t->levels[l].size() : 0;
+}
+void sy_tree_level_blocks(const sy_tree* h, int l, int* out) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    if (l < 1 || l > t->maxAmrLevel) return;
+    std::memcpy(out, t->levels[l].data(), t->levels[l].size() * sizeof(int));
+}
+
+}  // extern "C"

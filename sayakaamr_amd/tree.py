@@ -1,0 +1,86 @@
+"""Python mirror of the host tree (AmrTree, SayakaTree.h:195-333) over the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as N
+
+
+class AmrTree:
+    """Owning handle of a host AmrTree; also a tree backend for ``workloads.build_tree``."""
+
+    def __init__(self, ndim: int = 3, max_blocks: int = 40000):
+        self.ndim = ndim
+        self.max_blocks = max_blocks
+        self.lib = N.host()
+        self.h = None
+        self.spec = None
+
+    def create(self, spec):
+        if self.h:
+            self.lib.sy_tree_destroy(self.h)
+        self.spec = spec
+        self.ndim = spec.ndim
+        nd = np.array(spec.ndiv, dtype=np.int32)
+        lo = np.array(spec.lo, dtype=np.float64)
+        hi = np.array(spec.hi, dtype=np.float64)
+        bc = np.array(spec.bc, dtype=np.int32)
+        self.h = self.lib.sy_tree_create(spec.ndim, spec.nb, spec.ngrow, spec.minlevel, spec.maxlevel,
+                                         self.max_blocks, N.ptr(nd), N.ptr(lo), N.ptr(hi), N.ptr(bc))
+        if not self.h:
+            raise ValueError(f"bad tree spec {spec}")
+        return self.numblocks
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.sy_tree_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def numblocks(self) -> int:
+        return self.lib.sy_tree_numblocks(self.h)
+
+    @property
+    def finest(self) -> int:
+        return self.lib.sy_tree_finest(self.h)
+
+    def set_refine_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        if self.lib.sy_tree_set_flags(self.h, N.ptr(f), None, f.size) != 0:
+            raise ValueError("flag array larger than the block pool")
+
+    def set_coarsen_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        if self.lib.sy_tree_set_flags(self.h, None, N.ptr(f), f.size) != 0:
+            raise ValueError("flag array larger than the block pool")
+
+    def regrid(self) -> int:
+        n = self.lib.sy_tree_regrid(self.h)
+        if n < 0:
+            raise MemoryError("AmrTree::regrid: block pool overflow")
+        return n
+
+    def export(self) -> dict:
+        n = self.numblocks
+        t = dict(level=np.zeros(n, np.int32), parent=np.zeros(n, np.int32),
+                 which_child=np.zeros(n, np.int32), child=np.zeros((n, 8), np.int32),
+                 neighbor=np.zeros((n, 6), np.int32), node_type=np.zeros(n, np.int32),
+                 node_type2=np.zeros((n, 6), np.int32), lo=np.zeros((n, 3)), hi=np.zeros((n, 3)),
+                 center=np.zeros((n, 3)), length=np.zeros((n, 3)),
+                 surr=np.zeros((n, 27), np.int32), surr_type=np.zeros((n, 27), np.int32),
+                 surr_bcref=np.zeros((n, 27), np.int32))
+        self.lib.sy_tree_export(self.h, *[N.ptr(t[k]) for k in ("level", "parent", "which_child", "child",
+                                                                "neighbor", "node_type", "node_type2",
+                                                                "lo", "hi", "center", "length", "surr",
+                                                                "surr_type", "surr_bcref")])
+        return t
+
+    def level_blocks(self, level: int) -> np.ndarray:
+        n = self.lib.sy_tree_level_count(self.h, level)
+        out = np.zeros(n, np.int32)
+        if n:
+            self.lib.sy_tree_level_blocks(self.h, level, N.ptr(out))
+        return out
