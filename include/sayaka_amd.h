@@ -1,0 +1,139 @@
+/* sayaka_amd.h -- C ABI of the MI355X (gfx950) MAC Poisson solve.
+ *
+ * This is the thin layer between the host-side tree/solver objects and the HIP kernels:
+ * plain pointers and sizes, int status codes, no C++ types.  The reference has no device
+ * and no FFI; each entry point names the reference MacSolver member (or step) whose
+ * work it performs on the device.  The host C++ MacSolver (and the Python mirror) call
+ * these after every regrid: topology -> BC -> coefficients -> right-hand side -> solve.
+ *
+ * Data layout on the host side of every call:
+ *   cell field  [block][k][j][i], nb^d values per block (valid cells only)
+ *   face field  dir d: [block][k][j][i] over the cell box staggered in d
+ * Field ids follow the reference solver buffer components (SayakaMacSolver.h:34-53).
+ *
+ * Status: 0 ok; 1 solver breakdown; 2 not converged; negative = argument / HIP error
+ * (text via sayaka_dev_last_error).  Nothing here exits or throws.
+ * Threading: one context per host thread; every call is stream-ordered on the context's
+ * stream and returns after the device work it needs has completed.
+ */
+#ifndef SAYAKA_AMD_H
+#define SAYAKA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sayaka_dev_ctx sayaka_dev_ctx;
+
+enum sayaka_field {
+    SAYAKA_RHS = 0, SAYAKA_SOL, SAYAKA_RESID, SAYAKA_CORR, SAYAKA_TMP, SAYAKA_X0, SAYAKA_X,
+    SAYAKA_R, SAYAKA_RHAT, SAYAKA_P, SAYAKA_PHAT, SAYAKA_S, SAYAKA_SHAT, SAYAKA_V, SAYAKA_T,
+    SAYAKA_NFIELD
+};
+
+enum sayaka_precision {
+    SAYAKA_FP64 = 0,   /* every kernel in FP64, V-cycle exactly as the reference */
+    SAYAKA_MIXED = 1   /* FP32 V(2,2)-cycle inside the FP64 BiCGSTAB, FP64 coarsest solve */
+};
+
+typedef struct {
+    double tol_rel;    /* MacSolver::setTolerance (SayakaMacSolver.h:350) */
+    double tol_abs;
+    int max_iter;      /* MacSolver::setMaxIteration (SayakaMacSolver.h:349) */
+    int precond;       /* MacSolver::setCGUseMGPrecond (SayakaMacSolver.h:358) */
+    int precision;     /* sayaka_precision */
+    int pre_smooth;    /* mixed mode only (the reference V-cycle has none) */
+    int post_smooth;   /* MacSolver::setPostSmoothNum (SayakaMacSolver.h:353), default 2 */
+    int hist_cap;      /* capacity of stats->hist */
+} sayaka_solve_params;
+
+typedef struct {
+    int status;        /* 0 converged, 1 breakdown, 2 max_iter reached */
+    int iters;         /* outer BiCGSTAB iterations (the reference's "iter=%d" count) */
+    int half_step;     /* 1 when convergence was detected on |s| */
+    int bottom_calls;
+    long long bottom_iters;
+    double rnorm0, rnorm;  /* Linf of initial / final recursive residual */
+    int nhist;
+    double* hist;      /* caller-owned, hist_cap entries: -|s| (half) and +|r| (full) */
+} sayaka_solve_stats;
+
+const char* sayaka_dev_last_error(void);
+
+/* MacSolver(tree) + MacSolver::define (SayakaMacSolver.h:56-87, SayakaMacOperator.cpp:13-60):
+ * device buffers for up to max_blocks blocks of nb^ndim cells. */
+int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_ctx** out);
+int sayaka_dev_destroy(sayaka_dev_ctx* ctx);
+
+/* MacSolver::prepareSolver (SayakaMacSolver.cpp:12-26): registerMGLevels from the tree
+ * (SayakaMacOperator.cpp:62-78, SayakaTower.cpp:28-187) and unit coefficients.  The tree is
+ * passed as its per-node arrays (AmrTreeNode, SayakaTree.h:52-82). */
+int sayaka_dev_set_topology(sayaka_dev_ctx* ctx, int nblk, int minlevel, int maxlevel, const int* level,
+                            const int* parent, const int* which_child, const int* child8, const int* nbr6,
+                            const int* node_type, const double* length3);
+
+/* MacSolver::setBC (SayakaMacOperator.cpp:1462-1469): per block face math BC type
+ * (0 Neumann, 1 Dirichlet, 2 SimpleFill) on wall faces, -1 elsewhere; [nblk][6]. */
+int sayaka_dev_set_bc(sayaka_dev_ctx* ctx, const int* bc_type6);
+
+/* MacSolver::setScalars + setIBFracByLS/fractions + setCoefB (SayakaMacSolver.h:346,
+ * SayakaMacIB.cpp:34-97, SayakaMacCoef.cpp:319-367): leaf values of the volume fraction,
+ * area fractions and B per direction (NULL = unit); the device synchronises coarse/fine
+ * faces, averages down to parent blocks and precomputes the face weights. */
+int sayaka_dev_set_coefs(sayaka_dev_ctx* ctx, double alpha, double beta, const double* vf,
+                         const double* const* af, const double* const* b);
+/* read back the solver's coefficient state (tests): any pointer may be NULL;
+ * w is [nblk][ndim][faces], wcf [nblk][2*ndim][nb^(ndim-1)] */
+int sayaka_dev_get_coefs(sayaka_dev_ctx* ctx, double* vf, double* const* af, double* const* b, double* w,
+                         double* wcf);
+
+/* MacSolver::setRhs / setInitSol / getSol (SayakaMacSolver.h:117-132) on any field id */
+int sayaka_dev_upload(sayaka_dev_ctx* ctx, int field, const double* host_valid);
+int sayaka_dev_download(sayaka_dev_ctx* ctx, int field, double* host_valid);
+
+/* MacSolver::fixBC (SayakaMacOperator.cpp:1470-1594): phi carries BC values in ngrow ghost
+ * layers ([block] over the grown box); adds the wall corrections to field rhs_field. */
+int sayaka_dev_fix_bc(sayaka_dev_ctx* ctx, const double* phi_grown, int ngrow, int rhs_field);
+
+/* Operators on MG tower m (tests and fine-grained callers) */
+int sayaka_dev_apply(sayaka_dev_ctx* ctx, int m, int out, int in);               /* mg_apply_level */
+int sayaka_dev_residual(sayaka_dev_ctx* ctx, int m, int r, int x, int rhs);      /* mg_residual_level */
+int sayaka_dev_smooth(sayaka_dev_ctx* ctx, int m, int sol, int rhs);             /* mg_smooth_level */
+int sayaka_dev_restrict(sayaka_dev_ctx* ctx, int crse_level, int field);         /* mg_restict_wgt */
+int sayaka_dev_prolong(sayaka_dev_ctx* ctx, int fine_level, int field);          /* mg_prolong_wgt */
+int sayaka_dev_dot(sayaka_dev_ctx* ctx, int m, int a, int b, double* dot, double* maxabs_a);
+                                                     /* mg_dotprod_level, mg_norm_level */
+int sayaka_dev_precond(sayaka_dev_ctx* ctx, int m, int sol, int rhs);            /* mg_precond_level */
+int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, double tol_abs, int max_iter,
+                      int* iters, int* status);      /* mg_solve_bottom */
+
+/* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
+ * the SOL field as initial guess; SOL receives the solution on convergence. */
+int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);
+
+/* Set every value of a field to zero on the device (the reference's setZeroSol,
+ * SayakaMacSolver.h:125-128, without a host copy). */
+int sayaka_dev_fill(sayaka_dev_ctx* ctx, int field, double value);
+
+/* Launch timing per kernel class, with HIP events on the context stream (no reference
+ * counterpart; used by the benchmark).  enable != 0 clears and starts recording. */
+enum sayaka_kernel_class {
+    SAYAKA_KCLASS_APPLY = 0,     /* FP64 operator / residual stencils (fused dot products) */
+    SAYAKA_KCLASS_SMOOTH,        /* FP64 red-black Gauss-Seidel colour sweeps */
+    SAYAKA_KCLASS_SMOOTH32,      /* FP32 red-black sweeps of the mixed V-cycle */
+    SAYAKA_KCLASS_BOTTOM,        /* coarsest-level BiCGSTAB (one cooperative launch) */
+    SAYAKA_KCLASS_VEC,           /* fused vector updates and reductions */
+    SAYAKA_KCLASS_TRANSFER,      /* restriction / prolongation */
+    SAYAKA_KCLASS_N
+};
+int sayaka_dev_profile(sayaka_dev_ctx* ctx, int kclass, int enable);
+int sayaka_dev_profile_read(sayaka_dev_ctx* ctx, int kclass, double* total_ms, long long* launches,
+                            long long* cells);
+
+/* Block until the context's stream is idle. */
+int sayaka_dev_sync(sayaka_dev_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

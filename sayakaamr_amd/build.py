@@ -1,0 +1,66 @@
+"""In-tree build of the native libraries (gfx950).
+
+  libsayaka_dev.so   hipcc, every .hip kernel for gfx950, FP64 parity kernels with
+                     -ffp-contract=off (no FMA contraction), packed-FP32 codegen disabled
+  libsayaka_host.so  g++, host tree / tables / solver driver (C ABI include/sayaka_host.h),
+                     linked against libsayaka_dev.so
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+DEV_SOURCES = ["dev/sayaka_dev.hip"]
+DEV_HEADERS = ["dev/dev_common.h", "dev/kernels.h", "dev/bottom.h", "dev/coef.h", "dev/mixed.h"]
+HOST_SOURCES = ["host/amr_tree.cpp", "host/capi_tree.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build_dev(verbose=False) -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsayaka_dev.so")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", f"-I{INC}",
+           *[os.path.join(CSRC, s) for s in DEV_SOURCES], "-o", out + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_host(verbose=False) -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsayaka_host.so")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}",
+           f"-I{os.path.join(CSRC, 'host')}", *[os.path.join(CSRC, s) for s in HOST_SOURCES],
+           "-o", out + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(verbose=False):
+    return [build_dev(verbose), build_host(verbose)]
+
+
+if __name__ == "__main__":
+    for p in build_all(verbose=True):
+        print("built", p)

@@ -1,0 +1,583 @@
+// kernels.h -- HIP kernels of the MAC solve, templated on dimension D and block edge NB.
+//
+// Arithmetic is written to reproduce the reference operation order (files cited per
+// kernel) so that, built with -ffp-contract=off, the FP64 kernels are bit-identical to
+// the CPU oracle; only reductions (dot products) sum in a different, fixed order.
+#pragma once
+
+#include "dev_common.h"
+
+namespace sydev {
+
+// ------------------------------------------------------------------ reductions (fixed order)
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide reduction of up to three values (two sums, one max), deterministic for a fixed
+// block size.  Result valid in thread 0.
+template <int NT>
+__device__ inline void block_reduce3(double& s0, double& s1, double& mx, double* lds) {
+    constexpr int NW = (NT + 63) / 64;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    mx = wave_max(mx);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wv] = s0;
+        lds[NW + wv] = s1;
+        lds[2 * NW + wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0, m = 0.0;
+        for (int w = 0; w < NW; w++) {
+            a += lds[w];
+            b += lds[NW + w];
+            m = fmax(m, lds[2 * NW + w]);
+        }
+        s0 = a;
+        s1 = b;
+        mx = m;
+    }
+    __syncthreads();
+}
+
+// Scalars of the Krylov loops kept on the device.
+enum Scalar {
+    S_RHO = 0, S_RHO1, S_ALPHA, S_OMEGA, S_ALPHA_DEN, S_TS, S_TT, S_SNORM, S_RNORM, S_RNORM0,
+    S_DOT, S_NORM, S_AUX0, S_AUX1, S_NSCALAR
+};
+
+// What the last workgroup computes from the global sums.
+enum Finalize {
+    FIN_NONE = 0,
+    FIN_INIT,       // rnorm0 = max, rho = sum0
+    FIN_ALPHA,      // alpha_den = sum0, alpha = rho / alpha_den
+    FIN_SNORM,      // snorm = max
+    FIN_OMEGA,      // ts = sum0, tt = sum1, omega = ts / tt
+    FIN_RUPDATE,    // rnorm = max, rho1 = rho, rho = sum0
+    FIN_DOT,        // dot = sum0
+    FIN_NORM,       // norm = max
+    FIN_DOTNORM     // dot = sum0, aux0 = sum1, norm = max
+};
+
+struct RedBuf {
+    double* partials;   // 3 per workgroup
+    unsigned* counter;  // zero between launches
+    double* scal;       // S_NSCALAR doubles
+    int exact;          // 1: sums in the reference's order (cells k,j,i in a block, then
+                        //    blocks in list order: DotProd / mg_dotprod_level,
+                        //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
+};
+
+// Sum, in cell order, of per-cell values held by the threads (cell c = tid + q*NT);
+// `scratch` is an LDS buffer of at least NCELL doubles.  Result valid in thread 0.
+template <int NT, int NCELL, int CPT>
+__device__ inline double seq_block_sum(const double (&v)[CPT], double* scratch) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = threadIdx.x + q * NT;
+        if (c < NCELL) scratch[c] = v[q];
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < NCELL; c += 8) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = c + u < NCELL ? scratch[c + u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (c + u < NCELL) s += t[u];
+        }
+    }
+    __syncthreads();
+    return s;
+}
+
+// Coherent load of a value another workgroup published before a device-scope fence.
+__device__ inline double load_published(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of n published triples (sum, sum, max) into every thread's (a, b, m): exact mode sums
+// in index order (thread 0 after staging chunks in LDS), fast mode by a strided tree.
+template <int NT>
+__device__ inline void sum_published(const double* part, unsigned n, bool exact, double& a, double& b, double& m,
+                                     double* chunk /* 3*NT doubles of LDS */, double* lds) {
+    a = 0.0;
+    b = 0.0;
+    m = 0.0;
+    if (exact) {
+        for (unsigned base = 0; base < n; base += NT) {
+            const unsigned g = base + threadIdx.x;
+            if (g < n) {
+                chunk[3 * threadIdx.x] = load_published(part + 3 * (size_t)g);
+                chunk[3 * threadIdx.x + 1] = load_published(part + 3 * (size_t)g + 1);
+                chunk[3 * threadIdx.x + 2] = load_published(part + 3 * (size_t)g + 2);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const unsigned cnt = n - base < (unsigned)NT ? n - base : (unsigned)NT;
+                for (unsigned q = 0; q < cnt; q++) {
+                    a += chunk[3 * q];
+                    b += chunk[3 * q + 1];
+                    m = fmax(m, chunk[3 * q + 2]);
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        for (unsigned g = threadIdx.x; g < n; g += NT) {
+            a += load_published(part + 3 * (size_t)g);
+            b += load_published(part + 3 * (size_t)g + 1);
+            m = fmax(m, load_published(part + 3 * (size_t)g + 2));
+        }
+        block_reduce3<NT>(a, b, m, lds);
+    }
+}
+
+// Per-workgroup totals (valid in thread 0) to the global result: in the last workgroup to
+// finish, sum the partials (in list order when exact) and update the Krylov scalars.
+template <int NT>
+__device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf& rb, int fin, double* lds) {
+    __shared__ int is_last;
+    __shared__ double chunk[3 * NT];
+    if (threadIdx.x == 0) {
+        double* p = rb.partials + 3 * (size_t)blockIdx.x;
+        p[0] = s0;
+        p[1] = s1;
+        p[2] = mx;
+        __threadfence();
+        const unsigned t = atomicAdd(rb.counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double a, b, m;
+    sum_published<NT>(rb.partials, gridDim.x, rb.exact != 0, a, b, m, chunk, lds);
+    if (threadIdx.x == 0) {
+        double* S = rb.scal;
+        switch (fin) {
+            case FIN_INIT: S[S_RNORM0] = m; S[S_RHO] = a; break;
+            case FIN_ALPHA: S[S_ALPHA_DEN] = a; S[S_ALPHA] = a != 0.0 ? S[S_RHO] / a : 0.0; break;
+            case FIN_SNORM: S[S_SNORM] = m; break;
+            case FIN_OMEGA: S[S_TS] = a; S[S_TT] = b; S[S_OMEGA] = b != 0.0 ? a / b : 0.0; break;
+            case FIN_RUPDATE: S[S_RNORM] = m; S[S_RHO1] = S[S_RHO]; S[S_RHO] = a; break;
+            case FIN_DOT: S[S_DOT] = a; break;
+            case FIN_NORM: S[S_NORM] = m; break;
+            case FIN_DOTNORM: S[S_DOT] = a; S[S_AUX0] = b; S[S_NORM] = m; break;
+            default: break;
+        }
+        *rb.counter = 0u;
+        __threadfence();
+    }
+}
+
+// Block totals of two per-cell sums (q0, q1) and a max, then grid_finish.
+template <int NT, int NCELL, int CPT>
+__device__ inline void grid_reduce_cells(const double (&q0)[CPT], const double (&q1)[CPT], double mx, int nsum,
+                                         const RedBuf& rb, int fin, double* scratch, double* lds) {
+    const bool two = nsum > 1;
+    double s0 = 0.0, s1 = 0.0;
+    if (rb.exact) {
+        if (nsum > 0) s0 = seq_block_sum<NT, NCELL, CPT>(q0, scratch);
+        if (two) s1 = seq_block_sum<NT, NCELL, CPT>(q1, scratch);
+        double z0 = 0.0, z1 = 0.0;
+        block_reduce3<NT>(z0, z1, mx, lds);
+    } else {
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            s0 += q0[q];
+            s1 += q1[q];
+        }
+        block_reduce3<NT>(s0, s1, mx, lds);
+    }
+    grid_finish<NT>(s0, s1, mx, rb, fin, lds);
+}
+
+// ---------------------------------------------------------------------------- stencil core
+template <class T>
+struct StencilArgsT {
+    const int* list;          // block ids processed by this launch
+    int nlist;
+    int m;                    // MG tower level (face types depend on it)
+    const BlockInfo* binfo;
+    const BlockMetric* met;
+    const T* w;               // own face weights [blk][D][NFACE]
+    const T* wcf;             // coarse-side coarse/fine weights [blk][NF][NSLAB]
+    const T* vf;              // [blk][NCELL]
+    const T* x;               // input field (smoother: updated in place)
+    T* out;                   // apply / residual output
+    const T* rhs;             // residual / smoother right-hand side
+    const T* dotv;            // second operand of the fused dot (may be null)
+    T* out2;                  // second output (residual init: rhat)
+    T* out3;                  // third output (residual init: x zeroed)
+    int colour;
+    T alpha;                  // MAC alpha (0 in every configuration, SayakaMacOperator.cpp:1268)
+};
+using StencilArgs = StencilArgsT<double>;
+
+enum StencilMode { ST_APPLY = 0, ST_RESID = 1, ST_SMOOTH = 2, ST_APPLY_DOT = 3, ST_APPLY_DOT2 = 4, ST_RESID_INIT = 5 };
+
+template <int D, int NB>
+__device__ inline int face_type(const BlockInfo& bi, int f, int m) {
+    const int n = bi.nbr[f];
+    if (n <= PHYS_BNDRY) return FT_BC;
+    if (n < 0) return FT_FC;
+    if (bi.level < m && !bi.nbr_leaf[f]) return FT_CF;
+    return FT_FF;
+}
+
+// Stage block b of field x and its one-cell face halo into LDS.  The halo value of a face
+// whose neighbour is coarser is the coarse leaf's cell that the reference's injection from
+// the parent's ghost resolves to (SayakaFillPatch.cpp:192-299, SayakaTreeData.h:41-44).
+template <int D, int NB, class T>
+__device__ inline void load_tile(T* tile, const T* __restrict__ x, const BlockInfo& bi, int b) {
+    using G = Geo<D, NB>;
+    for (int c = threadIdx.x; c < G::NCELL; c += blockDim.x) {
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        tile[G::tidx(i, j, k)] = x[(size_t)b * G::NCELL + c];
+    }
+    for (int g = threadIdx.x; g < G::NGHOST; g += blockDim.x) {
+        const int f = g / G::NSLAB, p = g % G::NSLAB;
+        const int d = f >> 1, s = f & 1;
+        int a1, a2;
+        tang_axes<D>(d, a1, a2);
+        const int ext = d == 2 ? G::NBZ : NB;
+        int q[3] = {0, 0, 0};
+        q[a1] = p % NB;
+        if (D == 3) q[a2] = p / NB;
+        int gq[3] = {q[0], q[1], q[2]};
+        gq[d] = s ? ext : -1;
+        const int n = bi.nbr[f];
+        T v = T(0);
+        if (n >= 0) {
+            q[d] = s ? 0 : ext - 1;
+            v = x[(size_t)n * G::NCELL + G::cidx(q[0], q[1], q[2])];
+        } else if (n > PHYS_BNDRY) {
+            const int cb = bi.cnbr[f];
+            const int wc = bi.which_child;
+            int r[3] = {0, 0, 0};
+            r[d] = s ? 0 : ext - 1;
+            r[a1] = (q[a1] >> 1) + side_of(wc, a1) * (NB / 2);
+            if (D == 3) r[a2] = (q[a2] >> 1) + side_of(wc, a2) * (NB / 2);
+            v = x[(size_t)cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
+        }
+        tile[G::tidx(gq[0], gq[1], gq[2])] = v;
+    }
+}
+
+// Sum of the fine-side contributions w_f * phi_f over the 2^(d-1) fine faces that cover
+// one coarse face cell, in the order of restrictFaceBlockData_Average
+// (SayakaInterpPatch.cpp:659-687): higher tangential axis outer.  These are the values
+// the reference saves on FINE_CRSE faces (SayakaMacOperator.cpp:1446-1455, 1310-1319) and
+// sums into the coarse block's CRSE_FINE face (SayakaTower.cpp:360-413).
+template <int D, int NB, class T>
+__device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* __restrict__ binfo,
+                           const T* __restrict__ x, const T* __restrict__ w) {
+    using G = Geo<D, NB>;
+    const int d = f >> 1, s = f & 1;
+    int a1, a2;
+    tang_axes<D>(d, a1, a2);
+    const int half = NB / 2;
+    const int s1 = t1 >= half ? 1 : 0;
+    const int s2 = (D == 3 && t2 >= half) ? 1 : 0;
+    int ch = (1 - s) << d;
+    ch |= s1 << a1;
+    if (D == 3) ch |= s2 << a2;
+    const int fb = binfo[nbrblk].child[ch];
+    const int ext = d == 2 ? G::NBZ : NB;
+    T acc = T(0);
+    for (int q2 = 0; q2 <= (D == 3 ? 1 : 0); q2++) {
+        for (int q1 = 0; q1 <= 1; q1++) {
+            int c[3] = {0, 0, 0}, fq[3] = {0, 0, 0};
+            c[d] = s ? 0 : ext - 1;
+            fq[d] = s ? 0 : ext;
+            c[a1] = fq[a1] = (t1 - s1 * half) * 2 + q1;
+            if (D == 3) c[a2] = fq[a2] = (t2 - s2 * half) * 2 + q2;
+            const T wv = w[((size_t)fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
+            const T ph = x[(size_t)fb * G::NCELL + G::cidx(c[0], c[1], c[2])];
+            acc += wv * ph;
+        }
+    }
+    return acc;
+}
+
+// One cell of the operator (MODE apply) or of a Gauss-Seidel update (MODE smooth):
+// mg_apply_block_usefill (SayakaMacOperator.cpp:1372-1458) and
+// mg_smooth_block_rbcycle_usefill (SayakaMacOperator.cpp:1232-1320) with the face
+// coefficients of cell_eval_coef (1115-1167) already folded into w / wcf.
+template <int D, int NB, bool SMOOTH, class T>
+__device__ inline T cell_op(const T* tile, int b, int i, int j, int k, const BlockInfo& bi, int m,
+                            const StencilArgsT<T>& a, T dv, T vfv, T rhsv) {
+    using G = Geo<D, NB>;
+    T diag = a.alpha * dv;
+    T offdiag = T(0);
+    const T phic = tile[G::tidx(i, j, k)];
+#pragma unroll
+    for (int f = 0; f < G::NF; f++) {
+        const int d = f >> 1, s = f & 1;
+        const int idx = d == 0 ? i : (d == 1 ? j : k);
+        const int ext = d == 2 ? G::NBZ : NB;
+        const bool bnd = s ? idx == ext - 1 : idx == 0;
+        const int fi = G::fidx(d, i + (f == 1), j + (f == 3), k + (f == 5));
+        const int ni = G::tidx(i + (f == 1) - (f == 0), j + (f == 3) - (f == 2), k + (f == 5) - (f == 4));
+        T off, offval;
+        if (!bnd) {
+            off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+            offval = off * tile[ni];
+        } else {
+            const int ft = face_type<D, NB>(bi, f, m);
+            if (ft == FT_CF) {
+                int a1, a2;
+                tang_axes<D>(d, a1, a2);
+                const int ijk[3] = {i, j, k};
+                const int t1 = ijk[a1], t2 = D == 3 ? ijk[a2] : 0;
+                off = a.wcf[((size_t)b * G::NF + f) * G::NSLAB + t1 + NB * t2];
+                offval = cf_sum<D, NB, T>(f, t1, t2, bi.nbr[f], a.binfo, a.x, a.w);
+            } else if (ft == FT_BC) {
+                if (bi.bct[f] == BC_NEUMANN) {
+                    off = T(0);
+                    offval = T(0);
+                } else {
+                    off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+                    if (SMOOTH) {
+                        offval = T(0);
+                        off *= T(2);
+                    } else {
+                        offval = -off * phic;
+                    }
+                }
+            } else {
+                off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+                offval = off * tile[ni];
+            }
+        }
+        diag += off;
+        offdiag += offval;
+    }
+    if (SMOOTH) {
+        return vfv > 0 ? (rhsv * dv * vfv + offdiag) / diag : T(0);
+    } else {
+        const T Lphi = (phic * diag - offdiag) / dv;
+        return vfv > 0 ? Lphi / vfv : T(0);
+    }
+}
+
+template <int D, int NB, int MODE>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_stencil(StencilArgs a, RedBuf rb, int fin) {
+    using G = Geo<D, NB>;
+    constexpr int NT = G::THREADS;
+    constexpr int CPT = (G::NCELL + NT - 1) / NT;
+    __shared__ double tile[G::NTILE];
+    __shared__ double lds[3 * ((NT + 63) / 64)];
+    __shared__ BlockInfo bi;
+    const int b = a.list[blockIdx.x];
+    if (threadIdx.x == 0) bi = a.binfo[b];
+    __syncthreads();
+    const double dv = a.met[b].dv;
+    load_tile<D, NB, double>(tile, a.x, bi, b);
+    __syncthreads();
+    double q0[CPT], q1[CPT], mx = 0.0;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        q0[q] = q1[q] = 0.0;
+        const int c = threadIdx.x + q * NT;
+        if (c >= G::NCELL) continue;
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const size_t gi = (size_t)b * G::NCELL + c;
+        const double vfv = a.vf[gi];
+        if (MODE == ST_SMOOTH) {
+            if (((i + j + k) & 1) != a.colour) continue;
+            const double u = cell_op<D, NB, true, double>(tile, b, i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
+            const_cast<double*>(a.x)[gi] = u;
+        } else {
+            const double Lp = cell_op<D, NB, false, double>(tile, b, i, j, k, bi, a.m, a, dv, vfv, 0.0);
+            if (MODE == ST_APPLY || MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) {
+                a.out[gi] = Lp;
+                if (MODE == ST_APPLY_DOT) q0[q] = a.dotv[gi] * Lp;   // rhat . v
+                if (MODE == ST_APPLY_DOT2) {
+                    q0[q] = Lp * a.dotv[gi];                          // t . s
+                    q1[q] = Lp * Lp;                                  // t . t
+                }
+            } else {
+                const double r = vfv > 0 ? a.rhs[gi] - Lp : 0.0;
+                a.out[gi] = r;
+                if (MODE == ST_RESID_INIT) {
+                    a.out2[gi] = r;
+                    a.out3[gi] = 0.0;
+                    q0[q] = r * r;                                    // rhat . r with rhat = r
+                    mx = fmax(mx, fabs(r));
+                }
+            }
+        }
+    }
+    if (MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT)
+        grid_reduce_cells<NT, G::NCELL, CPT>(q0, q1, mx, MODE == ST_APPLY_DOT2 ? 2 : 1, rb, fin, tile, lds);
+}
+
+// --------------------------------------------------------------- restriction / prolongation
+// mg_restict_wgt onto the non-leaf blocks of one level (SayakaMacIB.cpp:157-232 with
+// restrictCellBlockData_Average, SayakaInterpPatch.cpp:398-444): coarse =
+// ((sum over the 2^d children, kk,jj,ii order, of x*vf) / 2^d) / vf_coarse, 0 if vf_coarse=0.
+template <int D, int NB>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_restrict(const int* list, const BlockInfo* binfo, const double* vf, double* x) {
+    using G = Geo<D, NB>;
+    const int b = list[blockIdx.x];
+    const double wgt = 1.0 / G::NCHILD;
+    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const int h = NB / 2;
+        const int si = i >= h, sj = j >= h, sk = (D == 3 && k >= h);
+        const int ch = si | (sj << 1) | (sk << 2);
+        const int fb = binfo[b].child[ch];
+        const int fi0 = (i - si * h) * 2, fj0 = (j - sj * h) * 2, fk0 = D == 3 ? (k - sk * h) * 2 : 0;
+        double sum = 0.0;
+        for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
+            for (int jj = 0; jj <= 1; jj++)
+                for (int ii = 0; ii <= 1; ii++) {
+                    const size_t fi = (size_t)fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
+                    sum += x[fi] * vf[fi];
+                }
+        const double v = vf[(size_t)b * G::NCELL + c];
+        const double r = sum * wgt;
+        x[(size_t)b * G::NCELL + c] = v > 0 ? r / v : 0.0;
+    }
+}
+
+// mg_prolong_wgt on every block of one level (SayakaMacIB.cpp:234-283): injection from the
+// parent cell, zero where vf = 0.
+template <int D, int NB>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_prolong(const int* list, const BlockInfo* binfo, const double* vf, double* x) {
+    using G = Geo<D, NB>;
+    const int b = list[blockIdx.x];
+    const int p = binfo[b].parent;
+    const int wc = binfo[b].which_child;
+    const int h = NB / 2;
+    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
+                               D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
+        const size_t gi = (size_t)b * G::NCELL + c;
+        x[gi] = vf[gi] > 0 ? x[(size_t)p * G::NCELL + pc] : 0.0;
+    }
+}
+
+// --------------------------------------------------------------------------- vector kernels
+enum VecOp {
+    V_PINIT = 0,   // p = r ; resid = p masked ; (phat = p if no preconditioner)
+    V_PUPDATE,     // p = r + beta*(p - omega*v) ; resid = p masked ; (phat = p)
+    V_SUPDATE,     // s = r - alpha*v ; x = x + alpha*phat ; resid = s masked ; |s| ; (shat = s)
+    V_RUPDATE,     // x = x + omega*shat ; r = s - omega*t ; |r| ; rhat.r
+    V_FINAL,       // sol = sol + x
+    V_COPYPLUS0,   // dst = 0.0 + src   (SOL = 0 + CORR, then phat = SOL)
+    V_MASKCOPY,    // dst = vf>0 ? src : 0
+    V_ZERO,        // dst = 0
+    V_DOT,         // dot(a, b), max|a|
+};
+
+struct VecArgs {
+    const int* list;
+    const double* vf;
+    const double* r;
+    double* p;
+    const double* v;
+    double* resid;
+    double* phat;     // when non-null and no preconditioner, receives a copy
+    double* s;
+    double* x;
+    const double* t;
+    const double* rhat;
+    const double* src;
+    double* dst;
+    const double* scal;
+};
+
+template <int NCELL, int NT>
+__global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fin) {
+    constexpr int CPT = (NCELL + NT - 1) / NT;
+    __shared__ double lds[3 * ((NT + 63) / 64)];
+    __shared__ double scratch[NCELL];
+    const int b = a.list[blockIdx.x];
+    double q0[CPT], q1[CPT], mx = 0.0;
+    const double* S = a.scal;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        q0[q] = q1[q] = 0.0;
+        const int c = threadIdx.x + q * NT;
+        if (c >= NCELL) continue;
+        const size_t gi = (size_t)b * NCELL + c;
+        switch (op) {
+            case V_PINIT: {
+                const double pv = a.r[gi];
+                a.p[gi] = pv;
+                a.resid[gi] = a.vf[gi] > 0 ? pv : 0.0;
+                if (a.phat) a.phat[gi] = pv;
+                break;
+            }
+            case V_PUPDATE: {
+                // beta = (rho/rho1)*(alpha/omega); p = p - omega*v; p = r + beta*p
+                // (SayakaMacSolver.cpp:782-790, Calc_axpby SayakaBlockData.cpp:253-255)
+                const double beta = (S[S_RHO] / S[S_RHO1]) * (S[S_ALPHA] / S[S_OMEGA]);
+                const double p1 = 1.0 * a.p[gi] + -S[S_OMEGA] * a.v[gi];
+                const double pv = 1.0 * a.r[gi] + beta * p1;
+                a.p[gi] = pv;
+                a.resid[gi] = a.vf[gi] > 0 ? pv : 0.0;
+                if (a.phat) a.phat[gi] = pv;
+                break;
+            }
+            case V_SUPDATE: {
+                const double al = S[S_ALPHA];
+                const double sv = 1.0 * a.r[gi] + -al * a.v[gi];
+                a.s[gi] = sv;
+                a.x[gi] = 1.0 * a.x[gi] + al * a.phat[gi];
+                if (a.resid) a.resid[gi] = a.vf[gi] > 0 ? sv : 0.0;
+                if (a.dst) a.dst[gi] = sv;
+                mx = fmax(mx, fabs(sv));
+                break;
+            }
+            case V_RUPDATE: {
+                const double om = S[S_OMEGA];
+                a.x[gi] = 1.0 * a.x[gi] + om * a.src[gi];
+                const double rv = 1.0 * a.s[gi] + -om * a.t[gi];
+                const_cast<double*>(a.r)[gi] = rv;
+                mx = fmax(mx, fabs(rv));
+                q0[q] = a.rhat[gi] * rv;
+                break;
+            }
+            case V_FINAL: a.dst[gi] = a.dst[gi] + a.x[gi]; break;
+            case V_COPYPLUS0: a.dst[gi] = 0.0 + a.src[gi]; break;
+            case V_MASKCOPY: a.dst[gi] = a.vf[gi] > 0 ? a.src[gi] : 0.0; break;
+            case V_ZERO: a.dst[gi] = 0.0; break;
+            case V_DOT: {
+                const double u = a.src[gi];
+                q0[q] = u * a.t[gi];
+                mx = fmax(mx, fabs(u));
+                break;
+            }
+            default: break;
+        }
+    }
+    if (fin != FIN_NONE)
+        grid_reduce_cells<NT, NCELL, CPT>(q0, q1, mx, (op == V_RUPDATE || op == V_DOT) ? 1 : 0, rb, fin, scratch, lds);
+}
+
+}  // namespace sydev

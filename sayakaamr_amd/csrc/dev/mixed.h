@@ -1,0 +1,142 @@
+// mixed.h -- FP32 multigrid preconditioner inside the FP64 BiCGSTAB (no reference
+// counterpart: the reference runs everything in FP64 with post-smoothing only).
+//
+// V(l, f_l) approximately solves A_l u_l = f_l on tower T(l) = leaves of levels < l plus
+// every block of level l (the reference's MG level, SayakaTower.h:121-129):
+//   l == 1 : FP64 BiCGSTAB on the level-1 blocks (the reference's bottom solve);
+//   else   : u_l = 0; nu1 red-black sweeps; r = f_l - A_l u_l;
+//            f_{l-1} = r on the leaves shared with T(l-1), and the volume-fraction weighted
+//            average of r from the children on the non-leaf blocks of level l-1;
+//            u_{l-1} = V(l-1, f_{l-1});
+//            u_l += u_{l-1} on the shared leaves, += injection of u_{l-1} (zero where vf=0)
+//            on the blocks of level l; nu2 red-black sweeps.
+// Sweeps, residual, transfer and accumulation are FP32; the operator weights are the FP64
+// weights rounded once to FP32.  Each level keeps its own u and f (the towers share their
+// coarse leaves, so one in-place array as in the reference cannot hold both corrections).
+#pragma once
+
+#include "bottom.h"
+#include "kernels.h"
+
+struct sayaka_dev_ctx;
+
+namespace sydev {
+
+constexpr int MIX_MAXLV = 16;
+
+struct MixedState {
+    int nblk = 0, ncell = 0, coef_version = -1, nlev = 0;
+    size_t cap_cells = 0, cap_faces = 0, cap_cf = 0;
+    float* w32 = nullptr;
+    float* wcf32 = nullptr;
+    float* vf32 = nullptr;
+    float* r32 = nullptr;
+    float* u[MIX_MAXLV + 1] = {};
+    float* f[MIX_MAXLV + 1] = {};
+    double bottom_tol_rel = 1.0e-6, bottom_tol_abs = 1.0e-6;
+    int bottom_max_iter = 2000;
+};
+
+inline void mixed_free(MixedState& ms) {
+    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32};
+    for (float* p : ptrs) if (p) (void)hipFree(p);
+    for (int l = 0; l <= MIX_MAXLV; l++) {
+        if (ms.u[l]) (void)hipFree(ms.u[l]);
+        if (ms.f[l]) (void)hipFree(ms.f[l]);
+        ms.u[l] = ms.f[l] = nullptr;
+    }
+    ms.w32 = ms.wcf32 = ms.vf32 = ms.r32 = nullptr;
+    ms.cap_cells = ms.cap_faces = ms.cap_cf = 0;
+    ms.coef_version = -1;
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = (float)src[q];
+}
+
+enum MixOp { M_ZERO = 0, M_COPY, M_ADD, M_PROLONG_ADD, M_F64_TO_F32, M_F32_TO_F64 };
+
+template <int D, int NB>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_mix(const int* list, int op, const BlockInfo* __restrict__ binfo, const float* __restrict__ vf,
+      float* dst, const float* src, double* dst64, const double* src64) {
+    using G = Geo<D, NB>;
+    const int b = list[blockIdx.x];
+    const int h = NB / 2;
+    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
+        const size_t gi = (size_t)b * G::NCELL + c;
+        switch (op) {
+            case M_ZERO: dst[gi] = 0.0f; break;
+            case M_COPY: dst[gi] = src[gi]; break;
+            case M_ADD: dst[gi] += src[gi]; break;
+            case M_PROLONG_ADD: {
+                int i, j, k;
+                G::cell_ijk(c, i, j, k);
+                const int p = binfo[b].parent, wc = binfo[b].which_child;
+                const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
+                                       D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
+                if (vf[gi] > 0.0f) dst[gi] += src[(size_t)p * G::NCELL + pc];
+                break;
+            }
+            case M_F64_TO_F32: dst[gi] = (float)src64[gi]; break;
+            case M_F32_TO_F64: dst64[gi] = (double)src[gi]; break;
+            default: break;
+        }
+    }
+}
+
+// FP32 sweep (SMOOTH) or residual r = f - A u (RESID) over one block list.
+template <int D, int NB, int MODE>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_stencil32(StencilArgsT<float> a) {
+    using G = Geo<D, NB>;
+    __shared__ float tile[G::NTILE];
+    __shared__ BlockInfo bi;
+    const int b = a.list[blockIdx.x];
+    if (threadIdx.x == 0) bi = a.binfo[b];
+    __syncthreads();
+    const float dv = (float)a.met[b].dv;
+    load_tile<D, NB, float>(tile, a.x, bi, b);
+    __syncthreads();
+    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const size_t gi = (size_t)b * G::NCELL + c;
+        const float vfv = a.vf[gi];
+        if (MODE == ST_SMOOTH) {
+            if (((i + j + k) & 1) != a.colour) continue;
+            const_cast<float*>(a.x)[gi] = cell_op<D, NB, true, float>(tile, b, i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
+        } else {
+            const float Lp = cell_op<D, NB, false, float>(tile, b, i, j, k, bi, a.m, a, dv, vfv, 0.0f);
+            a.out[gi] = vfv > 0.0f ? a.rhs[gi] - Lp : 0.0f;
+        }
+    }
+}
+
+// vf-weighted average of the children's residual onto the non-leaf blocks of one level.
+template <int D, int NB>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_restrict32(const int* list, const BlockInfo* binfo, const float* vf, const float* src, float* dst) {
+    using G = Geo<D, NB>;
+    const int b = list[blockIdx.x];
+    const float wgt = 1.0f / G::NCHILD;
+    const int h = NB / 2;
+    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const int si = i >= h, sj = j >= h, sk = (D == 3 && k >= h);
+        const int fb = binfo[b].child[si | (sj << 1) | (sk << 2)];
+        const int fi0 = (i - si * h) * 2, fj0 = (j - sj * h) * 2, fk0 = D == 3 ? (k - sk * h) * 2 : 0;
+        float sum = 0.0f;
+        for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
+            for (int jj = 0; jj <= 1; jj++)
+                for (int ii = 0; ii <= 1; ii++) {
+                    const size_t fi = (size_t)fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
+                    sum += src[fi] * vf[fi];
+                }
+        const float v = vf[(size_t)b * G::NCELL + c];
+        dst[(size_t)b * G::NCELL + c] = v > 0.0f ? (sum * wgt) / v : 0.0f;
+    }
+}
+
+}  // namespace sydev

@@ -1,0 +1,1026 @@
+// sayaka_dev.hip -- device context, index tables and the C ABI of include/sayaka_amd.h.
+//
+// The host keeps the tree; after each regrid it hands the per-node arrays to
+// sayaka_dev_set_topology, which builds the per-block index table (BlockInfo), the block
+// metric and the block lists every launch iterates over:
+//   level l, all blocks / leaves / non-leaves (ascending id, SayakaRegrid.cpp:962-976);
+//   tower T(m) = leaves of levels < m and every block of level m, ordered m..1
+//   (MGLevelTower::update, SayakaTower.cpp:28-68), with one sub-list per level.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/sayaka_amd.h"
+#include "bottom.h"
+#include "coef.h"
+#include "kernels.h"
+#include "mixed.h"
+
+using namespace sydev;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(-3, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int MAXLV = 16;
+
+struct ListRef {
+    int off = 0, n = 0;
+};
+
+}  // namespace
+
+struct sayaka_dev_ctx {
+    int device = 0, ndim = 3, nb = 8, cap = 0;
+    int ncell = 0, nface = 0, nslab = 0, nf = 0;
+    int nblk = 0, minlevel = 1, maxlevel = 1, finest = 1;
+    hipStream_t stream = nullptr;
+    double alpha = 0.0, beta = 1.0;
+
+    std::vector<BlockInfo> h_binfo;
+    std::vector<BlockMetric> h_met;
+    BlockInfo* d_binfo = nullptr;
+    BlockMetric* d_met = nullptr;
+
+    std::vector<int> h_lists;
+    int* d_lists = nullptr;
+    ListRef lv_all[MAXLV + 1], lv_leaf[MAXLV + 1], lv_nonleaf[MAXLV + 1];
+    ListRef tw[MAXLV + 1];                    // whole tower, sub-levels m..1
+    ListRef tw_sub[MAXLV + 1][MAXLV + 1];     // [m][l]
+    ListRef all_blocks;
+
+    double* d_field[SAYAKA_NFIELD] = {};
+    double* d_vf = nullptr;
+    double* d_af = nullptr;   // [blk][D][NFACE]
+    double* d_b = nullptr;
+    double* d_w = nullptr;
+    double* d_wcf = nullptr;
+
+    double* d_partials = nullptr;
+    unsigned* d_counter = nullptr;
+    double* d_scal = nullptr;
+    double* h_scal = nullptr;  // pinned
+    double* d_bot_pg = nullptr;
+    double* d_bot_sg = nullptr;
+    double* d_bot_part = nullptr;
+    int* d_bot_stats = nullptr;
+    int* h_bot_stats = nullptr;  // pinned
+
+    MixedState mixed;
+    // per-kernel-class launch timing (HIP events on the context stream)
+    struct ProfRec {
+        hipEvent_t a, b;
+        long long cells;
+    };
+    bool prof_on[SAYAKA_KCLASS_N] = {};
+    std::vector<ProfRec> prof[SAYAKA_KCLASS_N];
+    std::vector<hipEvent_t> ev_pool;
+    int coef_version = 0;
+    int exact = 1;  // reference-order reductions (FP64 solves); the mixed solve clears it
+};
+
+namespace {
+
+hipEvent_t pool_event(sayaka_dev_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch of kernel class k with events when timing of that class is on.
+struct ProfScope {
+    sayaka_dev_ctx* c;
+    int k;
+    long long cells;
+    hipEvent_t a = nullptr;
+    ProfScope(sayaka_dev_ctx* c_, int k_, long long cells_) : c(c_), k(k_), cells(cells_) {
+        if (c->prof_on[k]) {
+            a = pool_event(c);
+            if (a) hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = pool_event(c);
+        if (!b) return;
+        hipEventRecord(b, c->stream);
+        c->prof[k].push_back({a, b, cells});
+    }
+};
+
+RedBuf redbuf(sayaka_dev_ctx* c) { return RedBuf{c->d_partials, c->d_counter, c->d_scal, c->exact}; }
+
+const int* dlist(sayaka_dev_ctx* c, ListRef r) { return c->d_lists + r.off; }
+
+StencilArgs stencil_args(sayaka_dev_ctx* c, ListRef lst, int m) {
+    StencilArgs a{};
+    a.list = dlist(c, lst);
+    a.nlist = lst.n;
+    a.m = m;
+    a.binfo = c->d_binfo;
+    a.met = c->d_met;
+    a.w = c->d_w;
+    a.wcf = c->d_wcf;
+    a.vf = c->d_vf;
+    a.alpha = c->alpha;
+    return a;
+}
+
+template <int D, int NB>
+int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
+    using G = Geo<D, NB>;
+    if (a.nlist <= 0) return 0;
+    dim3 grid(a.nlist), block(G::THREADS);
+    RedBuf rb = redbuf(c);
+    ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL);
+    switch (mode) {
+        case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_SMOOTH: hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_RESID_INIT: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID_INIT>), grid, block, 0, c->stream, a, rb, fin); break;
+        default: return fail(-2, "bad stencil mode");
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int D, int NB>
+int launch_vec(sayaka_dev_ctx* c, ListRef lst, int op, const VecArgs& a, int fin) {
+    using G = Geo<D, NB>;
+    if (lst.n <= 0) return 0;
+    ProfScope ps(c, SAYAKA_KCLASS_VEC, (long long)lst.n * G::NCELL);
+    hipLaunchKernelGGL((k_vec<G::NCELL, G::THREADS>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, a, op,
+                       redbuf(c), fin);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+VecArgs vec_args(sayaka_dev_ctx* c, ListRef lst) {
+    VecArgs a{};
+    a.list = dlist(c, lst);
+    a.vf = c->d_vf;
+    a.scal = c->d_scal;
+    return a;
+}
+
+// ------------------------------------------------------------------- dimension dispatch
+template <int D, int NB>
+struct Impl {
+    using G = Geo<D, NB>;
+    static constexpr int D_ = D, NB_ = NB;
+
+    static int apply(sayaka_dev_ctx* c, int m, int out, int in) {
+        StencilArgs a = stencil_args(c, c->tw[m], m);
+        a.x = c->d_field[in];
+        a.out = c->d_field[out];
+        return launch_stencil<D, NB>(c, ST_APPLY, a, FIN_NONE);
+    }
+    static int residual(sayaka_dev_ctx* c, int m, int r, int x, int rhs) {
+        StencilArgs a = stencil_args(c, c->tw[m], m);
+        a.x = c->d_field[x];
+        a.out = c->d_field[r];
+        a.rhs = c->d_field[rhs];
+        return launch_stencil<D, NB>(c, ST_RESID, a, FIN_NONE);
+    }
+    // One mg_smooth_level pass on tower m (SayakaMacOperator.cpp:1028-1112): sub-levels m..1,
+    // stop at the first empty one, red then black, each colour one launch.
+    static int smooth(sayaka_dev_ctx* c, int m, double* x, const double* rhs) {
+        for (int l = m; l >= 1; l--) {
+            ListRef sub = c->tw_sub[m][l];
+            if (sub.n <= 0) break;
+            for (int rb = 0; rb < 2; rb++) {
+                StencilArgs a = stencil_args(c, sub, m);
+                a.x = x;
+                a.rhs = rhs;
+                a.colour = rb;
+                int e = launch_stencil<D, NB>(c, ST_SMOOTH, a, FIN_NONE);
+                if (e) return e;
+            }
+        }
+        return 0;
+    }
+    static int restrict_(sayaka_dev_ctx* c, int cl, double* x) {
+        if (cl >= c->maxlevel || cl < 1) return 0;
+        ListRef lst = c->lv_nonleaf[cl];
+        if (lst.n <= 0 || c->lv_all[cl + 1].n <= 0) return 0;
+        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL);
+        hipLaunchKernelGGL((k_restrict<D, NB>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, dlist(c, lst),
+                           c->d_binfo, c->d_vf, x);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    static int prolong(sayaka_dev_ctx* c, int fl, double* x) {
+        if (fl <= 1 || fl > c->maxlevel) return 0;
+        ListRef lst = c->lv_all[fl];
+        if (lst.n <= 0) return 0;
+        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL);
+        hipLaunchKernelGGL((k_prolong<D, NB>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, dlist(c, lst),
+                           c->d_binfo, c->d_vf, x);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    static int bottom(sayaka_dev_ctx* c, double* sol, const double* rhs, double tol_rel, double tol_abs, int max_iter) {
+        ListRef lst = c->tw[1];
+        if (lst.n <= 0) return 0;
+        BottomArgs A{};
+        A.st = stencil_args(c, lst, 1);
+        A.rhs = rhs;
+        A.sol = sol;
+        A.pg = c->d_bot_pg;
+        A.sg = c->d_bot_sg;
+        A.partials = c->d_bot_part;
+        A.stats = c->d_bot_stats;
+        A.tol_rel = tol_rel;
+        A.tol_abs = tol_abs;
+        A.max_iter = max_iter;
+        A.exact = c->exact;
+        void* args[] = {&A};
+        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL);
+        HIPCHK(hipLaunchCooperativeKernel((const void*)k_bottom<D, NB>, dim3(lst.n), dim3(G::THREADS), args, 0,
+                                          c->stream));
+        return 0;
+    }
+    // mg_precond_level -> mg_relax_vcycle, one cycle from a zero guess
+    // (SayakaMacSolver.cpp:245-374, 924-949).  On entry RESID on T(L) already holds the
+    // masked right-hand side (rhs - L*0); the zero-guess residual is a masked copy and the
+    // residual after the correction is never read (it is only logged), so neither is
+    // recomputed here.  phat = 0 + CORR on T(L).
+    static int vcycle(sayaka_dev_ctx* c, int L, double* phat) {
+        int e;
+        double* resid = c->d_field[SAYAKA_RESID];
+        double* corr = c->d_field[SAYAKA_CORR];
+        for (int l = L; l >= 2; l--)
+            if ((e = restrict_(c, l - 1, resid))) return e;
+        if ((e = bottom(c, corr, resid, 1.0e-6, 1.0e-6, 2000))) return e;
+        for (int l = 2; l <= L; l++) {
+            if ((e = prolong(c, l, corr))) return e;
+            for (int s = 0; s < 2; s++)
+                if ((e = smooth(c, l, corr, resid))) return e;
+        }
+        VecArgs a = vec_args(c, c->tw[L]);
+        a.src = corr;
+        a.dst = phat;
+        return launch_vec<D, NB>(c, c->tw[L], V_COPYPLUS0, a, FIN_NONE);
+    }
+
+    // mg_precond_level (SayakaMacSolver.cpp:924-949): zero-guess V-cycle on tower m.
+    static int precond(sayaka_dev_ctx* c, int m, double* sol, const double* rhs) {
+        VecArgs a = vec_args(c, c->tw[m]);
+        a.src = rhs;
+        a.dst = c->d_field[SAYAKA_RESID];
+        int e = launch_vec<D, NB>(c, c->tw[m], V_MASKCOPY, a, FIN_NONE);
+        if (e) return e;
+        return vcycle(c, m, sol);
+    }
+
+    // ------------------------------------------------------------- mixed precision
+    static int mix(sayaka_dev_ctx* c, ListRef lst, int op, float* dst, const float* src, double* dst64,
+                   const double* src64) {
+        if (lst.n <= 0) return 0;
+        hipLaunchKernelGGL((k_mix<D, NB>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, dlist(c, lst), op,
+                           c->d_binfo, c->mixed.vf32, dst, src, dst64, src64);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    static StencilArgsT<float> args32(sayaka_dev_ctx* c, ListRef lst, int m) {
+        StencilArgsT<float> a{};
+        a.list = dlist(c, lst);
+        a.nlist = lst.n;
+        a.m = m;
+        a.binfo = c->d_binfo;
+        a.met = c->d_met;
+        a.w = c->mixed.w32;
+        a.wcf = c->mixed.wcf32;
+        a.vf = c->mixed.vf32;
+        a.alpha = (float)c->alpha;
+        return a;
+    }
+    static int mixed_prepare(sayaka_dev_ctx* c) {
+        MixedState& ms = c->mixed;
+        const size_t cells = (size_t)c->cap * c->ncell;
+        const size_t faces = (size_t)c->cap * D * c->nface;
+        const size_t cf = (size_t)c->cap * c->nf * c->nslab;
+        if (ms.cap_cells < cells || ms.nlev < c->maxlevel) {
+            mixed_free(ms);
+            HIPCHK(hipMalloc(&ms.w32, faces * sizeof(float)));
+            HIPCHK(hipMalloc(&ms.wcf32, cf * sizeof(float)));
+            HIPCHK(hipMalloc(&ms.vf32, cells * sizeof(float)));
+            HIPCHK(hipMalloc(&ms.r32, cells * sizeof(float)));
+            for (int l = 1; l <= c->maxlevel; l++) {
+                HIPCHK(hipMalloc(&ms.u[l], cells * sizeof(float)));
+                HIPCHK(hipMalloc(&ms.f[l], cells * sizeof(float)));
+            }
+            ms.cap_cells = cells;
+            ms.cap_faces = faces;
+            ms.cap_cf = cf;
+            ms.nlev = c->maxlevel;
+            ms.coef_version = -1;
+        }
+        if (ms.coef_version != c->coef_version) {
+            const size_t n1 = (size_t)c->nblk * D * c->nface, n2 = (size_t)c->nblk * c->nf * c->nslab,
+                         n3 = (size_t)c->nblk * c->ncell;
+            hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, c->stream, c->d_w, ms.w32, n1);
+            hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, c->stream, c->d_wcf, ms.wcf32, n2);
+            hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, c->stream, c->d_vf, ms.vf32, n3);
+            HIPCHK(hipGetLastError());
+            ms.coef_version = c->coef_version;
+        }
+        return 0;
+    }
+    static int smooth32(sayaka_dev_ctx* c, int m, float* u, const float* f) {
+        for (int l = m; l >= 1; l--) {
+            ListRef sub = c->tw_sub[m][l];
+            if (sub.n <= 0) break;
+            for (int rb = 0; rb < 2; rb++) {
+                StencilArgsT<float> a = args32(c, sub, m);
+                a.x = u;
+                a.rhs = f;
+                a.colour = rb;
+                ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL);
+                hipLaunchKernelGGL((k_stencil32<D, NB, ST_SMOOTH>), dim3(sub.n), dim3(G::THREADS), 0, c->stream, a);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    static int vrec(sayaka_dev_ctx* c, int l, int pre, int post) {
+        MixedState& ms = c->mixed;
+        int e;
+        if (l == 1) {
+            double* resid = c->d_field[SAYAKA_RESID];
+            double* corr = c->d_field[SAYAKA_CORR];
+            if ((e = mix(c, c->tw[1], M_F32_TO_F64, nullptr, ms.f[1], resid, nullptr))) return e;
+            if ((e = bottom(c, corr, resid, ms.bottom_tol_rel, ms.bottom_tol_abs, ms.bottom_max_iter))) return e;
+            return mix(c, c->tw[1], M_F64_TO_F32, ms.u[1], nullptr, nullptr, corr);
+        }
+        ListRef T = c->tw[l];
+        ListRef top = c->tw_sub[l][l];
+        ListRef below{top.off + top.n, T.n - top.n};
+        if ((e = mix(c, T, M_ZERO, ms.u[l], nullptr, nullptr, nullptr))) return e;
+        for (int s = 0; s < pre; s++)
+            if ((e = smooth32(c, l, ms.u[l], ms.f[l]))) return e;
+        {
+            StencilArgsT<float> a = args32(c, T, l);
+            a.x = ms.u[l];
+            a.rhs = ms.f[l];
+            a.out = ms.r32;
+            hipLaunchKernelGGL((k_stencil32<D, NB, ST_RESID>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if ((e = mix(c, below, M_COPY, ms.f[l - 1], ms.r32, nullptr, nullptr))) return e;
+        ListRef nl = c->lv_nonleaf[l - 1];
+        if (nl.n > 0) {
+            hipLaunchKernelGGL((k_restrict32<D, NB>), dim3(nl.n), dim3(G::THREADS), 0, c->stream, dlist(c, nl),
+                               c->d_binfo, ms.vf32, ms.r32, ms.f[l - 1]);
+            HIPCHK(hipGetLastError());
+        }
+        if ((e = vrec(c, l - 1, pre, post))) return e;
+        if ((e = mix(c, below, M_ADD, ms.u[l], ms.u[l - 1], nullptr, nullptr))) return e;
+        if ((e = mix(c, c->lv_all[l], M_PROLONG_ADD, ms.u[l], ms.u[l - 1], nullptr, nullptr))) return e;
+        for (int s = 0; s < post; s++)
+            if ((e = smooth32(c, l, ms.u[l], ms.f[l]))) return e;
+        return 0;
+    }
+    static int mixed_vcycle(sayaka_dev_ctx* c, int L, const double* resid64, double* phat64, int pre, int post) {
+        MixedState& ms = c->mixed;
+        int e;
+        if ((e = mix(c, c->tw[L], M_F64_TO_F32, ms.f[L], nullptr, nullptr, resid64))) return e;
+        if ((e = vrec(c, L, pre, post))) return e;
+        return mix(c, c->tw[L], M_F32_TO_F64, nullptr, ms.u[L], phat64, nullptr);
+    }
+
+    static int set_coefs(sayaka_dev_ctx* c) {
+        const int L = c->finest;
+        for (int which = 0; which < 2; which++) {
+            double* fd = which == 0 ? c->d_b : c->d_af;
+            for (int d = 0; d < D; d++) {
+                for (int l = L; l >= 2; l--) {
+                    ListRef fine = c->tw_sub[L][l], crse = c->tw_sub[L][l - 1];
+                    if (c->tw[L].n == 0 || fine.n <= 0 || crse.n <= 0) continue;
+                    hipLaunchKernelGGL((k_sync_fine<D, NB>), dim3(fine.n), dim3(64), 0, c->stream, dlist(c, fine),
+                                       c->d_binfo, fd, d);
+                    hipLaunchKernelGGL((k_sync_crse<D, NB>), dim3(crse.n), dim3(64), 0, c->stream, dlist(c, crse),
+                                       c->d_binfo, fd, d, L);
+                }
+                for (int l = L - 1; l >= 1; l--) {
+                    ListRef nl = c->lv_nonleaf[l];
+                    if (nl.n <= 0) continue;
+                    hipLaunchKernelGGL((k_restrict_face<D, NB>), dim3(nl.n), dim3(256), 0, c->stream, dlist(c, nl),
+                                       c->d_binfo, fd, d);
+                }
+            }
+        }
+        for (int l = L - 1; l >= 1; l--) {
+            ListRef nl = c->lv_nonleaf[l];
+            if (nl.n <= 0) continue;
+            hipLaunchKernelGGL((k_restrict_cell_avg<D, NB>), dim3(nl.n), dim3(G::THREADS), 0, c->stream,
+                               dlist(c, nl), c->d_binfo, c->d_vf);
+        }
+        hipLaunchKernelGGL((k_weights<D, NB>), dim3(c->all_blocks.n), dim3(256), 0, c->stream,
+                           dlist(c, c->all_blocks), c->d_binfo, c->d_met, c->d_b, c->d_af, c->beta, c->d_w,
+                           c->d_wcf);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    static int fix_bc(sayaka_dev_ctx* c, const double* dphi, int ng, double* rhs) {
+        ListRef lst = c->tw[c->finest];
+        if (lst.n <= 0) return 0;
+        hipLaunchKernelGGL((k_fix_bc<D, NB>), dim3(lst.n), dim3(64), 0, c->stream, dlist(c, lst), c->d_binfo,
+                           c->d_met, c->d_b, c->d_af, c->d_vf, c->beta, dphi, ng, rhs);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    static int read_scalars(sayaka_dev_ctx* c) {
+        HIPCHK(hipMemcpyAsync(c->h_scal, c->d_scal, S_NSCALAR * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+
+    static int dot(sayaka_dev_ctx* c, int m, const double* x, const double* y, double* dotv, double* mx) {
+        VecArgs a = vec_args(c, c->tw[m]);
+        a.src = x;
+        a.t = y;
+        int e = launch_vec<D, NB>(c, c->tw[m], V_DOT, a, FIN_DOTNORM);
+        if (e) return e;
+        if ((e = read_scalars(c))) return e;
+        if (dotv) *dotv = c->h_scal[S_DOT];
+        if (mx) *mx = c->h_scal[S_NORM];
+        return 0;
+    }
+
+    // MacSolver::cg_solve_level on the finest tower (SayakaMacSolver.cpp:683-920), scalars
+    // kept on the device; the host reads them back only at the two convergence tests of
+    // each iteration.
+    static int solve(sayaka_dev_ctx* c, const sayaka_solve_params* P, sayaka_solve_stats* st) {
+        const int L = c->finest;
+        ListRef T = c->tw[L];
+        double** F = c->d_field;
+        int e;
+        const bool mixed = P->precision == SAYAKA_MIXED;
+        if (mixed && (e = mixed_prepare(c))) return e;
+        struct ExactScope {
+            sayaka_dev_ctx* c;
+            int saved;
+            ~ExactScope() { c->exact = saved; }
+        } exact_scope{c, c->exact};
+        c->exact = mixed ? 0 : 1;
+        int precond = P->precond;
+        if (precond && L == 1) precond = 0;
+        HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 4 * sizeof(int), c->stream));
+        st->status = 2;
+        st->iters = 0;
+        st->half_step = 0;
+        st->nhist = 0;
+        // x0 = SOL ; r = RHS - A x0 ; rhat = r ; x = 0 ; rnorm0 = |r| ; rho = rhat.r
+        {
+            StencilArgs a = stencil_args(c, T, L);
+            a.x = F[SAYAKA_SOL];
+            a.rhs = F[SAYAKA_RHS];
+            a.out = F[SAYAKA_R];
+            a.out2 = F[SAYAKA_RHAT];
+            a.out3 = F[SAYAKA_X];
+            if ((e = launch_stencil<D, NB>(c, ST_RESID_INIT, a, FIN_INIT))) return e;
+        }
+        if ((e = read_scalars(c))) return e;
+        const double rnorm0 = c->h_scal[S_RNORM0];
+        st->rnorm0 = rnorm0;
+        st->rnorm = rnorm0;
+        if (rnorm0 == 0.0) {
+            st->status = 0;
+            return 0;
+        }
+        auto push = [&](double v) {
+            if (st->hist && st->nhist < P->hist_cap) st->hist[st->nhist] = v;
+            st->nhist++;
+        };
+        bool conv = false;
+        int stat = 0, iter;
+        double* phat = precond ? F[SAYAKA_PHAT] : F[SAYAKA_P];
+        double* shat = precond ? F[SAYAKA_SHAT] : F[SAYAKA_S];
+        for (iter = 1; iter <= P->max_iter; iter++) {
+            st->iters = iter;
+            if (c->h_scal[S_RHO] == 0.0) { stat = 1; break; }
+            {
+                VecArgs a = vec_args(c, T);
+                a.r = F[SAYAKA_R];
+                a.p = F[SAYAKA_P];
+                a.v = F[SAYAKA_V];
+                a.resid = F[SAYAKA_RESID];
+                if ((e = launch_vec<D, NB>(c, T, iter == 1 ? V_PINIT : V_PUPDATE, a, FIN_NONE))) return e;
+            }
+            if (precond) {
+                if (mixed) e = mixed_vcycle(c, L, F[SAYAKA_RESID], phat, P->pre_smooth, P->post_smooth);
+                else e = vcycle(c, L, phat);
+                if (e) return e;
+            }
+            {
+                StencilArgs a = stencil_args(c, T, L);
+                a.x = phat;
+                a.out = F[SAYAKA_V];
+                a.dotv = F[SAYAKA_RHAT];
+                if ((e = launch_stencil<D, NB>(c, ST_APPLY_DOT, a, FIN_ALPHA))) return e;
+            }
+            {
+                VecArgs a = vec_args(c, T);
+                a.r = F[SAYAKA_R];
+                a.v = F[SAYAKA_V];
+                a.s = F[SAYAKA_S];
+                a.x = F[SAYAKA_X];
+                a.phat = phat;
+                a.resid = F[SAYAKA_RESID];
+                if ((e = launch_vec<D, NB>(c, T, V_SUPDATE, a, FIN_SNORM))) return e;
+            }
+            if ((e = read_scalars(c))) return e;
+            if (c->h_scal[S_ALPHA_DEN] == 0.0) { stat = 1; break; }
+            const double snorm = c->h_scal[S_SNORM];
+            push(-snorm);
+            st->rnorm = snorm;
+            if (snorm < rnorm0 * P->tol_rel || snorm < P->tol_abs) {
+                conv = true;
+                st->half_step = 1;
+                break;
+            }
+            if (precond) {
+                if (mixed) e = mixed_vcycle(c, L, F[SAYAKA_RESID], shat, P->pre_smooth, P->post_smooth);
+                else e = vcycle(c, L, shat);
+                if (e) return e;
+            }
+            {
+                StencilArgs a = stencil_args(c, T, L);
+                a.x = shat;
+                a.out = F[SAYAKA_T];
+                a.dotv = F[SAYAKA_S];
+                if ((e = launch_stencil<D, NB>(c, ST_APPLY_DOT2, a, FIN_OMEGA))) return e;
+            }
+            {
+                VecArgs a = vec_args(c, T);
+                a.r = F[SAYAKA_R];
+                a.s = F[SAYAKA_S];
+                a.t = F[SAYAKA_T];
+                a.x = F[SAYAKA_X];
+                a.src = shat;
+                a.rhat = F[SAYAKA_RHAT];
+                if ((e = launch_vec<D, NB>(c, T, V_RUPDATE, a, FIN_RUPDATE))) return e;
+            }
+            if ((e = read_scalars(c))) return e;
+            if (c->h_scal[S_TT] == 0.0) { stat = 1; break; }
+            const double rnorm = c->h_scal[S_RNORM];
+            push(rnorm);
+            st->rnorm = rnorm;
+            if (rnorm < rnorm0 * P->tol_rel + P->tol_abs) { conv = true; break; }
+            if (c->h_scal[S_OMEGA] == 0.0) { stat = 1; break; }
+        }
+        if (conv) {
+            VecArgs a = vec_args(c, T);
+            a.x = F[SAYAKA_X];
+            a.dst = F[SAYAKA_SOL];
+            if ((e = launch_vec<D, NB>(c, T, V_FINAL, a, FIN_NONE))) return e;
+        }
+        HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        st->bottom_calls = c->h_bot_stats[0];
+        st->bottom_iters = c->h_bot_stats[1];
+        st->status = conv ? 0 : (stat ? 1 : 2);
+        return st->status;
+    }
+};
+
+template <class F>
+int dispatch(sayaka_dev_ctx* c, F&& f) {
+    if (c->ndim == 3 && c->nb == 8) return f(Impl<3, 8>{});
+    if (c->ndim == 2 && c->nb == 8) return f(Impl<2, 8>{});
+    if (c->ndim == 3 && c->nb == 16) return f(Impl<3, 16>{});
+    if (c->ndim == 2 && c->nb == 16) return f(Impl<2, 16>{});
+    return fail(-2, "unsupported (ndim, nb)");
+}
+
+bool field_ok(int f) { return f >= 0 && f < SAYAKA_NFIELD; }
+
+}  // namespace
+
+// =============================================================================== C ABI
+extern "C" {
+
+const char* sayaka_dev_last_error(void) { return g_err.c_str(); }
+
+int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_ctx** out) {
+    if (!out) return fail(-1, "null out");
+    *out = nullptr;
+    if (!((ndim == 3 || ndim == 2) && (nb == 8 || nb == 16))) return fail(-1, "supported blocks: 8^d, 16^d");
+    if (max_blocks < 1) return fail(-1, "max_blocks < 1");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(-1, "no such device");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new sayaka_dev_ctx;
+    c->device = device;
+    c->ndim = ndim;
+    c->nb = nb;
+    c->cap = max_blocks;
+    const int nbz = ndim == 3 ? nb : 1;
+    c->ncell = nb * nb * nbz;
+    c->nface = (nb + 1) * nb * nbz;
+    c->nslab = nb * nbz;
+    c->nf = 2 * ndim;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    const size_t cells = (size_t)max_blocks * c->ncell;
+    const size_t faces = (size_t)max_blocks * ndim * c->nface;
+    for (int f = 0; f < SAYAKA_NFIELD && e == hipSuccess; f++) e = hipMalloc(&c->d_field[f], cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_vf, cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_af, faces * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_b, faces * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_w, faces * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_wcf, (size_t)max_blocks * c->nf * c->nslab * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_partials, (size_t)(max_blocks + 64) * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_counter, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&c->d_scal, S_NSCALAR * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_scal, S_NSCALAR * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_pg, cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_sg, cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_part, (size_t)(max_blocks + 64) * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 4 * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_scal, 0, S_NSCALAR * sizeof(double), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_wcf, 0, (size_t)max_blocks * c->nf * c->nslab * sizeof(double), c->stream);
+    for (int f = 0; f < SAYAKA_NFIELD && e == hipSuccess; f++) e = hipMemsetAsync(c->d_field[f], 0, cells * sizeof(double), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        std::string msg = std::string("allocation failed: ") + hipGetErrorString(e);
+        sayaka_dev_destroy(c);
+        return fail(-3, msg);
+    }
+    *out = c;
+    return 0;
+}
+
+int sayaka_dev_destroy(sayaka_dev_ctx* c) {
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& p : c->d_field) if (p) hipFree(p);
+    void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_counter, c->d_scal,
+                    c->d_bot_pg, c->d_bot_sg, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists};
+    for (void* p : ptrs) if (p) hipFree(p);
+    for (int k = 0; k < SAYAKA_KCLASS_N; k++)
+        for (auto& r : c->prof[k]) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->h_scal) hipHostFree(c->h_scal);
+    if (c->h_bot_stats) hipHostFree(c->h_bot_stats);
+    mixed_free(c->mixed);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxlevel, const int* level,
+                            const int* parent, const int* which_child, const int* child8, const int* nbr6,
+                            const int* node_type, const double* length3) {
+    if (!c) return fail(-1, "null ctx");
+    if (nblk < 1 || nblk > c->cap) return fail(-1, "nblk exceeds the context capacity");
+    if (minlevel < 2) return fail(-1, "minAmrLevel must be >= 2 (level-1 leaves alias the outer Krylov state)");
+    if (maxlevel < minlevel || maxlevel > MAXLV - 1) return fail(-1, "bad maxlevel");
+    HIPCHK(hipSetDevice(c->device));
+    c->nblk = nblk;
+    c->minlevel = minlevel;
+    c->maxlevel = maxlevel;
+    c->h_binfo.assign(nblk, BlockInfo{});
+    c->h_met.assign(nblk, BlockMetric{});
+    int finest = 0;
+    for (int b = 0; b < nblk; b++) {
+        if (level[b] < 1 || level[b] > maxlevel) return fail(-1, "block level out of range");
+        finest = std::max(finest, level[b]);
+    }
+    c->finest = finest;
+    for (int b = 0; b < nblk; b++) {
+        BlockInfo& bi = c->h_binfo[b];
+        bi.level = level[b];
+        bi.leaf = node_type[b] == 1;
+        bi.parent = parent[b];
+        bi.which_child = which_child[b];
+        for (int q = 0; q < 8; q++) bi.child[q] = child8[b * 8 + q];
+        for (int f = 0; f < 6; f++) {
+            const int n = f < c->nf ? nbr6[b * 6 + f] : PHYS_BNDRY - 1;
+            bi.nbr[f] = n;
+            bi.cnbr[f] = -1;
+            bi.nbr_leaf[f] = 0;
+            bi.bct[f] = -1;
+            if (n >= nblk) return fail(-1, "neighbour id out of range");
+            if (n >= 0) bi.nbr_leaf[f] = node_type[n] == 1;
+            else if (n > PHYS_BNDRY) {
+                if (parent[b] < 0) return fail(-1, "coarse face on a root block");
+                const int cn = nbr6[parent[b] * 6 + f];
+                if (cn < 0) return fail(-1, "coarse neighbour missing (2:1 balance violated)");
+                bi.cnbr[f] = cn;
+            }
+        }
+        // block metric (SayakaTree.h:277-301)
+        BlockMetric& mt = c->h_met[b];
+        double dx[3], v = 1.0;
+        for (int d = 0; d < 3; d++) dx[d] = length3[b * 3 + d];
+        for (int d = 0; d < c->ndim; d++) dx[d] /= c->nb;
+        for (int d = 0; d < c->ndim; d++) v *= dx[d];
+        for (int d = 0; d < 3; d++) {
+            mt.dx[d] = dx[d];
+            mt.ds[d] = d < c->ndim ? v / dx[d] : 1.0;
+        }
+        mt.dv = v;
+    }
+    // block lists
+    std::vector<int>& L = c->h_lists;
+    L.clear();
+    auto add = [&](ListRef& r, auto pred) {
+        r.off = (int)L.size();
+        for (int b = 0; b < nblk; b++)
+            if (pred(b)) L.push_back(b);
+        r.n = (int)L.size() - r.off;
+    };
+    for (int l = 0; l <= MAXLV; l++) c->lv_all[l] = c->lv_leaf[l] = c->lv_nonleaf[l] = c->tw[l] = ListRef{};
+    for (int l = 1; l <= maxlevel; l++) {
+        add(c->lv_all[l], [&](int b) { return level[b] == l; });
+        add(c->lv_leaf[l], [&](int b) { return level[b] == l && node_type[b] == 1; });
+        add(c->lv_nonleaf[l], [&](int b) { return level[b] == l && node_type[b] != 1; });
+    }
+    add(c->all_blocks, [&](int) { return true; });
+    for (int m = 1; m <= maxlevel; m++) {
+        for (int l = 0; l <= MAXLV; l++) c->tw_sub[m][l] = ListRef{};
+        c->tw[m].off = (int)L.size();
+        if (c->lv_all[m].n > 0) {
+            for (int l = m; l >= 1; l--) {
+                ListRef& s = c->tw_sub[m][l];
+                s.off = (int)L.size();
+                for (int b = 0; b < nblk; b++)
+                    if (level[b] == l && (node_type[b] == 1 || l == m)) L.push_back(b);
+                s.n = (int)L.size() - s.off;
+            }
+        }
+        c->tw[m].n = (int)L.size() - c->tw[m].off;
+    }
+    if (c->d_binfo) hipFree(c->d_binfo);
+    if (c->d_met) hipFree(c->d_met);
+    if (c->d_lists) hipFree(c->d_lists);
+    c->d_binfo = nullptr;
+    c->d_met = nullptr;
+    c->d_lists = nullptr;
+    HIPCHK(hipMalloc(&c->d_binfo, nblk * sizeof(BlockInfo)));
+    HIPCHK(hipMalloc(&c->d_met, nblk * sizeof(BlockMetric)));
+    HIPCHK(hipMalloc(&c->d_lists, std::max<size_t>(1, L.size()) * sizeof(int)));
+    HIPCHK(hipMemcpy(c->d_binfo, c->h_binfo.data(), nblk * sizeof(BlockInfo), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_met, c->h_met.data(), nblk * sizeof(BlockMetric), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_lists, L.data(), L.size() * sizeof(int), hipMemcpyHostToDevice));
+    // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32)
+    c->alpha = 0.0;
+    c->beta = 1.0;
+    return sayaka_dev_set_coefs(c, 0.0, 1.0, nullptr, nullptr, nullptr);
+}
+
+int sayaka_dev_set_bc(sayaka_dev_ctx* c, const int* bc6) {
+    if (!c || !bc6) return fail(-1, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    for (int b = 0; b < c->nblk; b++)
+        for (int f = 0; f < c->nf; f++) c->h_binfo[b].bct[f] = c->h_binfo[b].nbr[f] <= PHYS_BNDRY ? bc6[b * 6 + f] : -1;
+    HIPCHK(hipMemcpy(c->d_binfo, c->h_binfo.data(), c->nblk * sizeof(BlockInfo), hipMemcpyHostToDevice));
+    // Neumann / Dirichlet change the wall weights
+    int e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
+    c->coef_version++;
+    return e;
+}
+
+int sayaka_dev_set_coefs(sayaka_dev_ctx* c, double alpha, double beta, const double* vf, const double* const* af,
+                         const double* const* b) {
+    if (!c) return fail(-1, "null ctx");
+    if (alpha != 0.0) return fail(-1, "alpha must be 0 (the reference asserts it, SayakaMacOperator.cpp:1268)");
+    HIPCHK(hipSetDevice(c->device));
+    c->alpha = alpha;
+    c->beta = beta;
+    const size_t cells = (size_t)c->nblk * c->ncell;
+    const size_t nfd = (size_t)c->nface;
+    std::vector<double> ones(std::max(cells, (size_t)c->nblk * c->ndim * nfd), 1.0);
+    if (vf) HIPCHK(hipMemcpy(c->d_vf, vf, cells * sizeof(double), hipMemcpyHostToDevice));
+    else HIPCHK(hipMemcpy(c->d_vf, ones.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+    // host arrays are [dir][block][face]; device arrays are [block][dir][face]
+    std::vector<double> tmp((size_t)c->nblk * c->ndim * nfd);
+    for (int which = 0; which < 2; which++) {
+        const double* const* src = which == 0 ? b : af;
+        for (int blk = 0; blk < c->nblk; blk++)
+            for (int d = 0; d < c->ndim; d++) {
+                double* o = tmp.data() + ((size_t)blk * c->ndim + d) * nfd;
+                if (src && src[d]) std::memcpy(o, src[d] + (size_t)blk * nfd, nfd * sizeof(double));
+                else for (size_t q = 0; q < nfd; q++) o[q] = 1.0;
+            }
+        HIPCHK(hipMemcpy(which == 0 ? c->d_b : c->d_af, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
+    if (e) return e;
+    c->coef_version++;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_get_coefs(sayaka_dev_ctx* c, double* vf, double* const* af, double* const* b, double* w, double* wcf) {
+    if (!c) return fail(-1, "null ctx");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t cells = (size_t)c->nblk * c->ncell;
+    const size_t nfd = (size_t)c->nface;
+    if (vf) HIPCHK(hipMemcpy(vf, c->d_vf, cells * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> tmp((size_t)c->nblk * c->ndim * nfd);
+    for (int which = 0; which < 2; which++) {
+        double* const* dst = which == 0 ? b : af;
+        if (!dst) continue;
+        HIPCHK(hipMemcpy(tmp.data(), which == 0 ? c->d_b : c->d_af, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int blk = 0; blk < c->nblk; blk++)
+            for (int d = 0; d < c->ndim; d++)
+                if (dst[d]) std::memcpy(dst[d] + (size_t)blk * nfd, tmp.data() + ((size_t)blk * c->ndim + d) * nfd, nfd * sizeof(double));
+    }
+    if (w) HIPCHK(hipMemcpy(w, c->d_w, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (wcf) HIPCHK(hipMemcpy(wcf, c->d_wcf, (size_t)c->nblk * c->nf * c->nslab * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sayaka_dev_upload(sayaka_dev_ctx* c, int field, const double* h) {
+    if (!c || !h || !field_ok(field)) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_field[field], h, (size_t)c->nblk * c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_download(sayaka_dev_ctx* c, int field, double* h) {
+    if (!c || !h || !field_ok(field)) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(h, c->d_field[field], (size_t)c->nblk * c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_fix_bc(sayaka_dev_ctx* c, const double* phi, int ng, int rhs_field) {
+    if (!c || !phi || ng < 1 || !field_ok(rhs_field)) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int nbz = c->ndim == 3 ? c->nb : 1;
+    const size_t n = (size_t)c->nblk * (c->nb + 2 * ng) * (c->nb + 2 * ng) * (nbz + (c->ndim == 3 ? 2 * ng : 0));
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(double)));
+    hipError_t e = hipMemcpy(d, phi, n * sizeof(double), hipMemcpyHostToDevice);
+    int r = e == hipSuccess ? dispatch(c, [&](auto I) { return decltype(I)::fix_bc(c, d, ng, c->d_field[rhs_field]); })
+                            : fail(-3, hipGetErrorString(e));
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return r;
+}
+
+int sayaka_dev_apply(sayaka_dev_ctx* c, int m, int out, int in) {
+    if (!c || !field_ok(out) || !field_ok(in) || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::apply(c, m, out, in); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_residual(sayaka_dev_ctx* c, int m, int r, int x, int rhs) {
+    if (!c || !field_ok(r) || !field_ok(x) || !field_ok(rhs) || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::residual(c, m, r, x, rhs); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_smooth(sayaka_dev_ctx* c, int m, int sol, int rhs) {
+    if (!c || !field_ok(sol) || !field_ok(rhs) || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::smooth(c, m, c->d_field[sol], c->d_field[rhs]); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_restrict(sayaka_dev_ctx* c, int cl, int field) {
+    if (!c || !field_ok(field)) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::restrict_(c, cl, c->d_field[field]); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_prolong(sayaka_dev_ctx* c, int fl, int field) {
+    if (!c || !field_ok(field)) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::prolong(c, fl, c->d_field[field]); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_dot(sayaka_dev_ctx* c, int m, int a, int b, double* dotv, double* mx) {
+    if (!c || !field_ok(a) || !field_ok(b) || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
+    return dispatch(c, [&](auto I) { return decltype(I)::dot(c, m, c->d_field[a], c->d_field[b], dotv, mx); });
+}
+
+int sayaka_dev_precond(sayaka_dev_ctx* c, int m, int sol, int rhs) {
+    if (!c || !field_ok(sol) || !field_ok(rhs) || m < 2 || m > c->finest) return fail(-1, "bad argument");
+    int e = dispatch(c, [&](auto I) { return decltype(I)::precond(c, m, c->d_field[sol], c->d_field[rhs]); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_bottom(sayaka_dev_ctx* c, int sol, int rhs, double tol_rel, double tol_abs, int max_iter, int* iters,
+                      int* status) {
+    if (!c || !field_ok(sol) || !field_ok(rhs) || max_iter < 1) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 4 * sizeof(int), c->stream));
+    int e = dispatch(c, [&](auto I) {
+        return decltype(I)::bottom(c, c->d_field[sol], c->d_field[rhs], tol_rel, tol_abs, max_iter);
+    });
+    if (e) return e;
+    HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (iters) *iters = c->h_bot_stats[2];
+    if (status) *status = c->h_bot_stats[3];
+    return 0;
+}
+
+int sayaka_dev_solve(sayaka_dev_ctx* c, const sayaka_solve_params* p, sayaka_solve_stats* st) {
+    if (!c || !p || !st) return fail(-1, "null argument");
+    if (p->max_iter < 1 || p->tol_rel < 0 || p->tol_abs < 0) return fail(-1, "bad solver parameters");
+    if (p->precision != SAYAKA_FP64 && p->precision != SAYAKA_MIXED) return fail(-1, "bad precision");
+    HIPCHK(hipSetDevice(c->device));
+    return dispatch(c, [&](auto I) { return decltype(I)::solve(c, p, st); });
+}
+
+int sayaka_dev_fill(sayaka_dev_ctx* c, int field, double value) {
+    if (!c || !field_ok(field)) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    if (value != 0.0) return fail(-1, "only zero fill is supported");
+    int e = dispatch(c, [&](auto I) {
+        using Im = decltype(I);
+        VecArgs a = vec_args(c, c->all_blocks);
+        a.dst = c->d_field[field];
+        return launch_vec<Im::D_, Im::NB_>(c, c->all_blocks, V_ZERO, a, FIN_NONE);
+    });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_profile(sayaka_dev_ctx* c, int kclass, int enable) {
+    if (!c || kclass < 0 || kclass >= SAYAKA_KCLASS_N) return fail(-1, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& r : c->prof[kclass]) {
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->prof[kclass].clear();
+    c->prof_on[kclass] = enable != 0;
+    return 0;
+}
+
+int sayaka_dev_profile_read(sayaka_dev_ctx* c, int kclass, double* total_ms, long long* launches, long long* cells) {
+    if (!c || kclass < 0 || kclass >= SAYAKA_KCLASS_N) return fail(-1, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t = 0.0;
+    long long n = 0, cl = 0;
+    for (auto& r : c->prof[kclass]) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        n++;
+        cl += r.cells;
+    }
+    if (total_ms) *total_ms = t;
+    if (launches) *launches = n;
+    if (cells) *cells = cl;
+    return 0;
+}
+
+int sayaka_dev_sync(sayaka_dev_ctx* c) {
+    if (!c) return fail(-1, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""Python mirror of the device solver (include/sayaka_amd.h) over ctypes.
+
+``MacSolverGPU`` follows the reference MacSolver call sequence (Test7.h:612-637):
+prepare (topology) -> set_bc -> coefficients -> rhs / initial solution -> cg_solve -> solution.
+There is no CPU path: every call goes to libsayaka_dev.so and raises on error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native as N
+from .workloads import NEUMANN
+
+i32, f64, vp = C.c_int, C.c_double, C.c_void_p
+
+RHS, SOL, RESID, CORR, TMP, X0, X, R, RHAT, P, PHAT, S, SHAT, V, T = range(15)
+FP64, MIXED = 0, 1
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tol_rel", f64), ("tol_abs", f64), ("max_iter", i32), ("precond", i32),
+                ("precision", i32), ("pre_smooth", i32), ("post_smooth", i32), ("hist_cap", i32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", i32), ("iters", i32), ("half_step", i32), ("bottom_calls", i32),
+                ("bottom_iters", C.c_longlong), ("rnorm0", f64), ("rnorm", f64), ("nhist", i32),
+                ("hist", C.POINTER(f64))]
+
+
+_dev = None
+_SIG = {
+    "sayaka_dev_last_error": ([], C.c_char_p),
+    "sayaka_dev_create": ([i32, i32, i32, i32, C.POINTER(vp)], i32),
+    "sayaka_dev_destroy": ([vp], i32),
+    "sayaka_dev_set_topology": ([vp, i32, i32, i32] + [vp] * 7, i32),
+    "sayaka_dev_set_bc": ([vp, vp], i32),
+    "sayaka_dev_set_coefs": ([vp, f64, f64, vp, vp, vp], i32),
+    "sayaka_dev_get_coefs": ([vp, vp, vp, vp, vp, vp], i32),
+    "sayaka_dev_upload": ([vp, i32, vp], i32),
+    "sayaka_dev_download": ([vp, i32, vp], i32),
+    "sayaka_dev_fix_bc": ([vp, vp, i32, i32], i32),
+    "sayaka_dev_apply": ([vp, i32, i32, i32], i32),
+    "sayaka_dev_residual": ([vp, i32, i32, i32, i32], i32),
+    "sayaka_dev_smooth": ([vp, i32, i32, i32], i32),
+    "sayaka_dev_restrict": ([vp, i32, i32], i32),
+    "sayaka_dev_prolong": ([vp, i32, i32], i32),
+    "sayaka_dev_dot": ([vp, i32, i32, i32, C.POINTER(f64), C.POINTER(f64)], i32),
+    "sayaka_dev_precond": ([vp, i32, i32, i32], i32),
+    "sayaka_dev_bottom": ([vp, i32, i32, f64, f64, i32, C.POINTER(i32), C.POINTER(i32)], i32),
+    "sayaka_dev_solve": ([vp, C.POINTER(SolveParams), C.POINTER(SolveStats)], i32),
+    "sayaka_dev_sync": ([vp], i32),
+    "sayaka_dev_fill": ([vp, i32, f64], i32),
+    "sayaka_dev_profile": ([vp, i32, i32], i32),
+    "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], i32),
+}
+
+# kernel classes of sayaka_dev_profile (include/sayaka_amd.h)
+K_APPLY, K_SMOOTH, K_SMOOTH32, K_BOTTOM, K_VEC, K_TRANSFER = range(6)
+KCLASS_NAMES = ("apply", "smooth", "smooth32", "bottom", "vec", "transfer")
+
+
+def dev():
+    """libsayaka_dev.so; raises when it was not built (no fallback exists)."""
+    global _dev
+    if _dev is None:
+        if not os.path.exists(N.DEV_LIB):
+            raise N.NativeLibraryMissing(f"{N.DEV_LIB} not built; run __graft_entry__.build()")
+        lib = C.CDLL(N.DEV_LIB)
+        N.register(lib, _SIG)
+        _dev = lib
+    return _dev
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc < 0:
+        raise DeviceError(f"{what}: {dev().sayaka_dev_last_error().decode()}")
+    return rc
+
+
+def _parr(arrs):
+    keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+    pa = (vp * 3)(*([None if a is None else a.ctypes.data for a in keep] + [None] * (3 - len(keep))))
+    return keep, pa
+
+
+class MacSolverGPU:
+    """Device MAC solver for one tree (MacSolver, SayakaMacSolver.h:25-425)."""
+
+    def __init__(self, ndim: int, nb: int, max_blocks: int, device: int = 0):
+        self.lib = dev()
+        self.ndim, self.nb = ndim, nb
+        self.ncell = nb ** ndim
+        self.nface = (nb + 1) * nb ** (ndim - 1)
+        h = vp()
+        _chk(self.lib.sayaka_dev_create(device, ndim, nb, max_blocks, C.byref(h)), "create")
+        self.h = h
+        self.n = 0
+        self.finest = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sayaka_dev_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- MacSolver::prepareSolver (tables of the tree) ---------------------------------
+    def prepare(self, tree: dict, spec):
+        self.n = int(tree["level"].shape[0])
+        self.finest = int(tree["level"].max())
+        self._tree = {k: np.ascontiguousarray(tree[k], dtype=np.int32)
+                      for k in ("level", "parent", "which_child", "child", "neighbor", "node_type")}
+        ln = np.ascontiguousarray(tree["length"], dtype=np.float64)
+        t = self._tree
+        _chk(self.lib.sayaka_dev_set_topology(self.h, self.n, spec.minlevel, spec.maxlevel,
+                                              N.ptr(t["level"]), N.ptr(t["parent"]), N.ptr(t["which_child"]),
+                                              N.ptr(t["child"]), N.ptr(t["neighbor"]), N.ptr(t["node_type"]),
+                                              N.ptr(ln)), "set_topology")
+        self._nbr = t["neighbor"]
+
+    def set_bc(self, physbc, mathbc):
+        """MacSolver::setBC with one phys->math map for every wall."""
+        bc = np.full((self.n, 6), -1, np.int32)
+        for pb, mb in zip(physbc, mathbc):
+            bc[self._nbr == pb] = mb
+        _chk(self.lib.sayaka_dev_set_bc(self.h, N.ptr(bc)), "set_bc")
+
+    def set_coefs(self, vf=None, af=None, b=None, beta=1.0):
+        v = None if vf is None else np.ascontiguousarray(vf, dtype=np.float64)
+        ka, pa = _parr(af) if af is not None else (None, None)
+        kb, pb = _parr(b) if b is not None else (None, None)
+        _chk(self.lib.sayaka_dev_set_coefs(self.h, 0.0, beta, N.ptr(v), pa, pb), "set_coefs")
+
+    def get_coefs(self):
+        vf = np.zeros((self.n, self.ncell))
+        b = [np.zeros((self.n, self.nface)) for _ in range(self.ndim)]
+        a = [np.zeros((self.n, self.nface)) for _ in range(self.ndim)]
+        w = np.zeros((self.n, self.ndim, self.nface))
+        wcf = np.zeros((self.n, 2 * self.ndim, self.nb ** (self.ndim - 1)))
+        kb, pb = _parr(b)
+        ka, pa = _parr(a)
+        _chk(self.lib.sayaka_dev_get_coefs(self.h, N.ptr(vf), pa, pb, N.ptr(w), N.ptr(wcf)), "get_coefs")
+        return vf, kb, ka, w, wcf
+
+    # --- fields ----------------------------------------------------------------------
+    def upload(self, field, x):
+        _chk(self.lib.sayaka_dev_upload(self.h, field, N.ptr(np.ascontiguousarray(x, dtype=np.float64))), "upload")
+
+    def download(self, field):
+        out = np.zeros((self.n, self.ncell))
+        _chk(self.lib.sayaka_dev_download(self.h, field, N.ptr(out)), "download")
+        return out
+
+    def fix_bc(self, phi_grown, ngrow, rhs_field=RHS):
+        _chk(self.lib.sayaka_dev_fix_bc(self.h, N.ptr(np.ascontiguousarray(phi_grown, dtype=np.float64)), ngrow,
+                                        rhs_field), "fix_bc")
+
+    # --- operators -------------------------------------------------------------------
+    def apply(self, m, out, inp):
+        _chk(self.lib.sayaka_dev_apply(self.h, m, out, inp), "apply")
+
+    def residual(self, m, r, x, rhs):
+        _chk(self.lib.sayaka_dev_residual(self.h, m, r, x, rhs), "residual")
+
+    def smooth(self, m, sol, rhs):
+        _chk(self.lib.sayaka_dev_smooth(self.h, m, sol, rhs), "smooth")
+
+    def restrict(self, cl, field):
+        _chk(self.lib.sayaka_dev_restrict(self.h, cl, field), "restrict")
+
+    def prolong(self, fl, field):
+        _chk(self.lib.sayaka_dev_prolong(self.h, fl, field), "prolong")
+
+    def dot(self, m, a, b):
+        d = f64()
+        mx = f64()
+        _chk(self.lib.sayaka_dev_dot(self.h, m, a, b, C.byref(d), C.byref(mx)), "dot")
+        return d.value, mx.value
+
+    def precond(self, m, sol, rhs):
+        _chk(self.lib.sayaka_dev_precond(self.h, m, sol, rhs), "precond")
+
+    def bottom(self, sol, rhs, tol_rel=1e-6, tol_abs=1e-6, max_iter=2000):
+        it = i32()
+        st = i32()
+        _chk(self.lib.sayaka_dev_bottom(self.h, sol, rhs, tol_rel, tol_abs, max_iter, C.byref(it), C.byref(st)),
+             "bottom")
+        return it.value, st.value
+
+    def solve(self, tol_rel=1e-10, tol_abs=0.0, max_iter=2000, precond=1, precision=FP64, pre_smooth=2,
+              post_smooth=2, hist_cap=4096):
+        """MacSolver::cg_solve from the SOL field; returns the solve statistics."""
+        p = SolveParams(tol_rel, tol_abs, max_iter, precond, precision, pre_smooth, post_smooth, hist_cap)
+        hist = np.zeros(max(hist_cap, 1))
+        st = SolveStats()
+        st.hist = hist.ctypes.data_as(C.POINTER(f64))
+        rc = _chk(self.lib.sayaka_dev_solve(self.h, C.byref(p), C.byref(st)), "solve")
+        return dict(status=st.status, rc=rc, iters=st.iters, half_step=bool(st.half_step),
+                    bottom_calls=st.bottom_calls, bottom_iters=int(st.bottom_iters), rnorm0=st.rnorm0,
+                    rnorm=st.rnorm, hist=hist[:min(st.nhist, hist_cap)].copy())
+
+    def sync(self):
+        _chk(self.lib.sayaka_dev_sync(self.h), "sync")
+
+    def zero(self, field):
+        """setZeroSol on the device for any field."""
+        _chk(self.lib.sayaka_dev_fill(self.h, field, 0.0), "fill")
+
+    def profile(self, kclass, enable=True):
+        _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")
+
+    def profile_read(self, kclass):
+        t = f64()
+        n = C.c_longlong()
+        cells = C.c_longlong()
+        _chk(self.lib.sayaka_dev_profile_read(self.h, kclass, C.byref(t), C.byref(n), C.byref(cells)), "profile_read")
+        return dict(total_ms=t.value, launches=n.value, cells=cells.value)

@@ -1,0 +1,206 @@
+"""Write the golden vectors of the reference solve path into tests/golden/*.npz.
+
+Runs the reference itself (oracle/_ref/libsayaka_ref{2,3}.so, built from the reference
+sources by oracle/build_ref.sh) in this container; the .npz files are data only (inputs
+and the reference's outputs) and are what tests/ check the oracle and the product tree
+against on machines where the reference is absent.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import refbind as R  # noqa: E402
+from sayakaamr_amd import workloads as W  # noqa: E402
+
+
+def grown(v, nb, nd):
+    n = v.shape[0]
+    if nd == 3:
+        g = np.zeros((n, nb + 2, nb + 2, nb + 2))
+        g[:, 1:-1, 1:-1, 1:-1] = v.reshape(n, nb, nb, nb)
+    else:
+        g = np.zeros((n, nb + 2, nb + 2))
+        g[:, 1:-1, 1:-1] = v.reshape(n, nb, nb)
+    return g.reshape(n, -1)
+
+
+def valid(g, nb, nd):
+    n = g.shape[0]
+    if nd == 3:
+        return g.reshape(n, nb + 2, nb + 2, nb + 2)[:, 1:-1, 1:-1, 1:-1].reshape(n, -1)
+    return g.reshape(n, nb + 2, nb + 2)[:, 1:-1, 1:-1].reshape(n, -1)
+
+
+def tree_arrays(prefix, t):
+    return {f"{prefix}{k}": t[k] for k in W.TREE_FIELDS}
+
+
+def tower_arrays(prefix, mac, maxlevel):
+    out = {}
+    for m in range(1, maxlevel + 1):
+        tw = mac.tower(m)
+        for k, v in tw.items():
+            out[f"{prefix}m{m}_{k}"] = v
+    return out
+
+
+def case_ops(name, spec, crit, inp_fn, bc, seed=7):
+    """Tree, towers, coefficient state and the outputs of every operator on seeded inputs."""
+    rt = R.RefTree(spec.ndim)
+    tree = W.build_tree(rt, spec, crit)
+    n = tree["level"].shape[0]
+    nb, nd = spec.nb, spec.ndim
+    inp = inp_fn(tree, spec)
+    mac = R.RefMac(rt)
+    mac.prepare()
+    if inp.get("vf") is not None:
+        mac.set_ib_frac(inp["vf"], inp["af"])
+    if inp.get("b") is not None:
+        mac.set_coefb(inp["b"])
+    mac.set_bc([W.WALL], [bc], [0.0])
+    out = {}
+    out.update(tree_arrays("tree_", tree))
+    out.update(tower_arrays("tower_", mac, spec.maxlevel))
+    for k in ("vf", "rhs"):
+        if inp.get(k) is not None:
+            out[f"in_{k}"] = inp[k]
+    for d in range(nd):
+        if inp.get("af") is not None:
+            out[f"in_af{d}"] = inp["af"][d]
+        if inp.get("b") is not None:
+            out[f"in_b{d}"] = inp["b"][d]
+    vf, b, a = mac.get_coefs(nb)
+    out["coef_vf"] = vf
+    for d in range(nd):
+        out[f"coef_b{d}"] = b[d]
+        out[f"coef_a{d}"] = a[d]
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, nb ** nd))
+    y = rng.standard_normal((n, nb ** nd))
+    out["op_x"] = x
+    out["op_y"] = y
+    ng = (nb + 2) ** 2 * ((nb + 2) if nd == 3 else 1)
+    L = spec.maxlevel
+    for m in range(1, L + 1):
+        mac.buf_set(R.P, grown(x, nb, nd))
+        mac.lib.ref_apply(m, R.V, R.P)
+        out[f"op_apply_m{m}"] = valid(mac.buf_get(R.V, n, ng), nb, nd)
+    for m in range(2, L + 1):
+        mac.buf_set(R.CORR, grown(x, nb, nd))
+        mac.buf_set(R.RESID, grown(y, nb, nd))
+        mac.lib.ref_smooth(m, R.CORR, R.RESID)
+        out[f"op_smooth_m{m}"] = valid(mac.buf_get(R.CORR, n, ng), nb, nd)
+    mac.buf_set(R.RESID, grown(y, nb, nd))
+    for cl in range(L - 1, 0, -1):
+        mac.lib.ref_restrict(cl, R.RESID)
+    out["op_restrict"] = valid(mac.buf_get(R.RESID, n, ng), nb, nd)
+    mac.buf_set(R.CORR, grown(x, nb, nd))
+    for fl in range(2, L + 1):
+        mac.lib.ref_prolong(fl, R.CORR)
+    out["op_prolong"] = valid(mac.buf_get(R.CORR, n, ng), nb, nd)
+    mac.buf_set(R.P, grown(x, nb, nd))
+    mac.buf_set(R.V, grown(y, nb, nd))
+    out["op_dot"] = np.array([mac.lib.ref_dot(L, R.P, R.V), mac.lib.ref_norm(L, R.P)])
+    # one preconditioner application on the (masked) rhs
+    mac.buf_set(R.P, grown(inp["rhs"], nb, nd))
+    mac.lib.ref_precond(L, R.PHAT, R.P)
+    out["op_precond"] = valid(mac.buf_get(R.PHAT, n, ng), nb, nd)
+    # full solve
+    mac.set_rhs(inp["rhs"])
+    mac.set_zero_sol()
+    mac.set_params(1e-10, 0.0, 2000, 1)
+    res = mac.cg_solve()
+    out["solve_iters"] = np.array([res["iters"], int(res["converged"]), res["stat"]])
+    out["solve_hist"] = res["hist"]
+    out["solve_sol"] = mac.get_sol(n, nb ** nd)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: {n} blocks, iters {res['iters']}, wrote {len(out)} arrays")
+
+
+def c1_inputs(tree, spec):
+    return W.c1_inputs(tree, spec, seed=0)
+
+
+def c0_inputs(tree, spec):
+    rhs, exact = W.c0_rhs_exact(tree, spec)
+    return dict(rhs=rhs)
+
+
+def c1_summary():
+    """C1 at full size: tree tables and the solve outcome (solution sampled + checksums)."""
+    spec = W.C1_SPEC
+    rt = R.RefTree(3)
+    tree = W.build_tree(rt, spec, W.shell_criterion())
+    n = tree["level"].shape[0]
+    inp = W.c1_inputs(tree, spec, seed=0)
+    mac = R.RefMac(rt)
+    mac.prepare()
+    mac.set_ib_frac(inp["vf"], inp["af"])
+    mac.set_coefb(inp["b"])
+    mac.set_bc([W.WALL], [W.NEUMANN], [0.0])
+    out = tree_arrays("tree_", tree)
+    out.update(tower_arrays("tower_", mac, spec.maxlevel))
+    mac.set_rhs(inp["rhs"])
+    mac.set_zero_sol()
+    mac.set_params(1e-10, 0.0, 2000, 1)
+    res = mac.cg_solve()
+    sol = mac.get_sol(n, 512)
+    leaf = tree["node_type"] == 1
+    pb, pc = W.pin_cell(tree, spec)
+    rng = np.random.default_rng(11)
+    idx_b = rng.choice(np.nonzero(leaf)[0], 64, replace=False)
+    idx_c = rng.integers(0, 512, 64)
+    out["solve_iters"] = np.array([res["iters"], int(res["converged"]), res["stat"]])
+    out["solve_hist"] = res["hist"]
+    out["sample_blocks"] = idx_b
+    out["sample_cells"] = idx_c
+    out["sample_sol"] = sol[idx_b, idx_c]
+    out["sol_leaf_l2"] = np.array([np.linalg.norm(sol[leaf]), np.linalg.norm(sol[leaf] - sol[pb, pc])])
+    out["rhs_checksum"] = np.array([inp["rhs"].sum(), np.abs(inp["rhs"]).sum()])
+    np.savez_compressed(os.path.join(HERE, "c1_summary.npz"), **out)
+    print(f"c1_summary: {n} blocks, iters {res['iters']}")
+
+
+def regrid_sequence():
+    """A moving sphere with refinement and coarsening every step: tree tables per step."""
+    spec = W.small_spec(3, 4)
+    rt = R.RefTree(3)
+    out = {}
+    t = W.build_tree(rt, spec, W.shell_criterion(centre=(0.35, 0.5, 0.5), r0=0.18))
+    for step in range(6):
+        c = (0.35 + 0.05 * step, 0.5, 0.5)
+        crit = W.shell_criterion(centre=c, r0=0.18)
+        t = rt.export()
+        nblk = t["level"].shape[0]
+        leaf_or_parent = (t["node_type"] == 1) | (t["node_type"] == 2)
+        hit = np.zeros(nblk, bool)
+        cand = np.nonzero(leaf_or_parent)[0]
+        hit[cand] = crit(t, spec.nb, spec.ndim, cand)
+        refine = (hit & (t["node_type"] == 1)).astype(np.int32)
+        coarsen = ((~hit) & (t["node_type"] == 1) & (t["level"] > spec.minlevel)).astype(np.int32)
+        rt.set_refine_flags(refine)
+        rt.set_coarsen_flags(coarsen)
+        rt.regrid()
+        t = rt.export()
+        out[f"step{step}_refine"] = refine
+        out[f"step{step}_coarsen"] = coarsen
+        out.update(tree_arrays(f"step{step}_", t))
+    np.savez_compressed(os.path.join(HERE, "regrid_sequence.npz"), **out)
+    print("regrid_sequence: 6 steps")
+
+
+if __name__ == "__main__":
+    case_ops("c0_ops", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
+    case_ops("s3_ops", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs, W.NEUMANN)
+    regrid_sequence()
+    c1_summary()
