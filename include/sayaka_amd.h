@@ -116,7 +116,9 @@ int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_s
 int sayaka_dev_fill(sayaka_dev_ctx* ctx, int field, double value);
 
 /* Launch timing per kernel class, with HIP events on the context stream (no reference
- * counterpart; used by the benchmark).  enable != 0 clears and starts recording. */
+ * counterpart; used by the benchmark).  enable != 0 clears and starts recording; reading
+ * returns the summed event time, the launch count, the cells covered and the algorithmic
+ * HBM bytes (each value the launches must read or write once; 0 for the bottom solve). */
 enum sayaka_kernel_class {
     SAYAKA_KCLASS_APPLY = 0,     /* FP64 operator / residual stencils (fused dot products) */
     SAYAKA_KCLASS_SMOOTH,        /* FP64 red-black Gauss-Seidel colour sweeps */
@@ -128,7 +130,7 @@ enum sayaka_kernel_class {
 };
 int sayaka_dev_profile(sayaka_dev_ctx* ctx, int kclass, int enable);
 int sayaka_dev_profile_read(sayaka_dev_ctx* ctx, int kclass, double* total_ms, long long* launches,
-                            long long* cells);
+                            long long* cells, double* alg_bytes);
 
 /* Block until the context's stream is idle. */
 int sayaka_dev_sync(sayaka_dev_ctx* ctx);

@@ -85,6 +85,7 @@ struct sayaka_dev_ctx {
     struct ProfRec {
         hipEvent_t a, b;
         long long cells;
+        double bytes;
     };
     bool prof_on[SAYAKA_KCLASS_N] = {};
     std::vector<ProfRec> prof[SAYAKA_KCLASS_N];
@@ -106,13 +107,16 @@ hipEvent_t pool_event(sayaka_dev_ctx* c) {
     return e;
 }
 
-// Brackets one launch of kernel class k with events when timing of that class is on.
+// Brackets one launch of kernel class k with events when timing of that class is on;
+// cells = cells the launch covers, bytes = its algorithmic HBM bytes (below).
 struct ProfScope {
     sayaka_dev_ctx* c;
     int k;
     long long cells;
+    double bytes;
     hipEvent_t a = nullptr;
-    ProfScope(sayaka_dev_ctx* c_, int k_, long long cells_) : c(c_), k(k_), cells(cells_) {
+    ProfScope(sayaka_dev_ctx* c_, int k_, long long cells_, double bytes_)
+        : c(c_), k(k_), cells(cells_), bytes(bytes_) {
         if (c->prof_on[k]) {
             a = pool_event(c);
             if (a) hipEventRecord(a, c->stream);
@@ -123,8 +127,46 @@ struct ProfScope {
         hipEvent_t b = pool_event(c);
         if (!b) return;
         hipEventRecord(b, c->stream);
-        c->prof[k].push_back({a, b, cells});
+        c->prof[k].push_back({a, b, cells, bytes});
     }
+};
+
+// Algorithmic HBM bytes per block of one launch: every value the operation must read or
+// write once (valid cells; face weights at (nb+1)/nb faces per cell and direction; ghosts,
+// index tables and reduction partials excluded).  C = one cell field, Fw = the weights.
+template <int D, int NB, class T>
+struct AlgBytes {
+    static constexpr double C = (double)Geo<D, NB>::NCELL * sizeof(T);
+    static constexpr double Fw = (double)D * Geo<D, NB>::NFACE * sizeof(T);
+    static double stencil(int mode) {
+        switch (mode) {
+            case ST_APPLY: return 3 * C + Fw;                // x, vf, w -> out
+            case ST_RESID: return 4 * C + Fw;                // + rhs
+            case ST_SMOOTH: return 0.5 * (4 * C + Fw);       // half of a full sweep (x rw, rhs, vf, w)
+            case ST_APPLY_DOT: return 4 * C + Fw;            // apply + rhat
+            case ST_APPLY_DOT2: return 4 * C + Fw;           // apply + s
+            case ST_RESID_INIT: return 6 * C + Fw;           // resid -> r, rhat, x = 0
+            default: return 0.0;
+        }
+    }
+    static double vec(int op, bool extra) {
+        switch (op) {
+            case V_PINIT: return (extra ? 5 : 4) * C;        // r, vf -> p, resid (, phat)
+            case V_PUPDATE: return (extra ? 7 : 6) * C;      // r, p, v, vf -> p, resid (, phat)
+            case V_SUPDATE: return (extra ? 9 : 8) * C;      // r, v, x, phat, vf -> s, x, resid (, shat)
+            case V_RUPDATE: return 7 * C;                    // x, shat, s, t, rhat -> x, r
+            case V_FINAL: return 3 * C;
+            case V_COPYPLUS0: return 2 * C;
+            case V_MASKCOPY: return 3 * C;
+            case V_ZERO: return C;
+            case V_DOT: return 2 * C;
+            default: return 0.0;
+        }
+    }
+    // per coarse (parent) block: 2^d children read x and vf, parent vf read and x written
+    static constexpr double restrict_() { return (1 << D) * 2 * C + 2 * C; }
+    // per fine block: parent value (1/2^d of a field), vf, write
+    static constexpr double prolong() { return 2 * C + C / (1 << D); }
 };
 
 RedBuf redbuf(sayaka_dev_ctx* c) { return RedBuf{c->d_partials, c->d_counter, c->d_scal, c->exact}; }
@@ -151,7 +193,8 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
     if (a.nlist <= 0) return 0;
     dim3 grid(a.nlist), block(G::THREADS);
     RedBuf rb = redbuf(c);
-    ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL);
+    ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL,
+                 a.nlist * AlgBytes<D, NB, double>::stencil(mode));
     switch (mode) {
         case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID>), grid, block, 0, c->stream, a, rb, fin); break;
@@ -169,7 +212,8 @@ template <int D, int NB>
 int launch_vec(sayaka_dev_ctx* c, ListRef lst, int op, const VecArgs& a, int fin) {
     using G = Geo<D, NB>;
     if (lst.n <= 0) return 0;
-    ProfScope ps(c, SAYAKA_KCLASS_VEC, (long long)lst.n * G::NCELL);
+    const bool extra = (op == V_PINIT || op == V_PUPDATE) ? a.phat != nullptr : (op == V_SUPDATE && a.dst != nullptr);
+    ProfScope ps(c, SAYAKA_KCLASS_VEC, (long long)lst.n * G::NCELL, lst.n * AlgBytes<D, NB, double>::vec(op, extra));
     hipLaunchKernelGGL((k_vec<G::NCELL, G::THREADS>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, a, op,
                        redbuf(c), fin);
     HIPCHK(hipGetLastError());
@@ -224,7 +268,8 @@ struct Impl {
         if (cl >= c->maxlevel || cl < 1) return 0;
         ListRef lst = c->lv_nonleaf[cl];
         if (lst.n <= 0 || c->lv_all[cl + 1].n <= 0) return 0;
-        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL);
+        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL * G::NCHILD,
+                     lst.n * AlgBytes<D, NB, double>::restrict_());
         hipLaunchKernelGGL((k_restrict<D, NB>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, dlist(c, lst),
                            c->d_binfo, c->d_vf, x);
         HIPCHK(hipGetLastError());
@@ -234,7 +279,7 @@ struct Impl {
         if (fl <= 1 || fl > c->maxlevel) return 0;
         ListRef lst = c->lv_all[fl];
         if (lst.n <= 0) return 0;
-        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL);
+        ProfScope ps(c, SAYAKA_KCLASS_TRANSFER, (long long)lst.n * G::NCELL, lst.n * AlgBytes<D, NB, double>::prolong());
         hipLaunchKernelGGL((k_prolong<D, NB>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, dlist(c, lst),
                            c->d_binfo, c->d_vf, x);
         HIPCHK(hipGetLastError());
@@ -256,7 +301,8 @@ struct Impl {
         A.max_iter = max_iter;
         A.exact = c->exact;
         void* args[] = {&A};
-        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL);
+        // iterative and latency-bound: no per-launch byte count
+        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
         HIPCHK(hipLaunchCooperativeKernel((const void*)k_bottom<D, NB>, dim3(lst.n), dim3(G::THREADS), args, 0,
                                           c->stream));
         return 0;
@@ -357,7 +403,8 @@ struct Impl {
                 a.x = u;
                 a.rhs = f;
                 a.colour = rb;
-                ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL);
+                ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL,
+                             sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH));
                 hipLaunchKernelGGL((k_stencil32<D, NB, ST_SMOOTH>), dim3(sub.n), dim3(G::THREADS), 0, c->stream, a);
             }
         }
@@ -999,10 +1046,11 @@ int sayaka_dev_profile(sayaka_dev_ctx* c, int kclass, int enable) {
     return 0;
 }
 
-int sayaka_dev_profile_read(sayaka_dev_ctx* c, int kclass, double* total_ms, long long* launches, long long* cells) {
+int sayaka_dev_profile_read(sayaka_dev_ctx* c, int kclass, double* total_ms, long long* launches, long long* cells,
+                            double* alg_bytes) {
     if (!c || kclass < 0 || kclass >= SAYAKA_KCLASS_N) return fail(-1, "bad argument");
     HIPCHK(hipStreamSynchronize(c->stream));
-    double t = 0.0;
+    double t = 0.0, by = 0.0;
     long long n = 0, cl = 0;
     for (auto& r : c->prof[kclass]) {
         float ms = 0.0f;
@@ -1010,10 +1058,12 @@ int sayaka_dev_profile_read(sayaka_dev_ctx* c, int kclass, double* total_ms, lon
         t += ms;
         n++;
         cl += r.cells;
+        by += r.bytes;
     }
     if (total_ms) *total_ms = t;
     if (launches) *launches = n;
     if (cells) *cells = cl;
+    if (alg_bytes) *alg_bytes = by;
     return 0;
 }
 

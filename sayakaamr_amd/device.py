@@ -55,7 +55,8 @@ _SIG = {
     "sayaka_dev_sync": ([vp], i32),
     "sayaka_dev_fill": ([vp, i32, f64], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
-    "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], i32),
+    "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                 C.POINTER(f64)], i32),
 }
 
 # kernel classes of sayaka_dev_profile (include/sayaka_amd.h)
@@ -225,5 +226,7 @@ class MacSolverGPU:
         t = f64()
         n = C.c_longlong()
         cells = C.c_longlong()
-        _chk(self.lib.sayaka_dev_profile_read(self.h, kclass, C.byref(t), C.byref(n), C.byref(cells)), "profile_read")
-        return dict(total_ms=t.value, launches=n.value, cells=cells.value)
+        by = f64()
+        _chk(self.lib.sayaka_dev_profile_read(self.h, kclass, C.byref(t), C.byref(n), C.byref(cells), C.byref(by)),
+             "profile_read")
+        return dict(total_ms=t.value, launches=n.value, cells=cells.value, alg_bytes=by.value)

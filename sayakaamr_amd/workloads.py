@@ -387,9 +387,12 @@ def c1_inputs(tree: dict, spec: TreeSpec, seed: int = 0, centre=(0.5, 0.5, 0.5),
     eps = 1.5 * dx_fine
     vf, af = sphere_fractions(tree, spec, centre, radius)
     b = density_coef(tree, spec, centre, radius, ratio, eps)
-    u = face_velocity(tree, spec, seed)
-    rhs = divergence_rhs(tree, spec, u, vf, af)
-    return dict(vf=vf, af=af, b=b, rhs=rhs)
+    return dict(vf=vf, af=af, b=b, rhs=c1_rhs(tree, spec, seed, vf, af))
+
+
+def c1_rhs(tree: dict, spec: TreeSpec, seed: int, vf, af):
+    """The seeded right-hand side alone (the geometry terms of c1_inputs are seed-free)."""
+    return divergence_rhs(tree, spec, face_velocity(tree, spec, seed), vf, af)
 
 
 def pin_cell(tree: dict, spec: TreeSpec, point=PIN_POINT):
