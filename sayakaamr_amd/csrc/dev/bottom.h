@@ -4,10 +4,24 @@
 // relative + 1e-6 absolute Linf residual, up to 2000 iterations, twice per outer
 // iteration (mg_solve_bottom -> cg_solve_level(1, ...), SayakaMacSolver.cpp:164-242,
 // 683-920).  Host round trips per iteration would dominate, so the whole loop runs in one
-// launch: one workgroup per level-1 block, every Krylov vector in registers, the two
-// vectors the operator reads across blocks (p and s) mirrored to global memory, and every
-// scalar reduced into all workgroups through grid barriers, so each workgroup evaluates the
-// same convergence and breakdown tests and leaves the loop at the same iteration.
+// launch.  Grid barriers are the cost that remains (about 1.7 us at 8 workgroups, 9.5 us at
+// 64 on MI355X), so the kernel is shaped to need few of them:
+//
+//  * BPW level-1 blocks per workgroup (4 for 8^3 blocks: 16 workgroups for a 4^3 root, the
+//    most blocks whose Krylov vectors still fit in registers), every vector in registers;
+//  * three barriers per iteration instead of five.  The operator needs p and s on the
+//    ghost cells (the neighbours' boundary cells).  Instead of publishing p and s and
+//    waiting for them, each workgroup recomputes them on its ghost cells from the values it
+//    has read anyway -- r and v are published at the barriers that carry the dot products,
+//    and p on the ghosts is kept from the previous iteration -- with the same expression
+//    and the same operands as the owning workgroup, so the ghost copies are bit-identical;
+//  * the half-step test on |s| travels with the t.s / t.t sums (t is computed before the
+//    test and discarded when it passes; x and the iteration count are unchanged).
+//
+// Reductions: exact mode reproduces the reference's order (DotProd, SayakaBlockData.cpp:
+// 185-208: cells k,j,i of a block in sequence; mg_dotprod_level, SayakaMacUtil.cpp:79-106:
+// block sums in tower order in sequence).  The per-block chains run concurrently in lanes
+// of one wave (one lane per block and sum), the chain over blocks after the barrier.
 #pragma once
 
 #include <hip/hip_cooperative_groups.h>
@@ -17,179 +31,404 @@
 namespace sydev {
 
 struct BottomArgs {
-    StencilArgs st;        // list = level-1 blocks, m = 1
+    StencilArgs st;        // list = level-1 blocks (tower 1), nlist of them, m = 1
     const double* rhs;     // RESID
-    double* sol;           // CORR (zeroed, written on convergence)
-    double* pg;            // mirror of p  [blk][NCELL]
-    double* sg;            // mirror of s
-    double* partials;      // 2 buffers x 3 values x gridDim.x
+    double* sol;           // CORR (written: zero, then the solution on convergence)
+    double* vpub;          // published v  [blk][NCELL]
+    double* rpub;          // published r
+    double* partials;      // 3 buffers x 3 values x nlist
     int* stats;            // [0] calls, [1] total iterations, [2] last iterations, [3] last status
     double tol_rel, tol_abs;
     int max_iter;
     int exact;             // reference-order sums (see RedBuf::exact)
 };
 
-// Two sums and a max over every cell of every workgroup, identical in all workgroups.
-template <int NT, int NCELL, int CPT>
-__device__ inline void coop_reduce(cooperative_groups::grid_group& grid, const double (&q0)[CPT],
-                                   const double (&q1)[CPT], int nsum, double& s0, double& s1, double& mx,
-                                   double* part, double* scratch, double* chunk, double* lds, int exact) {
-    double a0 = 0.0, a1 = 0.0;
-    if (exact) {
-        if (nsum > 0) a0 = seq_block_sum<NT, NCELL, CPT>(q0, scratch);
-        if (nsum > 1) a1 = seq_block_sum<NT, NCELL, CPT>(q1, scratch);
-        double z0 = 0.0, z1 = 0.0;
-        block_reduce3<NT>(z0, z1, mx, lds);
-    } else {
+template <int D, int NB>
+struct BotCfg {
+    using G = Geo<D, NB>;
+    static constexpr int BPW = G::NCELL >= 4096 ? 1 : 4;              // blocks per workgroup
+    static constexpr int WCELL = BPW * G::NCELL;
+    static constexpr int NT = WCELL < 512 ? WCELL : 512;
+    static constexpr int CPT = WCELL / NT;                             // cells per thread
+    static constexpr int WGHOST = BPW * G::NGHOST;
+    static constexpr int GPT = (WGHOST + NT - 1) / NT;                 // ghost slots per thread
+    static constexpr int NRED = 2 * BPW * (G::NCELL + 1);              // two sums staged
+    static constexpr int NTIL = BPW * G::NTILE;
+    // face weights of the workgroup's blocks staged in LDS when they fit beside the rest
+    static constexpr bool WLDS = BPW * D * G::NFACE * 8 <= 64 * 1024;
+    static constexpr int NWSH = WLDS ? BPW * D * G::NFACE : 1;
+    static constexpr int NSHARED = NRED > NTIL ? (NRED > 3 * NT ? NRED : 3 * NT) : (NTIL > 3 * NT ? NTIL : 3 * NT);
+    static_assert(WCELL % NT == 0, "cells per workgroup must be a multiple of the thread count");
+    static_assert((G::NCELL & (G::NCELL - 1)) == 0, "block cell count must be a power of two");
+};
+
+// Operator on one level-1 cell with the wall treatment folded into the staged data: on
+// Neumann wall faces the staged weight is 0, on Dirichlet / SimpleFill walls the ghost holds
+// -phi_c, so every face contributes off = w and offval = w * ghost.  Term by term (face
+// order, zeros added where the reference adds zeros, w * (-phi_c) == -w * phi_c) this is the
+// reference's apply (SayakaMacOperator.cpp:1409-1454) with m = 1, where every face is
+// same-level or a wall.
+template <int D, int NB>
+__device__ inline double bot_apply_cell(const double* tile, const double* wb, int i, int j, int k, double alpha,
+                                        double dv, double vfv) {
+    using G = Geo<D, NB>;
+    const double phic = tile[G::tidx(i, j, k)];
+    double diag = alpha * dv, offd = 0.0;
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            a0 += q0[q];
-            a1 += q1[q];
-        }
-        block_reduce3<NT>(a0, a1, mx, lds);
+    for (int f = 0; f < G::NF; f++) {
+        const int d = f >> 1;
+        const int fi = G::fidx(d, i + (f == 1), j + (f == 3), k + (f == 5));
+        const int ni = G::tidx(i + (f == 1) - (f == 0), j + (f == 3) - (f == 2), k + (f == 5) - (f == 4));
+        const double off = wb[d * G::NFACE + fi];
+        const double offval = off * tile[ni];
+        diag += off;
+        offd += offval;
     }
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = a0;
-        part[3 * blockIdx.x + 1] = a1;
-        part[3 * blockIdx.x + 2] = mx;
-    }
-    grid.sync();
-    double ga, gb, gm;
-    sum_published<NT>(part, gridDim.x, exact != 0, ga, gb, gm, chunk, lds);
-    __shared__ double bc[3];
-    if (threadIdx.x == 0) {
-        bc[0] = ga;
-        bc[1] = gb;
-        bc[2] = gm;
-    }
-    __syncthreads();
-    s0 = bc[0];
-    s1 = bc[1];
-    mx = bc[2];
-    __syncthreads();
+    const double Lphi = (phic * diag - offd) / dv;
+    return vfv > 0 ? Lphi / vfv : 0.0;
 }
 
 template <int D, int NB>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_bottom(BottomArgs A) {
+struct Bottom {
     using G = Geo<D, NB>;
-    constexpr int NT = G::THREADS;
-    constexpr int CPT = (G::NCELL + NT - 1) / NT;
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
-    __shared__ double tile[G::NTILE];
-    __shared__ double lds[3 * ((NT + 63) / 64)];
-    __shared__ double chunk[3 * NT];
-    const StencilArgs& a = A.st;
-    const int b = a.list[blockIdx.x];
-    __shared__ BlockInfo bi;
-    if (threadIdx.x == 0) bi = a.binfo[b];
-    __syncthreads();
-    const double dv = a.met[b].dv;
-    double* partA = A.partials;
-    double* partB = A.partials + 3 * gridDim.x;
-    int pb = 0;
-    auto part = [&]() { double* p = pb ? partB : partA; pb ^= 1; return p; };
+    using C = BotCfg<D, NB>;
+    static constexpr int BPW = C::BPW, NT = C::NT, CPT = C::CPT, GPT = C::GPT;
 
-    double x[CPT], r[CPT], rh[CPT], p[CPT], v[CPT], s[CPT], t[CPT], vfv[CPT], q0[CPT], q1[CPT];
-    // x0 = 0 (CORR zeroed by mg_solve_bottom), r = rhs - A*0 = rhs on fluid cells
-    double s0 = 0.0, s1 = 0.0, mx = 0.0;
-#pragma unroll
-    for (int q = 0; q < CPT; q++) {
-        const int c = threadIdx.x + q * NT;
-        x[q] = r[q] = rh[q] = p[q] = v[q] = s[q] = t[q] = q0[q] = q1[q] = 0.0;
-        vfv[q] = 0.0;
-        if (c < G::NCELL) {
-            const size_t gi = (size_t)b * G::NCELL + c;
-            A.sol[gi] = 0.0;
-            vfv[q] = a.vf[gi];
-            r[q] = vfv[q] > 0 ? A.rhs[gi] - 0.0 : 0.0;
-            rh[q] = r[q];
-            q0[q] = rh[q] * r[q];
-            mx = fmax(mx, fabs(r[q]));
+    // Stage one per-cell term of sum `which` (0, 1) for block blk, cell c.  Exact mode keeps
+    // it in LDS for the ordered chains; fast mode adds it to the thread's running sum.
+    __device__ static void stage(double* red, int exact, int which, int blk, int c, double val, double& acc) {
+        if (exact) {
+            stage_chain(red, G::NCELL, which * BPW + blk, c, val);
+        } else {
+            acc += val;
         }
     }
-    coop_reduce<NT, G::NCELL, CPT>(grid, q0, q1, 1, s0, s1, mx, part(), tile, chunk, lds, A.exact);
+
+    // Per-block sums of the staged terms (nsum of them) for the blocks of this workgroup,
+    // published with the workgroup's max; a grid barrier; then every workgroup reads all
+    // partials and forms the totals (in block order when exact).  a0/a1 are the thread's
+    // running sums in fast mode.
+    __device__ static void reduce(cooperative_groups::grid_group& grid, const BottomArgs& A, double a0, double a1,
+                                  int nsum, double mxl, double* part, double* red, double* wl, int nvalid,
+                                  double& s0, double& s1, double& mx) {
+        const int w0 = blockIdx.x * BPW;
+        const int nl = A.st.nlist;
+        mxl = wave_max(mxl);
+        if ((threadIdx.x & 63) == 0) wl[threadIdx.x >> 6] = mxl;
+        if (A.exact) {
+            __syncthreads();
+            const int l = threadIdx.x;
+            if (l < BPW * nsum) {
+                const double sv = lane_chain<G::NCELL>(red, l);
+                const int blk = l % BPW;
+                if (blk < nvalid) part[3 * (w0 + blk) + l / BPW] = sv;
+            }
+        } else {
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            if ((threadIdx.x & 63) == 0) {
+                wl[16 + (threadIdx.x >> 6)] = a0;
+                wl[32 + (threadIdx.x >> 6)] = a1;
+            }
+            __syncthreads();
+            if (threadIdx.x < BPW && (int)threadIdx.x < nvalid) {
+                double b0 = 0.0, b1 = 0.0;
+                if (threadIdx.x == 0)
+                    for (int w = 0; w < NT / 64; w++) {
+                        b0 += wl[16 + w];
+                        b1 += wl[32 + w];
+                    }
+                part[3 * (w0 + threadIdx.x)] = b0;
+                part[3 * (w0 + threadIdx.x) + 1] = b1;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < BPW && (int)threadIdx.x < nvalid) {
+            double m = 0.0;
+            for (int w = 0; w < NT / 64; w++) m = fmax(m, wl[w]);
+            part[3 * (w0 + threadIdx.x) + 2] = m;
+        }
+        grid.sync();
+        // totals: stage the partials in LDS (parallel loads; the max reduced on the way),
+        // chain the sums in lanes 0 / 1 (every workgroup, same order, same result)
+        double acc = 0.0, tm = 0.0;
+        for (int base = 0; base < nl; base += NT) {
+            const int cnt = nl - base < NT ? nl - base : NT;
+            const int cnt8 = (cnt + 7) & ~7;
+            for (int g = threadIdx.x; g < cnt8; g += NT) {
+                const bool ok = g < cnt;
+                red[g] = ok ? load_published(part + 3 * (base + g)) : 0.0;
+                red[NT + g] = ok && nsum > 1 ? load_published(part + 3 * (base + g) + 1) : 0.0;
+                if (ok) tm = fmax(tm, load_published(part + 3 * (base + g) + 2));
+            }
+            __syncthreads();
+            if ((int)threadIdx.x < (nsum > 1 ? 2 : 1)) acc = chain_n(red + threadIdx.x * NT, cnt8, acc);
+            __syncthreads();
+        }
+        tm = wave_max(tm);
+        if ((threadIdx.x & 63) == 0) wl[52 + (threadIdx.x >> 6)] = tm;
+        if (threadIdx.x < 2) wl[48 + threadIdx.x] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int w = 0; w < NT / 64; w++) m = fmax(m, wl[52 + w]);
+            wl[50] = m;
+        }
+        __syncthreads();
+        s0 = wl[48];
+        s1 = wl[49];
+        mx = wl[50];
+        __syncthreads();
+    }
+};
+
+template <int D, int NB>
+__global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
+    using G = Geo<D, NB>;
+    using C = BotCfg<D, NB>;
+    using B = Bottom<D, NB>;
+    constexpr int BPW = C::BPW, NT = C::NT, CPT = C::CPT, GPT = C::GPT;
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ double sh[C::NSHARED];   // operand tiles, then reduction staging
+    __shared__ double wl[64];
+    __shared__ BlockInfo bi[BPW];
+    __shared__ double dvs[BPW];
+    __shared__ int gbs[BPW];            // global block id of each block of the workgroup
+    __shared__ int gsrc[C::WGHOST];     // ghost slot -> source cell (global index); -1 zero (Neumann
+                                        // wall / none); <= -2: Dirichlet wall, -(2 + interior tile index)
+    __shared__ int gtile[C::WGHOST];    // ghost slot -> tile position
+    __shared__ double rh[C::WCELL];     // rhat (constant over the iterations)
+    __shared__ double vfs[C::WCELL];    // volume fractions of the workgroup's cells
+    __shared__ double wsh[C::NWSH];     // face weights of the workgroup's blocks (when WLDS)
+    const StencilArgs& a = A.st;
+    const int w0 = blockIdx.x * BPW;
+    const int nvalid = a.nlist - w0 < BPW ? a.nlist - w0 : BPW;
+    const int ex = A.exact;
+    if ((int)threadIdx.x < BPW) {
+        const int q = threadIdx.x;
+        gbs[q] = -1;
+        if (q < nvalid) {
+            const int b = a.list[w0 + q];
+            gbs[q] = b;
+            bi[q] = a.binfo[b];
+            dvs[q] = a.met[b].dv;
+        }
+    }
+    __syncthreads();   // gbs / bi / dvs visible to every thread before they are used below
+    if (C::WLDS)
+        for (int e = threadIdx.x; e < C::NWSH; e += NT) {
+            const int blk = e / (D * G::NFACE), rem = e % (D * G::NFACE);
+            double wv = 0.0;
+            if (blk < nvalid) {
+                wv = a.w[gbs[blk] * (D * G::NFACE) + rem];
+                // zero on Neumann wall faces (their face is on the block boundary)
+                const int d = rem / G::NFACE, fi = rem % G::NFACE;
+                const int fx = G::fx(d), fy = G::fy(d);
+                const int cq[3] = {fi % fx, (fi / fx) % fy, fi / (fx * fy)};
+                const int ext = d == 2 ? G::NBZ : NB;
+                const int f = cq[d] == 0 ? 2 * d : (cq[d] == ext ? 2 * d + 1 : -1);
+                if (f >= 0 && bi[blk].nbr[f] <= PHYS_BNDRY && bi[blk].bct[f] == BC_NEUMANN) wv = 0.0;
+            }
+            wsh[e] = wv;
+        }
+    for (int e = threadIdx.x; e < C::WCELL; e += NT) {
+        const int blk = e / G::NCELL;
+        vfs[e] = blk < nvalid ? a.vf[gbs[blk] * G::NCELL + e % G::NCELL] : 0.0;
+    }
+    __syncthreads();
+    double* partA = A.partials;
+    double* partB = A.partials + 3 * a.nlist;
+    double* partC = A.partials + 6 * a.nlist;
+    // own cells: e = tid + q*NT -> block e / NCELL of this workgroup, cell e % NCELL
+    auto blk_of = [&](int q) { return (int)(threadIdx.x + q * NT) / G::NCELL; };
+    auto cell_of = [&](int q) { return (int)(threadIdx.x + q * NT) % G::NCELL; };
+    auto gidx = [&](int q) { return gbs[blk_of(q)] * G::NCELL + cell_of(q); };
+
+    double x[CPT], r[CPT], p[CPT], v[CPT], s[CPT], t[CPT];
+    double pG[GPT], vG[GPT], rG[GPT];   // ghost slot g = tid + u*NT
+
+    double mxl = 0.0, a0 = 0.0, a1 = 0.0;
+    int has_dir = 0;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        x[q] = r[q] = p[q] = v[q] = s[q] = t[q] = 0.0;
+        const int blk = blk_of(q);
+        if (blk < nvalid) {
+            const int gi = gidx(q);
+            A.sol[gi] = 0.0;
+            // r = rhs - A*x0 with x0 = 0 (mg_residual_level: 0 on solid cells)
+            r[q] = a.vf[gi] > 0 ? A.rhs[gi] - 0.0 : 0.0;
+            mxl = fmax(mxl, fabs(r[q]));
+        }
+        rh[threadIdx.x + q * NT] = r[q];
+        B::stage(sh, ex, 0, blk, cell_of(q), r[q] * r[q], a0);
+    }
+#pragma unroll
+    for (int u = 0; u < GPT; u++) {
+        const int g = threadIdx.x + u * NT;
+        pG[u] = vG[u] = rG[u] = 0.0;
+        if (g >= C::WGHOST) continue;
+        const int blk = g / G::NGHOST, gg = g % G::NGHOST;
+        const int f = gg / G::NSLAB, pp = gg % G::NSLAB;
+        const int d = f >> 1, sd = f & 1;
+        int a1_, a2_;
+        tang_axes<D>(d, a1_, a2_);
+        const int ext = d == 2 ? G::NBZ : NB;
+        int qc[3] = {0, 0, 0};
+        qc[a1_] = pp % NB;
+        if (D == 3) qc[a2_] = pp / NB;
+        int tq[3] = {qc[0], qc[1], qc[2]};
+        tq[d] = sd ? ext : -1;
+        gtile[g] = blk * G::NTILE + G::tidx(tq[0], tq[1], tq[2]);
+        int src = -1;
+        if (blk < nvalid) {
+            const int n = bi[blk].nbr[f];   // level-1 faces are same-level or walls
+            if (n >= 0) {
+                qc[d] = sd ? 0 : ext - 1;
+                src = n * G::NCELL + G::cidx(qc[0], qc[1], qc[2]);
+                rG[u] = a.vf[src] > 0 ? A.rhs[src] - 0.0 : 0.0;
+            } else if (C::WLDS && n <= PHYS_BNDRY && bi[blk].bct[f] != BC_NEUMANN) {
+                int iq[3] = {qc[0], qc[1], qc[2]};
+                iq[d] = sd ? ext - 1 : 0;
+                src = -2 - (blk * G::NTILE + G::tidx(iq[0], iq[1], iq[2]));
+                has_dir = 1;
+            }
+        }
+        gsrc[g] = src;
+    }
+    const int any_dir = __syncthreads_or(has_dir);
+    double s0, s1, mx;
+    B::reduce(grid, A, a0, a1, 1, mxl, partC, sh, wl, nvalid, s0, s1, mx);
     const double rnorm0 = mx;
     double rho = s0, rho1 = 0.0, alpha = 0.0, omega = 0.0;
     int iter = 0, conv = 0, stat = 0;
+
+    // operand tile of the workgroup's blocks from own registers + ghost registers
+    auto fill_tile = [&](const double (&own)[CPT], const double (&gh)[GPT]) {
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            int i, j, k;
+            G::cell_ijk(cell_of(q), i, j, k);
+            sh[blk_of(q) * G::NTILE + G::tidx(i, j, k)] = own[q];
+        }
+#pragma unroll
+        for (int u = 0; u < GPT; u++) {
+            const int g = threadIdx.x + u * NT;
+            if (g < C::WGHOST && gsrc[g] >= -1) sh[gtile[g]] = gsrc[g] >= 0 ? gh[u] : 0.0;
+        }
+        __syncthreads();
+        if (any_dir) {   // Dirichlet / SimpleFill wall ghosts = -phi_c of the adjacent cell
+#pragma unroll
+            for (int u = 0; u < GPT; u++) {
+                const int g = threadIdx.x + u * NT;
+                if (g < C::WGHOST && gsrc[g] <= -2) sh[gtile[g]] = -sh[-2 - gsrc[g]];
+            }
+            __syncthreads();
+        }
+    };
+    auto apply = [&](double (&out)[CPT]) {
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            out[q] = 0.0;
+            const int blk = blk_of(q);
+            if (blk >= nvalid) continue;
+            int i, j, k;
+            G::cell_ijk(cell_of(q), i, j, k);
+            if (C::WLDS)
+                out[q] = bot_apply_cell<D, NB>(sh + blk * G::NTILE, wsh + blk * (D * G::NFACE), i, j, k, a.alpha,
+                                               dvs[blk], vfs[threadIdx.x + q * NT]);
+            else
+                out[q] = cell_op<D, NB, false, double>(sh + blk * G::NTILE, a.w + gbs[blk] * (D * G::NFACE),
+                                                       a.wcf + gbs[blk] * (G::NF * G::NSLAB), i, j, k, bi[blk], 1,
+                                                       a, dvs[blk], vfs[threadIdx.x + q * NT], 0.0);
+        }
+        __syncthreads();   // tile free for the reduction staging
+    };
+    auto load_ghosts = [&](double (&dst)[GPT], const double* src) {
+#pragma unroll
+        for (int u = 0; u < GPT; u++) {
+            const int g = threadIdx.x + u * NT;
+            if (g < C::WGHOST && gsrc[g] >= 0) dst[u] = src[gsrc[g]];
+        }
+    };
+
     if (rnorm0 != 0.0) {
         for (iter = 1; iter <= A.max_iter; iter++) {
             if (rho == 0.0) { stat = 1; break; }
             const double beta = iter == 1 ? 0.0 : (rho / rho1) * (alpha / omega);
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = threadIdx.x + q * NT;
                 if (iter == 1) p[q] = r[q];
                 else {
                     const double p1 = 1.0 * p[q] + -omega * v[q];
                     p[q] = 1.0 * r[q] + beta * p1;
                 }
-                if (c < G::NCELL) A.pg[(size_t)b * G::NCELL + c] = p[q];
             }
-            grid.sync();
-            load_tile<D, NB, double>(tile, A.pg, bi, b);
-            __syncthreads();
-            mx = 0.0;
 #pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int c = threadIdx.x + q * NT;
-                q0[q] = 0.0;
-                if (c < G::NCELL) {
-                    int i, j, k;
-                    G::cell_ijk(c, i, j, k);
-                    v[q] = cell_op<D, NB, false, double>(tile, b, i, j, k, bi, 1, a, dv, vfv[q], 0.0);
-                    q0[q] = rh[q] * v[q];
+            for (int u = 0; u < GPT; u++) {
+                if (iter == 1) pG[u] = rG[u];
+                else {
+                    const double p1 = 1.0 * pG[u] + -omega * vG[u];
+                    pG[u] = 1.0 * rG[u] + beta * p1;
                 }
             }
-            coop_reduce<NT, G::NCELL, CPT>(grid, q0, q1, 1, s0, s1, mx, part(), tile, chunk, lds, A.exact);
+            // v = A p ; alpha_den = rhat . v  (barrier A, publishes v)
+            fill_tile(p, pG);
+            apply(v);
+            a0 = 0.0;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int blk = blk_of(q);
+                B::stage(sh, ex, 0, blk, cell_of(q), rh[threadIdx.x + q * NT] * v[q], a0);
+                if (blk < nvalid) A.vpub[gidx(q)] = v[q];
+            }
+            B::reduce(grid, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0, s1, mx);
+            load_ghosts(vG, A.vpub);
             if (s0 == 0.0) { stat = 1; break; }
             alpha = rho / s0;
-            mx = 0.0;
+            // s = r - alpha v ; x += alpha p ; t = A s ; |s|, t.s, t.t  (barrier B)
+            mxl = 0.0;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = threadIdx.x + q * NT;
                 s[q] = 1.0 * r[q] + -alpha * v[q];
                 x[q] = 1.0 * x[q] + alpha * p[q];
-                q0[q] = 0.0;
-                if (c < G::NCELL) {
-                    A.sg[(size_t)b * G::NCELL + c] = s[q];
-                    mx = fmax(mx, fabs(s[q]));
-                }
+                if (blk_of(q) < nvalid) mxl = fmax(mxl, fabs(s[q]));
             }
-            coop_reduce<NT, G::NCELL, CPT>(grid, q0, q1, 0, s0, s1, mx, part(), tile, chunk, lds, A.exact);
-            if (mx < rnorm0 * A.tol_rel || mx < A.tol_abs) { conv = 1; break; }
-            load_tile<D, NB, double>(tile, A.sg, bi, b);
-            __syncthreads();
-            mx = 0.0;
+            {
+                double sG[GPT];
+#pragma unroll
+                for (int u = 0; u < GPT; u++) sG[u] = 1.0 * rG[u] + -alpha * vG[u];
+                fill_tile(s, sG);
+            }
+            apply(t);
+            a0 = a1 = 0.0;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = threadIdx.x + q * NT;
-                q0[q] = q1[q] = 0.0;
-                if (c < G::NCELL) {
-                    int i, j, k;
-                    G::cell_ijk(c, i, j, k);
-                    t[q] = cell_op<D, NB, false, double>(tile, b, i, j, k, bi, 1, a, dv, vfv[q], 0.0);
-                    q0[q] = t[q] * s[q];
-                    q1[q] = t[q] * t[q];
-                }
+                B::stage(sh, ex, 0, blk_of(q), cell_of(q), t[q] * s[q], a0);
+                B::stage(sh, ex, 1, blk_of(q), cell_of(q), t[q] * t[q], a1);
             }
-            double ts = 0.0, tt = 0.0;
-            coop_reduce<NT, G::NCELL, CPT>(grid, q0, q1, 2, ts, tt, mx, part(), tile, chunk, lds, A.exact);
+            double ts, tt;
+            B::reduce(grid, A, a0, a1, 2, mxl, partB, sh, wl, nvalid, ts, tt, mx);
+            if (mx < rnorm0 * A.tol_rel || mx < A.tol_abs) { conv = 1; break; }
             if (tt == 0.0) { stat = 1; break; }
             omega = ts / tt;
-            mx = 0.0;
+            // x += omega s ; r = s - omega t ; |r|, rhat . r  (barrier C, publishes r)
+            mxl = 0.0;
+            a0 = 0.0;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = threadIdx.x + q * NT;
                 x[q] = 1.0 * x[q] + omega * s[q];
                 r[q] = 1.0 * s[q] + -omega * t[q];
-                q0[q] = 0.0;
-                if (c < G::NCELL) {
-                    mx = fmax(mx, fabs(r[q]));
-                    q0[q] = rh[q] * r[q];
+                const int blk = blk_of(q);
+                B::stage(sh, ex, 0, blk, cell_of(q), rh[threadIdx.x + q * NT] * r[q], a0);
+                if (blk < nvalid) {
+                    mxl = fmax(mxl, fabs(r[q]));
+                    A.rpub[gidx(q)] = r[q];
                 }
             }
-            coop_reduce<NT, G::NCELL, CPT>(grid, q0, q1, 1, s0, s1, mx, part(), tile, chunk, lds, A.exact);
+            B::reduce(grid, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0, s1, mx);
+            load_ghosts(rG, A.rpub);
             if (mx < rnorm0 * A.tol_rel + A.tol_abs) { conv = 1; break; }
             if (omega == 0.0) { stat = 1; break; }
             rho1 = rho;
@@ -199,10 +438,8 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_bottom(BottomArgs A) 
     }
     if (conv) {
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int c = threadIdx.x + q * NT;
-            if (c < G::NCELL) A.sol[(size_t)b * G::NCELL + c] = 0.0 + x[q];
-        }
+        for (int q = 0; q < CPT; q++)
+            if (blk_of(q) < nvalid) A.sol[gidx(q)] = 0.0 + x[q];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.stats[0] += 1;

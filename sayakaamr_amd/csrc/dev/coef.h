@@ -44,9 +44,9 @@ __global__ void k_sync_fine(const int* list, const BlockInfo* binfo, double* fda
                     fq[d] = s ? ext : 0;
                     fq[a1] = 2 * u1 + q1;
                     if (D == 3) fq[a2] = 2 * u2 + q2;
-                    val += fdata[((size_t)b * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
+                    val += fdata[(b * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
                 }
-            fdata[((size_t)p * D + d) * G::NFACE + G::fidx(d, cq[0], cq[1], cq[2])] = val * scale;
+            fdata[(p * D + d) * G::NFACE + G::fidx(d, cq[0], cq[1], cq[2])] = val * scale;
         }
     }
 }
@@ -72,8 +72,8 @@ __global__ void k_sync_crse(const int* list, const BlockInfo* binfo, double* fda
             if (D == 3) o[a2] = r[a2] = q / NB;
             o[d] = s ? ext : 0;
             r[d] = s ? 0 : ext;
-            fdata[((size_t)b * D + d) * G::NFACE + G::fidx(d, o[0], o[1], o[2])] =
-                fdata[((size_t)n * D + d) * G::NFACE + G::fidx(d, r[0], r[1], r[2])];
+            fdata[(b * D + d) * G::NFACE + G::fidx(d, o[0], o[1], o[2])] =
+                fdata[(n * D + d) * G::NFACE + G::fidx(d, r[0], r[1], r[2])];
         }
     }
 }
@@ -108,9 +108,9 @@ __global__ void k_restrict_face(const int* list, const BlockInfo* binfo, double*
                 fq[d] = (c[d] - off[d]) * 2;
                 fq[a1] = (c[a1] - off[a1]) * 2 + q1;
                 if (D == 3) fq[a2] = (c[a2] - off[a2]) * 2 + q2;
-                val += fdata[((size_t)fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
+                val += fdata[(fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
             }
-        fdata[((size_t)b * D + d) * G::NFACE + q] = val * scale;
+        fdata[(b * D + d) * G::NFACE + q] = val * scale;
     }
 }
 
@@ -131,8 +131,8 @@ __global__ void k_restrict_cell_avg(const int* list, const BlockInfo* binfo, dou
         double sum = 0.0;
         for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
             for (int jj = 0; jj <= 1; jj++)
-                for (int ii = 0; ii <= 1; ii++) sum += x[(size_t)fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk)];
-        x[(size_t)b * G::NCELL + c] = sum * wgt;
+                for (int ii = 0; ii <= 1; ii++) sum += x[fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk)];
+        x[b * G::NCELL + c] = sum * wgt;
     }
 }
 
@@ -157,7 +157,7 @@ __global__ void k_weights(const int* list, const BlockInfo* binfo, const BlockMe
             c[0] = q % G::fx(d);
             c[1] = (q / G::fx(d)) % G::fy(d);
             c[2] = q / (G::fx(d) * G::fy(d));
-            const size_t fi = ((size_t)b * D + d) * G::NFACE + q;
+            const int fi = (b * D + d) * G::NFACE + q;
             const double B = bco[fi], A = af[fi];
             const double dsd = mt.ds[d], dxd = mt.dx[d];
             double off;
@@ -173,7 +173,7 @@ __global__ void k_weights(const int* list, const BlockInfo* binfo, const BlockMe
                     oc *= B;
                     oc *= A;
                     const int slab = c[a1] + NB * (D == 3 ? c[a2] : 0);
-                    wcf[((size_t)b * G::NF + f) * G::NSLAB + slab] = oc;
+                    wcf[(b * G::NF + f) * G::NSLAB + slab] = oc;
                 }
             } else {
                 off = beta * dsd / dxd;
@@ -197,7 +197,7 @@ __global__ void k_fix_bc(const int* list, const BlockInfo* binfo, const BlockMet
     __syncthreads();
     const BlockMetric mt = met[b];
     const int lx = NB + 2 * ng, ly = NB + 2 * ng, kg = D == 3 ? ng : 0, lz = G::NBZ + 2 * kg;
-    const size_t gbase = (size_t)b * lx * ly * lz;
+    const int gbase = b * lx * ly * lz;
     auto P = [&](int i, int j, int k) { return phi[gbase + (i + ng) + lx * ((j + ng) + ly * (k + kg))]; };
     for (int f = 0; f < G::NF; f++) {
         if (bi.nbr[f] > PHYS_BNDRY) continue;
@@ -217,7 +217,7 @@ __global__ void k_fix_bc(const int* list, const BlockInfo* binfo, const BlockMet
             fq[d] += s;
             const double phi0 = P(c[0], c[1], c[2]);
             const double phi1 = P(g[0], g[1], g[2]);
-            const size_t fi = ((size_t)b * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2]);
+            const int fi = (b * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2]);
             double coef = beta * mt.ds[d] / mt.dx[d];
             coef *= bco[fi];
             coef *= af[fi];
@@ -228,7 +228,7 @@ __global__ void k_fix_bc(const int* list, const BlockInfo* binfo, const BlockMet
             } else if (bct == BC_SIMPLEFILL) {
                 corr = phi1 * 2.0 * coef;
             }
-            const size_t ci = (size_t)b * G::NCELL + G::cidx(c[0], c[1], c[2]);
+            const int ci = b * G::NCELL + G::cidx(c[0], c[1], c[2]);
             const double v = vf[ci];
             if (v > 0) {
                 corr /= (mt.dv * v);

@@ -78,30 +78,72 @@ struct RedBuf {
                         //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
 };
 
-// Sum, in cell order, of per-cell values held by the threads (cell c = tid + q*NT);
-// `scratch` is an LDS buffer of at least NCELL doubles.  Result valid in thread 0.
+// Lane l of the calling wave sums the N values staged for chain l, in index order.  Chain
+// l occupies red[l*(N+1) .. l*(N+1)+N) (the one-value pad puts concurrent chains on
+// distinct LDS banks).  A sequential FP64 chain is bound by the dependent-add latency, so
+// the next 8 values are loaded while the current 8 are added.
+template <int N>
+__device__ inline double lane_chain(const double* red, int l) {
+    static_assert(N % 16 == 0, "chain length must be a multiple of 16");
+    const double* p = red + l * (N + 1);
+    double s = 0.0;
+    double a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5], a6 = p[6], a7 = p[7];
+#pragma unroll 1
+    for (int c = 0; c < N; c += 16) {
+        const double* q = p + c;
+        const double b0 = q[8], b1 = q[9], b2 = q[10], b3 = q[11], b4 = q[12], b5 = q[13], b6 = q[14], b7 = q[15];
+        s += a0; s += a1; s += a2; s += a3; s += a4; s += a5; s += a6; s += a7;
+        if (c + 16 < N) {
+            a0 = q[16]; a1 = q[17]; a2 = q[18]; a3 = q[19]; a4 = q[20]; a5 = q[21]; a6 = q[22]; a7 = q[23];
+        }
+        s += b0; s += b1; s += b2; s += b3; s += b4; s += b5; s += b6; s += b7;
+    }
+    return s;
+}
+
+// Sequential sum of n (a multiple of 8) contiguous LDS values onto acc, loads one batch ahead.
+__device__ inline double chain_n(const double* p, int n, double acc) {
+    if (n <= 0) return acc;
+    double a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5], a6 = p[6], a7 = p[7];
+#pragma unroll 1
+    for (int c = 0; c < n; c += 8) {
+        const double* q = p + c;
+        double b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0, b6 = 0, b7 = 0;
+        if (c + 8 < n) {
+            b0 = q[8]; b1 = q[9]; b2 = q[10]; b3 = q[11]; b4 = q[12]; b5 = q[13]; b6 = q[14]; b7 = q[15];
+        }
+        acc += a0; acc += a1; acc += a2; acc += a3; acc += a4; acc += a5; acc += a6; acc += a7;
+        a0 = b0; a1 = b1; a2 = b2; a3 = b3; a4 = b4; a5 = b5; a6 = b6; a7 = b7;
+    }
+    return acc;
+}
+
+__device__ inline void stage_chain(double* red, int n, int l, int c, double v) { red[l * (n + 1) + c] = v; }
+
+// Sums, in cell order, of nsum (1 or 2) per-cell values held by the threads (cell
+// c = tid + q*NT); the two chains run concurrently in lanes 0 and 1.  `scratch` is LDS of
+// at least 2*NCELL+2 doubles.  Results valid in thread 0.
 template <int NT, int NCELL, int CPT>
-__device__ inline double seq_block_sum(const double (&v)[CPT], double* scratch) {
+__device__ inline void seq_block_sums(const double (&v0)[CPT], const double (&v1)[CPT], int nsum, double* scratch,
+                                      double& s0, double& s1) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const int c = threadIdx.x + q * NT;
-        if (c < NCELL) scratch[c] = v[q];
-    }
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < NCELL; c += 8) {
-            double t[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) t[u] = c + u < NCELL ? scratch[c + u] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (c + u < NCELL) s += t[u];
+        if (c < NCELL) {
+            stage_chain(scratch, NCELL, 0, c, v0[q]);
+            if (nsum > 1) stage_chain(scratch, NCELL, 1, c, v1[q]);
         }
     }
     __syncthreads();
-    return s;
+    double r = 0.0;
+    if ((int)threadIdx.x < nsum) r = lane_chain<NCELL>(scratch, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 1) scratch[0] = r;
+    __syncthreads();
+    s0 = r;
+    s1 = nsum > 1 ? scratch[0] : 0.0;
+    __syncthreads();
 }
 
 // Coherent load of a value another workgroup published before a device-scope fence.
@@ -109,38 +151,43 @@ __device__ inline double load_published(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sum of n published triples (sum, sum, max) into every thread's (a, b, m): exact mode sums
-// in index order (thread 0 after staging chunks in LDS), fast mode by a strided tree.
+// Sum of n published triples (sum, sum, max): exact mode sums in index order (chunks staged
+// in LDS by all threads, chained in lane 0 / lane 1), fast mode by a strided tree.  The max
+// is order-free.  Results valid in thread 0.
 template <int NT>
-__device__ inline void sum_published(const double* part, unsigned n, bool exact, double& a, double& b, double& m,
-                                     double* chunk /* 3*NT doubles of LDS */, double* lds) {
+__device__ inline void sum_published(const double* part, unsigned n, bool exact, int nsum, double& a, double& b,
+                                     double& m, double* chunk /* 2*NT doubles of LDS */, double* lds) {
+    static_assert(NT % 8 == 0, "chunk must be a multiple of 8");
     a = 0.0;
     b = 0.0;
     m = 0.0;
     if (exact) {
+        double ml = 0.0, acc = 0.0;
         for (unsigned base = 0; base < n; base += NT) {
-            const unsigned g = base + threadIdx.x;
-            if (g < n) {
-                chunk[3 * threadIdx.x] = load_published(part + 3 * (size_t)g);
-                chunk[3 * threadIdx.x + 1] = load_published(part + 3 * (size_t)g + 1);
-                chunk[3 * threadIdx.x + 2] = load_published(part + 3 * (size_t)g + 2);
+            const unsigned cnt = n - base < (unsigned)NT ? n - base : (unsigned)NT;
+            const unsigned cnt8 = (cnt + 7u) & ~7u;
+            const unsigned g = threadIdx.x;
+            if (g < cnt8) {
+                const bool ok = g < cnt;
+                chunk[g] = ok ? load_published(part + 3 * (base + g)) : 0.0;
+                chunk[NT + g] = ok && nsum > 1 ? load_published(part + 3 * (base + g) + 1) : 0.0;
+                if (ok) ml = fmax(ml, load_published(part + 3 * (base + g) + 2));
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                const unsigned cnt = n - base < (unsigned)NT ? n - base : (unsigned)NT;
-                for (unsigned q = 0; q < cnt; q++) {
-                    a += chunk[3 * q];
-                    b += chunk[3 * q + 1];
-                    m = fmax(m, chunk[3 * q + 2]);
-                }
-            }
+            if ((int)threadIdx.x < nsum) acc = chain_n(chunk + threadIdx.x * NT, (int)cnt8, acc);
             __syncthreads();
         }
+        if (threadIdx.x == 1) chunk[0] = acc;
+        double z0 = 0.0, z1 = 0.0;
+        block_reduce3<NT>(z0, z1, ml, lds);   // contains barriers
+        a = acc;
+        b = nsum > 1 ? chunk[0] : 0.0;
+        m = ml;
     } else {
         for (unsigned g = threadIdx.x; g < n; g += NT) {
-            a += load_published(part + 3 * (size_t)g);
-            b += load_published(part + 3 * (size_t)g + 1);
-            m = fmax(m, load_published(part + 3 * (size_t)g + 2));
+            a += load_published(part + 3 * g);
+            b += load_published(part + 3 * g + 1);
+            m = fmax(m, load_published(part + 3 * g + 2));
         }
         block_reduce3<NT>(a, b, m, lds);
     }
@@ -149,11 +196,12 @@ __device__ inline void sum_published(const double* part, unsigned n, bool exact,
 // Per-workgroup totals (valid in thread 0) to the global result: in the last workgroup to
 // finish, sum the partials (in list order when exact) and update the Krylov scalars.
 template <int NT>
-__device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf& rb, int fin, double* lds) {
+__device__ inline void grid_finish(double s0, double s1, double mx, int nsum, const RedBuf& rb, int fin,
+                                   double* lds) {
     __shared__ int is_last;
-    __shared__ double chunk[3 * NT];
+    __shared__ double chunk[2 * NT];
     if (threadIdx.x == 0) {
-        double* p = rb.partials + 3 * (size_t)blockIdx.x;
+        double* p = rb.partials + 3 * (int)blockIdx.x;
         p[0] = s0;
         p[1] = s1;
         p[2] = mx;
@@ -165,7 +213,7 @@ __device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf
     if (!is_last) return;
     __threadfence();
     double a, b, m;
-    sum_published<NT>(rb.partials, gridDim.x, rb.exact != 0, a, b, m, chunk, lds);
+    sum_published<NT>(rb.partials, gridDim.x, rb.exact != 0, nsum, a, b, m, chunk, lds);
     if (threadIdx.x == 0) {
         double* S = rb.scal;
         switch (fin) {
@@ -184,15 +232,14 @@ __device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf
     }
 }
 
-// Block totals of two per-cell sums (q0, q1) and a max, then grid_finish.
+// Block totals of nsum per-cell sums (q0, q1) and a max, then grid_finish.  `scratch` is
+// LDS of at least 2*NCELL+2 doubles.
 template <int NT, int NCELL, int CPT>
 __device__ inline void grid_reduce_cells(const double (&q0)[CPT], const double (&q1)[CPT], double mx, int nsum,
                                          const RedBuf& rb, int fin, double* scratch, double* lds) {
-    const bool two = nsum > 1;
     double s0 = 0.0, s1 = 0.0;
     if (rb.exact) {
-        if (nsum > 0) s0 = seq_block_sum<NT, NCELL, CPT>(q0, scratch);
-        if (two) s1 = seq_block_sum<NT, NCELL, CPT>(q1, scratch);
+        if (nsum > 0) seq_block_sums<NT, NCELL, CPT>(q0, q1, nsum, scratch, s0, s1);
         double z0 = 0.0, z1 = 0.0;
         block_reduce3<NT>(z0, z1, mx, lds);
     } else {
@@ -203,7 +250,7 @@ __device__ inline void grid_reduce_cells(const double (&q0)[CPT], const double (
         }
         block_reduce3<NT>(s0, s1, mx, lds);
     }
-    grid_finish<NT>(s0, s1, mx, rb, fin, lds);
+    grid_finish<NT>(s0, s1, mx, nsum, rb, fin, lds);
 }
 
 // ---------------------------------------------------------------------------- stencil core
@@ -248,7 +295,7 @@ __device__ inline void load_tile(T* tile, const T* __restrict__ x, const BlockIn
     for (int c = threadIdx.x; c < G::NCELL; c += blockDim.x) {
         int i, j, k;
         G::cell_ijk(c, i, j, k);
-        tile[G::tidx(i, j, k)] = x[(size_t)b * G::NCELL + c];
+        tile[G::tidx(i, j, k)] = x[b * G::NCELL + c];
     }
     for (int g = threadIdx.x; g < G::NGHOST; g += blockDim.x) {
         const int f = g / G::NSLAB, p = g % G::NSLAB;
@@ -265,7 +312,7 @@ __device__ inline void load_tile(T* tile, const T* __restrict__ x, const BlockIn
         T v = T(0);
         if (n >= 0) {
             q[d] = s ? 0 : ext - 1;
-            v = x[(size_t)n * G::NCELL + G::cidx(q[0], q[1], q[2])];
+            v = x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
         } else if (n > PHYS_BNDRY) {
             const int cb = bi.cnbr[f];
             const int wc = bi.which_child;
@@ -273,7 +320,7 @@ __device__ inline void load_tile(T* tile, const T* __restrict__ x, const BlockIn
             r[d] = s ? 0 : ext - 1;
             r[a1] = (q[a1] >> 1) + side_of(wc, a1) * (NB / 2);
             if (D == 3) r[a2] = (q[a2] >> 1) + side_of(wc, a2) * (NB / 2);
-            v = x[(size_t)cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
+            v = x[cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
         }
         tile[G::tidx(gq[0], gq[1], gq[2])] = v;
     }
@@ -307,8 +354,8 @@ __device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* _
             fq[d] = s ? 0 : ext;
             c[a1] = fq[a1] = (t1 - s1 * half) * 2 + q1;
             if (D == 3) c[a2] = fq[a2] = (t2 - s2 * half) * 2 + q2;
-            const T wv = w[((size_t)fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
-            const T ph = x[(size_t)fb * G::NCELL + G::cidx(c[0], c[1], c[2])];
+            const T wv = w[(fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
+            const T ph = x[fb * G::NCELL + G::cidx(c[0], c[1], c[2])];
             acc += wv * ph;
         }
     }
@@ -318,9 +365,11 @@ __device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* _
 // One cell of the operator (MODE apply) or of a Gauss-Seidel update (MODE smooth):
 // mg_apply_block_usefill (SayakaMacOperator.cpp:1372-1458) and
 // mg_smooth_block_rbcycle_usefill (SayakaMacOperator.cpp:1232-1320) with the face
-// coefficients of cell_eval_coef (1115-1167) already folded into w / wcf.
+// coefficients of cell_eval_coef (1115-1167) already folded into w / wcf.  wb and wcfb are
+// this block's own-face weights [D][NFACE] and coarse/fine weights [NF][NSLAB] (global or
+// LDS copies).
 template <int D, int NB, bool SMOOTH, class T>
-__device__ inline T cell_op(const T* tile, int b, int i, int j, int k, const BlockInfo& bi, int m,
+__device__ inline T cell_op(const T* tile, const T* wb, const T* wcfb, int i, int j, int k, const BlockInfo& bi, int m,
                             const StencilArgsT<T>& a, T dv, T vfv, T rhsv) {
     using G = Geo<D, NB>;
     T diag = a.alpha * dv;
@@ -336,7 +385,7 @@ __device__ inline T cell_op(const T* tile, int b, int i, int j, int k, const Blo
         const int ni = G::tidx(i + (f == 1) - (f == 0), j + (f == 3) - (f == 2), k + (f == 5) - (f == 4));
         T off, offval;
         if (!bnd) {
-            off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+            off = wb[d * G::NFACE + fi];
             offval = off * tile[ni];
         } else {
             const int ft = face_type<D, NB>(bi, f, m);
@@ -345,14 +394,14 @@ __device__ inline T cell_op(const T* tile, int b, int i, int j, int k, const Blo
                 tang_axes<D>(d, a1, a2);
                 const int ijk[3] = {i, j, k};
                 const int t1 = ijk[a1], t2 = D == 3 ? ijk[a2] : 0;
-                off = a.wcf[((size_t)b * G::NF + f) * G::NSLAB + t1 + NB * t2];
+                off = wcfb[f * G::NSLAB + t1 + NB * t2];
                 offval = cf_sum<D, NB, T>(f, t1, t2, bi.nbr[f], a.binfo, a.x, a.w);
             } else if (ft == FT_BC) {
                 if (bi.bct[f] == BC_NEUMANN) {
                     off = T(0);
                     offval = T(0);
                 } else {
-                    off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+                    off = wb[d * G::NFACE + fi];
                     if (SMOOTH) {
                         offval = T(0);
                         off *= T(2);
@@ -361,7 +410,7 @@ __device__ inline T cell_op(const T* tile, int b, int i, int j, int k, const Blo
                     }
                 }
             } else {
-                off = a.w[((size_t)b * D + d) * G::NFACE + fi];
+                off = wb[d * G::NFACE + fi];
                 offval = off * tile[ni];
             }
         }
@@ -382,7 +431,7 @@ k_stencil(StencilArgs a, RedBuf rb, int fin) {
     using G = Geo<D, NB>;
     constexpr int NT = G::THREADS;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;
-    __shared__ double tile[G::NTILE];
+    __shared__ double tile[G::NTILE > 2 * G::NCELL + 2 ? G::NTILE : 2 * G::NCELL + 2];
     __shared__ double lds[3 * ((NT + 63) / 64)];
     __shared__ BlockInfo bi;
     const int b = a.list[blockIdx.x];
@@ -399,14 +448,17 @@ k_stencil(StencilArgs a, RedBuf rb, int fin) {
         if (c >= G::NCELL) continue;
         int i, j, k;
         G::cell_ijk(c, i, j, k);
-        const size_t gi = (size_t)b * G::NCELL + c;
+        const int gi = b * G::NCELL + c;
         const double vfv = a.vf[gi];
         if (MODE == ST_SMOOTH) {
             if (((i + j + k) & 1) != a.colour) continue;
-            const double u = cell_op<D, NB, true, double>(tile, b, i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
+            const double u = cell_op<D, NB, true, double>(tile, a.w + b * (D * G::NFACE), a.wcf + b * (G::NF * G::NSLAB),
+                                                          i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
             const_cast<double*>(a.x)[gi] = u;
         } else {
-            const double Lp = cell_op<D, NB, false, double>(tile, b, i, j, k, bi, a.m, a, dv, vfv, 0.0);
+            const double Lp = cell_op<D, NB, false, double>(tile, a.w + b * (D * G::NFACE),
+                                                            a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi, a.m, a, dv,
+                                                            vfv, 0.0);
             if (MODE == ST_APPLY || MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) {
                 a.out[gi] = Lp;
                 if (MODE == ST_APPLY_DOT) q0[q] = a.dotv[gi] * Lp;   // rhat . v
@@ -452,12 +504,12 @@ k_restrict(const int* list, const BlockInfo* binfo, const double* vf, double* x)
         for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
             for (int jj = 0; jj <= 1; jj++)
                 for (int ii = 0; ii <= 1; ii++) {
-                    const size_t fi = (size_t)fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
+                    const int fi = fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
                     sum += x[fi] * vf[fi];
                 }
-        const double v = vf[(size_t)b * G::NCELL + c];
+        const double v = vf[b * G::NCELL + c];
         const double r = sum * wgt;
-        x[(size_t)b * G::NCELL + c] = v > 0 ? r / v : 0.0;
+        x[b * G::NCELL + c] = v > 0 ? r / v : 0.0;
     }
 }
 
@@ -476,8 +528,8 @@ k_prolong(const int* list, const BlockInfo* binfo, const double* vf, double* x) 
         G::cell_ijk(c, i, j, k);
         const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
                                D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
-        const size_t gi = (size_t)b * G::NCELL + c;
-        x[gi] = vf[gi] > 0 ? x[(size_t)p * G::NCELL + pc] : 0.0;
+        const int gi = b * G::NCELL + c;
+        x[gi] = vf[gi] > 0 ? x[p * G::NCELL + pc] : 0.0;
     }
 }
 
@@ -515,7 +567,7 @@ template <int NCELL, int NT>
 __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fin) {
     constexpr int CPT = (NCELL + NT - 1) / NT;
     __shared__ double lds[3 * ((NT + 63) / 64)];
-    __shared__ double scratch[NCELL];
+    __shared__ double scratch[2 * NCELL + 2];
     const int b = a.list[blockIdx.x];
     double q0[CPT], q1[CPT], mx = 0.0;
     const double* S = a.scal;
@@ -524,7 +576,7 @@ __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fi
         q0[q] = q1[q] = 0.0;
         const int c = threadIdx.x + q * NT;
         if (c >= NCELL) continue;
-        const size_t gi = (size_t)b * NCELL + c;
+        const int gi = b * NCELL + c;
         switch (op) {
             case V_PINIT: {
                 const double pv = a.r[gi];

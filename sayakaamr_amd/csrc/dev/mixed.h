@@ -65,7 +65,7 @@ k_mix(const int* list, int op, const BlockInfo* __restrict__ binfo, const float*
     const int b = list[blockIdx.x];
     const int h = NB / 2;
     for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
-        const size_t gi = (size_t)b * G::NCELL + c;
+        const int gi = b * G::NCELL + c;
         switch (op) {
             case M_ZERO: dst[gi] = 0.0f; break;
             case M_COPY: dst[gi] = src[gi]; break;
@@ -76,7 +76,7 @@ k_mix(const int* list, int op, const BlockInfo* __restrict__ binfo, const float*
                 const int p = binfo[b].parent, wc = binfo[b].which_child;
                 const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
                                        D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
-                if (vf[gi] > 0.0f) dst[gi] += src[(size_t)p * G::NCELL + pc];
+                if (vf[gi] > 0.0f) dst[gi] += src[p * G::NCELL + pc];
                 break;
             }
             case M_F64_TO_F32: dst[gi] = (float)src64[gi]; break;
@@ -101,13 +101,17 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_stencil32(StencilArgs
     for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
         int i, j, k;
         G::cell_ijk(c, i, j, k);
-        const size_t gi = (size_t)b * G::NCELL + c;
+        const int gi = b * G::NCELL + c;
         const float vfv = a.vf[gi];
         if (MODE == ST_SMOOTH) {
             if (((i + j + k) & 1) != a.colour) continue;
-            const_cast<float*>(a.x)[gi] = cell_op<D, NB, true, float>(tile, b, i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
+            const_cast<float*>(a.x)[gi] = cell_op<D, NB, true, float>(tile, a.w + b * (D * G::NFACE),
+                                                                          a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi,
+                                                                          a.m, a, dv, vfv, a.rhs[gi]);
         } else {
-            const float Lp = cell_op<D, NB, false, float>(tile, b, i, j, k, bi, a.m, a, dv, vfv, 0.0f);
+            const float Lp = cell_op<D, NB, false, float>(tile, a.w + b * (D * G::NFACE),
+                                                           a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi, a.m, a, dv, vfv,
+                                                           0.0f);
             a.out[gi] = vfv > 0.0f ? a.rhs[gi] - Lp : 0.0f;
         }
     }
@@ -131,11 +135,11 @@ k_restrict32(const int* list, const BlockInfo* binfo, const float* vf, const flo
         for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
             for (int jj = 0; jj <= 1; jj++)
                 for (int ii = 0; ii <= 1; ii++) {
-                    const size_t fi = (size_t)fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
+                    const int fi = fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
                     sum += src[fi] * vf[fi];
                 }
-        const float v = vf[(size_t)b * G::NCELL + c];
-        dst[(size_t)b * G::NCELL + c] = v > 0.0f ? (sum * wgt) / v : 0.0f;
+        const float v = vf[b * G::NCELL + c];
+        dst[b * G::NCELL + c] = v > 0.0f ? (sum * wgt) / v : 0.0f;
     }
 }
 

@@ -74,8 +74,8 @@ struct sayaka_dev_ctx {
     unsigned* d_counter = nullptr;
     double* d_scal = nullptr;
     double* h_scal = nullptr;  // pinned
-    double* d_bot_pg = nullptr;
-    double* d_bot_sg = nullptr;
+    double* d_bot_vpub = nullptr;
+    double* d_bot_rpub = nullptr;
     double* d_bot_part = nullptr;
     int* d_bot_stats = nullptr;
     int* h_bot_stats = nullptr;  // pinned
@@ -292,8 +292,8 @@ struct Impl {
         A.st = stencil_args(c, lst, 1);
         A.rhs = rhs;
         A.sol = sol;
-        A.pg = c->d_bot_pg;
-        A.sg = c->d_bot_sg;
+        A.vpub = c->d_bot_vpub;
+        A.rpub = c->d_bot_rpub;
         A.partials = c->d_bot_part;
         A.stats = c->d_bot_stats;
         A.tol_rel = tol_rel;
@@ -303,8 +303,9 @@ struct Impl {
         void* args[] = {&A};
         // iterative and latency-bound: no per-launch byte count
         ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
-        HIPCHK(hipLaunchCooperativeKernel((const void*)k_bottom<D, NB>, dim3(lst.n), dim3(G::THREADS), args, 0,
-                                          c->stream));
+        using BC = BotCfg<D, NB>;
+        const int nwg = (lst.n + BC::BPW - 1) / BC::BPW;
+        HIPCHK(hipLaunchCooperativeKernel((const void*)k_bottom<D, NB>, dim3(nwg), dim3(BC::NT), args, 0, c->stream));
         return 0;
     }
     // mg_precond_level -> mg_relax_vcycle, one cycle from a zero guess
@@ -705,9 +706,9 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_counter, sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&c->d_scal, S_NSCALAR * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_scal, S_NSCALAR * sizeof(double), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&c->d_bot_pg, cells * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->d_bot_sg, cells * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->d_bot_part, (size_t)(max_blocks + 64) * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_vpub, cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_rpub, cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_part, (size_t)(max_blocks + 64) * 9 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 4 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 4 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream);
@@ -730,7 +731,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_counter, c->d_scal,
-                    c->d_bot_pg, c->d_bot_sg, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists};
+                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < SAYAKA_KCLASS_N; k++)
         for (auto& r : c->prof[k]) {
