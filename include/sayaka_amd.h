@@ -61,7 +61,8 @@ typedef struct {
 const char* sayaka_dev_last_error(void);
 
 /* MacSolver(tree) + MacSolver::define (SayakaMacSolver.h:56-87, SayakaMacOperator.cpp:13-60):
- * device buffers for up to max_blocks blocks of nb^ndim cells. */
+ * device buffers for up to max_blocks blocks of nb^ndim cells (nb = 8 or 16 in 2D / 3D, or
+ * 4 in 3D). */
 int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_ctx** out);
 int sayaka_dev_destroy(sayaka_dev_ctx* ctx);
 

@@ -662,6 +662,7 @@ int dispatch(sayaka_dev_ctx* c, F&& f) {
     if (c->ndim == 2 && c->nb == 8) return f(Impl<2, 8>{});
     if (c->ndim == 3 && c->nb == 16) return f(Impl<3, 16>{});
     if (c->ndim == 2 && c->nb == 16) return f(Impl<2, 16>{});
+    if (c->ndim == 3 && c->nb == 4) return f(Impl<3, 4>{});
     return fail(-2, "unsupported (ndim, nb)");
 }
 
@@ -677,7 +678,8 @@ const char* sayaka_dev_last_error(void) { return g_err.c_str(); }
 int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_ctx** out) {
     if (!out) return fail(-1, "null out");
     *out = nullptr;
-    if (!((ndim == 3 || ndim == 2) && (nb == 8 || nb == 16))) return fail(-1, "supported blocks: 8^d, 16^d");
+    if (!((ndim == 3 || ndim == 2) && (nb == 8 || nb == 16)) && !(ndim == 3 && nb == 4))
+        return fail(-1, "supported blocks: 8^d, 16^d (d = 2, 3) and 4^3");
     if (max_blocks < 1) return fail(-1, "max_blocks < 1");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));

@@ -1,0 +1,62 @@
+"""Host AmrTree (product) against the reference's recorded tree tables: bit-exact.
+
+The golden tables were written by the compiled reference (AmrTree::init/regrid,
+SayakaTree.cpp:34-233, SayakaRegrid.cpp:26-976) in tests/golden/make_golden.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from golden_cases import TREE_KEYS, tree_of
+from sayakaamr_amd import workloads as W
+from sayakaamr_amd.tree import AmrTree
+
+
+def assert_tree_equal(t, g):
+    for k in TREE_KEYS:
+        if k in g:
+            assert np.array_equal(t[k], g[k]), k
+
+
+@pytest.mark.parametrize("name,spec,crit", [
+    ("c0_ops", W.C0_SPEC, W.ring_criterion()),
+    ("s3_ops", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25)),
+    ("c1_summary", W.C1_SPEC, W.shell_criterion()),
+])
+def test_tree_matches_reference(golden, name, spec, crit):
+    g = tree_of(golden(name))
+    t = W.build_tree(AmrTree(spec.ndim), spec, crit)
+    assert_tree_equal(t, g)
+
+
+def test_c1_tree_size(golden):
+    g = tree_of(golden("c1_summary"))
+    assert g["level"].shape[0] == 4864
+    assert int((g["node_type"] == 1).sum()) == 4264
+    assert int(g["level"].max()) == 4
+
+
+def test_regrid_sequence_with_coarsening(golden):
+    """Six steps of a moving sphere, refining and coarsening every step (SayakaRegrid.cpp)."""
+    gs = golden("regrid_sequence")
+    spec = W.small_spec(3, 4)
+    rt = AmrTree(3)
+    W.build_tree(rt, spec, W.shell_criterion(centre=(0.35, 0.5, 0.5), r0=0.18))
+    for step in range(6):
+        rt.set_refine_flags(gs[f"step{step}_refine"])
+        rt.set_coarsen_flags(gs[f"step{step}_coarsen"])
+        rt.regrid()
+        assert_tree_equal(rt.export(), tree_of(gs, f"step{step}_"))
+
+
+def test_regrid_overflow_raises():
+    spec = W.small_spec(3, 4)
+    t = AmrTree(3, max_blocks=20)
+    t.create(spec)
+    n = t.numblocks
+    with pytest.raises((MemoryError, ValueError)):
+        for _ in range(3):
+            t.set_refine_flags(np.ones(t.numblocks, np.int32))
+            t.regrid()
+    assert t.numblocks >= n
