@@ -1,0 +1,127 @@
+// mac_solver.h -- MAC Poisson solver over an AmrTree, solving on the GPU.
+//
+// The call surface of the reference MacSolver (SayakaMacSolver.h:25-425) for the solve path,
+// with the same call order as its drivers (Test7.h:612-653):
+//
+//     MacSolver mac(tree);
+//     mac.prepareSolver();            // after every regrid
+//     mac.setBC(bcreg);
+//     mac.setIBFracByLS(ls, 0);       // or setIBFrac / setUnitIBFrac
+//     mac.setCoefB(bcoefs, 0);
+//     mac.setRhs(rhs, 0);  mac.setZeroSol();
+//     mac.setTolerance(1e-10, 0.0);  mac.setMaxIteration(2000);
+//     int stat = mac.cg_solve();      // 0 solved or not converged, 1 breakdown (as the reference)
+//     mac.getSol(phi, 0);
+//
+// The tree stays on the host and is owned by the caller (it must outlive the solver); the
+// solver owns a device context (include/sayaka_amd.h) that holds every field as packed
+// valid-only blocks.  prepareSolver() sends the tree's tables; the coefficient, BC, rhs and
+// solution calls copy the valid region of the caller's TreeData to or from the device.
+// Errors the reference treats with exit(1) throw std::runtime_error here.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "amr_tree.h"
+#include "bc_register.h"
+#include "tree_data.h"
+
+struct sayaka_dev_ctx;
+
+namespace sayaka {
+inline namespace amd {
+
+struct SolveStats {
+    int status = 0;          // 0 converged, 1 breakdown, 2 iteration limit
+    int iters = 0;
+    bool halfStep = false;
+    int bottomCalls = 0;
+    long long bottomIters = 0;
+    double rnorm0 = 0.0, rnorm = 0.0;
+    std::vector<double> hist;   // -|s| at half steps, +|r| at full steps
+};
+
+class MacSolver {
+   public:
+    enum Precision { FP64 = 0, Mixed = 1 };
+
+    explicit MacSolver(const AmrTree& tree, int device = 0);
+    ~MacSolver();
+    MacSolver(const MacSolver&) = delete;
+    MacSolver& operator=(const MacSolver&) = delete;
+
+    void setVerbose(int v) { verbose_ = v; }
+    // MacSolver::prepareSolver (SayakaMacSolver.cpp:12-26): tables of the current tree, unit
+    // coefficients.  Call again after every regrid.
+    void prepareSolver();
+    void clearSolver() {}
+
+    // coefficients (SayakaMacIB.cpp:12-154, SayakaMacCoef.cpp:8-76, 319-367)
+    void setUnitIBFrac();
+    void setIBFracByLS(const TreeData& ls, int lscomp);
+    // fractional volume / area fractions given directly (cut cells); af[dir] face data
+    void setIBFrac(const TreeData& vf, int vfcomp, const std::vector<const TreeData*>& af, int afcomp);
+    void setDefaultCoefs();
+    void setUniformCoefB(double b);
+    void setCoefB(const std::vector<TreeData*>& bcoef, int bcomp);
+    void setCoefB(int dir, const TreeData& bcoef, int bcomp);
+    void setScalars(double alpha, double beta);
+    void setAlpha(double alpha) { setScalars(alpha, beta_); }
+    void setBeta(double beta) { setScalars(alpha_, beta); }
+
+    // boundary conditions (SayakaMacOperator.cpp:1462-1594)
+    void setBC(const BCRegister& reg);
+    void fixBC(const TreeData& phi, TreeData& rhs, int phicomp, int rhscomp) const;
+
+    // data in / out (SayakaMacSolver.h:117-132)
+    void setRhs(const TreeData& rhs, int comp);
+    void setInitSol(const TreeData& sol, int comp);
+    void setZeroSol();
+    void getSol(TreeData& sol, int comp) const;
+
+    // parameters (SayakaMacSolver.h:349-358)
+    void setTolerance(double rel, double abs) { tolRel_ = rel; tolAbs_ = abs; }
+    void setMaxIteration(int n) { maxIter_ = n; }
+    void setPreSmoothNum(int n) { preSmooth_ = n; }
+    void setPostSmoothNum(int n) { postSmooth_ = n; }
+    void setCGUseMGPrecond(int on) { precond_ = on; }
+    void setPrecision(Precision p) { precision_ = p; }
+
+    // The same calls on packed valid-only arrays ([blk][k][j][i] over the tree's blocks, face
+    // arrays per direction; the level set over the grown box with ngrow >= 1 ghost layers):
+    // the form the flat C ABI (include/sayaka_host.h) passes through.
+    void setIBFracByLSPacked(const double* ls_grown, int ngrow);
+    void setIBFracPacked(const double* vf, const double* const* af);
+    void setCoefBPacked(const double* const* bcoef);
+    void setRhsPacked(const double* rhs);
+    void setInitSolPacked(const double* sol);
+    void getSolPacked(double* sol) const;
+    int numBlocks() const { return nblk_; }
+
+    // MacSolver::cg_solve (SayakaMacSolver.cpp:455-680)
+    int cg_solve();
+    const SolveStats& lastStats() const { return stats_; }
+
+    const AmrTree& tree() const { return *tree_; }
+    sayaka_dev_ctx* deviceContext() const { return ctx_; }
+
+   private:
+    void uploadCoefs();
+    void check(int rc, const char* what) const;
+
+    const AmrTree* tree_;
+    sayaka_dev_ctx* ctx_ = nullptr;
+    int nblk_ = 0, ncell_ = 0, nface_ = 0;
+    int verbose_ = 0;
+    double alpha_ = 0.0, beta_ = 1.0;
+    double tolRel_ = 1e-10, tolAbs_ = 0.0;
+    int maxIter_ = 2000, preSmooth_ = 2, postSmooth_ = 2, precond_ = 1;
+    Precision precision_ = FP64;
+    bool unitIB_ = true, unitB_ = true;
+    std::vector<double> vf_, af_[3], b_[3];   // leaf values, packed valid [blk][...]
+    SolveStats stats_;
+};
+
+}  // namespace amd
+}  // namespace sayaka

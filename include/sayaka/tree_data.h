@@ -1,0 +1,65 @@
+// tree_data.h -- block data of one tree on the host (the caller's fields).
+//
+// Mirrors the reference TreeData / GridData (SayakaTreeData.h:86-306, SayakaBlockData.h:12-215):
+// one slot per node of the tree's pool, each block stored [comp][k][j][i] over its box grown
+// by ngrow cells (i fastest, SayakaBox.h:214-217); face data is staggered in one direction
+// (one more face than cells along it).  Valid cells are indexed from 0, ghosts from -ngrow.
+// The solver on the device keeps its own packed valid-only copies; these host objects are
+// what callers fill and read, exactly as with the reference.
+#pragma once
+
+#include <cstddef>
+#include <vector>
+
+#include "amr_tree.h"
+
+namespace sayaka {
+inline namespace amd {
+
+class TreeData {
+   public:
+    TreeData() = default;
+
+    // TreeData::MakeCellData / MakeFaceData (SayakaTreeData.h:245-306)
+    static TreeData MakeCellData(const AmrTree& tree, int ncomp, int ngrow);
+    static TreeData MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow);
+
+    int numComp() const { return ncomp_; }
+    int numGrow() const { return ngrow_; }
+    int faceDir() const { return dir_; }   // -1 for cell data
+    int numBlockSlots() const { return nslot_; }
+    bool defined() const { return nslot_ > 0; }
+
+    // extents of the grown box along each direction
+    int extent(int d) const { return d == 0 ? lx_ : (d == 1 ? ly_ : lz_); }
+    // valid extent along d (cells, or faces in the staggered direction)
+    int validExtent(int d) const;
+
+    double& operator()(int blk, int i, int j, int k, int comp) { return data_[offset(blk, i, j, k, comp)]; }
+    double operator()(int blk, int i, int j, int k, int comp) const { return data_[offset(blk, i, j, k, comp)]; }
+
+    // TreeData::SetValue (SayakaTreeData.cpp:230-263): every slot, grown region included
+    void SetValue(double v);
+    void SetValue(double v, int comp);
+
+    // Valid region of component comp of blocks [0, nblk) to / from a packed array
+    // [blk][k][j][i] (the layout of the device C ABI).
+    void packValid(int comp, int nblk, double* out) const;
+    void unpackValid(int comp, int nblk, const double* in);
+    // Component comp of blocks [0, nblk), grown region included, to a packed array
+    // [blk][grown box] (the layout sayaka_dev_fix_bc reads).
+    void packGrown(int comp, int nblk, double* out) const;
+
+   private:
+    std::size_t offset(int blk, int i, int j, int k, int comp) const {
+        return (std::size_t)blk * blocksize_ + (std::size_t)comp * compsize_ +
+               (std::size_t)((i + ngrow_) + lx_ * ((j + ngrow_) + ly_ * (k + kg_)));
+    }
+    int ndim_ = 3, nb_ = 8, ngrow_ = 0, ncomp_ = 0, dir_ = -1, nslot_ = 0;
+    int lx_ = 0, ly_ = 0, lz_ = 0, kg_ = 0;
+    std::size_t compsize_ = 0, blocksize_ = 0;
+    std::vector<double> data_;
+};
+
+}  // namespace amd
+}  // namespace sayaka

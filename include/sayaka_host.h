@@ -41,6 +41,35 @@ void sy_tree_export(const sy_tree* t, int* level, int* parent, int* which_child,
 int sy_tree_level_count(const sy_tree* t, int level);
 void sy_tree_level_blocks(const sy_tree* t, int level, int* out);
 
+/* ---- MacSolver (SayakaMacSolver.h:25-425) over a tree; solves on the GPU -------------
+ * Integer returns: 0 ok (cg_solve: the reference's 0 / 1), < 0 error (sy_mac_last_error). */
+const char* sy_mac_last_error(void);
+/* MacSolver(const AmrTree&) (SayakaMacSolver.h:56): device context sized by the tree's pool */
+sy_mac* sy_mac_create(sy_tree* tree, int device);
+void sy_mac_destroy(sy_mac* m);
+/* MacSolver::prepareSolver (SayakaMacSolver.cpp:12-26); call after every regrid */
+int sy_mac_prepare(sy_mac* m);
+/* MacSolver::setBC (SayakaMacOperator.cpp:1462-1469) with a BCRegister that maps each
+ * phys_bc[q] to math_bc[q] (0 Neumann, 1 Dirichlet, 2 SimpleFill) on every face */
+int sy_mac_set_bc(sy_mac* m, int n, const int* phys_bc, const int* math_bc);
+/* MacSolver::setIBFracByLS (SayakaMacIB.cpp:12-154): level set over the grown box [blk][...] */
+int sy_mac_set_ib_ls(sy_mac* m, const double* ls_grown, int ngrow);
+/* cut-cell volume fractions [blk][cells] and area fractions af[dir][blk][faces] */
+int sy_mac_set_ib_frac(sy_mac* m, const double* vf, const double* const* af);
+/* MacSolver::setCoefB (SayakaMacCoef.cpp:319-367): b[dir][blk][faces] */
+int sy_mac_set_coefb(sy_mac* m, const double* const* b);
+/* MacSolver::setRhs / setInitSol / setZeroSol / getSol (SayakaMacSolver.h:117-132) */
+int sy_mac_set_rhs(sy_mac* m, const double* rhs);
+int sy_mac_set_init_sol(sy_mac* m, const double* sol);
+int sy_mac_set_zero_sol(sy_mac* m);
+int sy_mac_get_sol(sy_mac* m, double* sol);
+/* setTolerance / setMaxIteration / setCGUseMGPrecond / setPre-PostSmoothNum
+ * (SayakaMacSolver.h:349-358); precision 0 = FP64, 1 = mixed */
+int sy_mac_set_params(sy_mac* m, double tol_rel, double tol_abs, int max_iter, int precond, int precision,
+                      int pre_smooth, int post_smooth);
+/* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680) */
+int sy_mac_cg_solve(sy_mac* m, int* iters, int* status, double* rnorm0, double* rnorm);
+
 #ifdef __cplusplus
 }
 #endif

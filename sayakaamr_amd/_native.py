@@ -38,6 +38,20 @@ _HOST_SIG = {
     "sy_tree_export": ([vp] + [vp] * 14, None),
     "sy_tree_level_count": ([vp, i32], i32),
     "sy_tree_level_blocks": ([vp, i32, vp], None),
+    "sy_mac_last_error": ([], C.c_char_p),
+    "sy_mac_create": ([vp, i32], vp),
+    "sy_mac_destroy": ([vp], None),
+    "sy_mac_prepare": ([vp], i32),
+    "sy_mac_set_bc": ([vp, i32, vp, vp], i32),
+    "sy_mac_set_ib_ls": ([vp, vp, i32], i32),
+    "sy_mac_set_ib_frac": ([vp, vp, vp], i32),
+    "sy_mac_set_coefb": ([vp, vp], i32),
+    "sy_mac_set_rhs": ([vp, vp], i32),
+    "sy_mac_set_init_sol": ([vp, vp], i32),
+    "sy_mac_set_zero_sol": ([vp], i32),
+    "sy_mac_get_sol": ([vp, vp], i32),
+    "sy_mac_set_params": ([vp, f64, f64, i32, i32, i32, i32, i32], i32),
+    "sy_mac_cg_solve": ([vp, vp, vp, vp, vp], i32),
 }
 
 

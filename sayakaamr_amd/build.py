@@ -22,7 +22,9 @@ ARCH = "gfx950"
 
 DEV_SOURCES = ["dev/sayaka_dev.hip"]
 DEV_HEADERS = ["dev/dev_common.h", "dev/kernels.h", "dev/bottom.h", "dev/coef.h", "dev/mixed.h"]
-HOST_SOURCES = ["host/amr_tree.cpp", "host/capi_tree.cpp"]
+HOST_SOURCES = ["host/amr_tree.cpp", "host/capi_tree.cpp", "host/tree_data.cpp", "host/mac_solver.cpp",
+                "host/capi_mac.cpp"]
+DRIVERS = {"sayaka_c1_solve": "drivers/c1_solve.cpp"}
 
 
 def _run(cmd, verbose):
@@ -47,18 +49,33 @@ def build_dev(verbose=False) -> str:
 
 
 def build_host(verbose=False) -> str:
+    """Host library; links libsayaka_dev.so (the MacSolver drives the device through its C ABI)."""
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsayaka_host.so")
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}",
-           f"-I{os.path.join(CSRC, 'host')}", *[os.path.join(CSRC, s) for s in HOST_SOURCES],
-           "-o", out + ".tmp"]
+           f"-I{os.path.join(INC, 'sayaka')}", *[os.path.join(CSRC, s) for s in HOST_SOURCES],
+           f"-L{LIB}", "-lsayaka_dev", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     _run(cmd, verbose)
     os.replace(out + ".tmp", out)
     return out
 
 
+def build_drivers(verbose=False):
+    """C++ drivers written against the MacSolver / AmrTree / TreeData headers."""
+    outs = []
+    for name, src in DRIVERS.items():
+        out = os.path.join(LIB, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INC}", f"-I{os.path.join(INC, 'sayaka')}",
+               os.path.join(CSRC, src), f"-L{LIB}", "-lsayaka_host", "-lsayaka_dev", "-Wl,-rpath,$ORIGIN",
+               "-o", out + ".tmp"]
+        _run(cmd, verbose)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 def build_all(verbose=False):
-    return [build_dev(verbose), build_host(verbose)]
+    return [build_dev(verbose), build_host(verbose), *build_drivers(verbose)]
 
 
 if __name__ == "__main__":

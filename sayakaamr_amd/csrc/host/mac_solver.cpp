@@ -1,0 +1,264 @@
+// mac_solver.cpp -- host MacSolver: the reference's call surface over the device C ABI.
+#include "mac_solver.h"
+
+#include <cstdio>
+#include <stdexcept>
+
+#include "sayaka_amd.h"
+
+namespace sayaka {
+inline namespace amd {
+
+MacSolver::MacSolver(const AmrTree& tree, int device) : tree_(&tree) {
+    ncell_ = tree.cellsPerBlock();
+    nface_ = (tree.nb + 1) * tree.nb * (tree.ndim == 3 ? tree.nb : 1);
+    check(sayaka_dev_create(device, tree.ndim, tree.nb, tree.maxBlockNum, &ctx_), "MacSolver: device context");
+}
+
+MacSolver::~MacSolver() {
+    if (ctx_) sayaka_dev_destroy(ctx_);
+}
+
+void MacSolver::check(int rc, const char* what) const {
+    if (rc < 0) throw std::runtime_error(std::string(what) + ": " + sayaka_dev_last_error());
+}
+
+void MacSolver::prepareSolver() {
+    const AmrTree& t = *tree_;
+    nblk_ = t.numBlocks;
+    std::vector<int> level(nblk_), parent(nblk_), wc(nblk_), child(8 * nblk_), nbr(6 * nblk_), type(nblk_);
+    std::vector<double> len(3 * nblk_);
+    for (int b = 0; b < nblk_; b++) {
+        const AmrTreeNode& n = t[b];
+        level[b] = n.refineLevel;
+        parent[b] = n.parent;
+        wc[b] = n.whichChild;
+        type[b] = n.nodeType;
+        for (int q = 0; q < 8; q++) child[8 * b + q] = n.child[q];
+        for (int f = 0; f < 6; f++) nbr[6 * b + f] = n.neighbor[f];
+        for (int d = 0; d < 3; d++) len[3 * b + d] = n.blockLength(d);
+    }
+    check(sayaka_dev_set_topology(ctx_, nblk_, t.minAmrLevel, t.maxAmrLevel, level.data(), parent.data(), wc.data(),
+                                  child.data(), nbr.data(), type.data(), len.data()),
+          "MacSolver::prepareSolver");
+    // prepareSolver resets the coefficients to unit values (SayakaMacSolver.cpp:12-26)
+    unitIB_ = unitB_ = true;
+    alpha_ = 0.0;
+    beta_ = 1.0;
+    vf_.clear();
+    for (int d = 0; d < 3; d++) {
+        af_[d].clear();
+        b_[d].clear();
+    }
+}
+
+void MacSolver::uploadCoefs() {
+    const double* af[3] = {nullptr, nullptr, nullptr};
+    const double* bb[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < tree_->ndim; d++) {
+        if (!unitIB_) af[d] = af_[d].data();
+        if (!unitB_) bb[d] = b_[d].data();
+    }
+    check(sayaka_dev_set_coefs(ctx_, alpha_, beta_, unitIB_ ? nullptr : vf_.data(), unitIB_ ? nullptr : af,
+                               unitB_ ? nullptr : bb),
+          "MacSolver: coefficients");
+}
+
+void MacSolver::setUnitIBFrac() {
+    unitIB_ = true;
+    uploadCoefs();
+}
+
+// setIBFracByLS -> calcBlockIBVolumeFrac / calcBlockIBAreaFrac (SayakaMacIB.cpp:34-154):
+// on the blocks of the finest tower a cell is solid where ls <= 0 and a face is closed when
+// either adjacent cell is solid; the device then synchronises coarse/fine faces and averages
+// the fractions down to the parent blocks.
+void MacSolver::setIBFracByLS(const TreeData& ls, int lscomp) {
+    if (ls.numGrow() < 1 || ls.faceDir() >= 0) throw std::invalid_argument("setIBFracByLS: cell data with one ghost layer");
+    std::vector<double> g((std::size_t)nblk_ * ls.extent(0) * ls.extent(1) * ls.extent(2));
+    ls.packGrown(lscomp, nblk_, g.data());
+    setIBFracByLSPacked(g.data(), ls.numGrow());
+}
+
+void MacSolver::setIBFracByLSPacked(const double* lsg, int ng) {
+    const AmrTree& t = *tree_;
+    if (ng < 1) throw std::invalid_argument("setIBFracByLS: the level set needs one ghost layer");
+    const int nb = t.nb, nbz = t.ndim == 3 ? nb : 1, finest = t.currentFinestLevel();
+    const int kg = t.ndim == 3 ? ng : 0, lx = nb + 2 * ng, ly = nb + 2 * ng, lz = nbz + 2 * kg;
+    auto ls = [&](int b, int i, int j, int k) {
+        return lsg[(std::size_t)b * lx * ly * lz + (i + ng) + lx * ((j + ng) + ly * (k + kg))];
+    };
+    vf_.assign((std::size_t)nblk_ * ncell_, 1.0);
+    for (int d = 0; d < t.ndim; d++) af_[d].assign((std::size_t)nblk_ * nface_, 1.0);
+    for (int b = 0; b < nblk_; b++) {
+        if (!(t[b].isLeaf() || t[b].refineLevel == finest)) continue;
+        for (int k = 0; k < nbz; k++)
+            for (int j = 0; j < nb; j++)
+                for (int i = 0; i < nb; i++)
+                    vf_[(std::size_t)b * ncell_ + i + nb * (j + nb * k)] = ls(b, i, j, k) <= 0 ? 0.0 : 1.0;
+        for (int d = 0; d < t.ndim; d++) {
+            const int fx = nb + (d == 0), fy = nb + (d == 1), fz = t.ndim == 3 ? nb + (d == 2) : 1;
+            const int di = d == 0, dj = d == 1, dk = d == 2;
+            for (int k = 0; k < fz; k++)
+                for (int j = 0; j < fy; j++)
+                    for (int i = 0; i < fx; i++)
+                        af_[d][(std::size_t)b * nface_ + i + fx * (j + fy * k)] =
+                            (ls(b, i - di, j - dj, k - dk) <= 0 || ls(b, i, j, k) <= 0) ? 0.0 : 1.0;
+        }
+    }
+    unitIB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setIBFracPacked(const double* vf, const double* const* af) {
+    vf_.assign(vf, vf + (std::size_t)nblk_ * ncell_);
+    for (int d = 0; d < tree_->ndim; d++) af_[d].assign(af[d], af[d] + (std::size_t)nblk_ * nface_);
+    unitIB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setCoefBPacked(const double* const* bcoef) {
+    for (int d = 0; d < tree_->ndim; d++) b_[d].assign(bcoef[d], bcoef[d] + (std::size_t)nblk_ * nface_);
+    unitB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setRhsPacked(const double* rhs) { check(sayaka_dev_upload(ctx_, SAYAKA_RHS, rhs), "MacSolver::setRhs"); }
+
+void MacSolver::setInitSolPacked(const double* sol) {
+    check(sayaka_dev_upload(ctx_, SAYAKA_SOL, sol), "MacSolver::setInitSol");
+}
+
+void MacSolver::getSolPacked(double* sol) const { check(sayaka_dev_download(ctx_, SAYAKA_SOL, sol), "MacSolver::getSol"); }
+
+void MacSolver::setIBFrac(const TreeData& vf, int vfcomp, const std::vector<const TreeData*>& af, int afcomp) {
+    if ((int)af.size() < tree_->ndim) throw std::invalid_argument("setIBFrac: one face array per direction");
+    vf_.resize((std::size_t)nblk_ * ncell_);
+    vf.packValid(vfcomp, nblk_, vf_.data());
+    for (int d = 0; d < tree_->ndim; d++) {
+        af_[d].resize((std::size_t)nblk_ * nface_);
+        af[d]->packValid(afcomp, nblk_, af_[d].data());
+    }
+    unitIB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setDefaultCoefs() {
+    unitB_ = true;
+    alpha_ = 0.0;
+    beta_ = 1.0;
+    uploadCoefs();
+}
+
+void MacSolver::setUniformCoefB(double bval) {
+    for (int d = 0; d < tree_->ndim; d++) b_[d].assign((std::size_t)nblk_ * nface_, bval);
+    unitB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setCoefB(const std::vector<TreeData*>& bcoef, int bcomp) {
+    if ((int)bcoef.size() < tree_->ndim) throw std::invalid_argument("setCoefB: one face array per direction");
+    for (int d = 0; d < tree_->ndim; d++) {
+        b_[d].resize((std::size_t)nblk_ * nface_);
+        bcoef[d]->packValid(bcomp, nblk_, b_[d].data());
+    }
+    unitB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setCoefB(int dir, const TreeData& bcoef, int bcomp) {
+    if (unitB_)
+        for (int d = 0; d < tree_->ndim; d++) b_[d].assign((std::size_t)nblk_ * nface_, 1.0);
+    b_[dir].resize((std::size_t)nblk_ * nface_);
+    bcoef.packValid(bcomp, nblk_, b_[dir].data());
+    unitB_ = false;
+    uploadCoefs();
+}
+
+void MacSolver::setScalars(double alpha, double beta) {
+    alpha_ = alpha;
+    beta_ = beta;
+    uploadCoefs();
+}
+
+// setBC -> BlockBCRecord::setBlock (SayakaMacOperator.cpp:1462-1469, SayakaBoundaryPatch.h:148-167)
+void MacSolver::setBC(const BCRegister& reg) {
+    const AmrTree& t = *tree_;
+    std::vector<int> bc(6 * (std::size_t)nblk_, -1);
+    for (int b = 0; b < nblk_; b++)
+        for (int f = 0; f < t.numFace(); f++) {
+            const int nbr = t[b].neighbor[f];
+            if (nbr > PHYS_BNDRY) continue;
+            int o[3] = {0, 0, 0};
+            o[face_dir(f)] = face_side(f) ? 1 : -1;
+            const int m = reg.getBCMap(o[0], o[1], o[2], nbr);
+            if (m < 0) throw std::runtime_error("MacSolver::setBC: unknown BC mapping for phys_bc=" + std::to_string(nbr));
+            bc[6 * b + f] = m;
+        }
+    check(sayaka_dev_set_bc(ctx_, bc.data()), "MacSolver::setBC");
+}
+
+void MacSolver::fixBC(const TreeData& phi, TreeData& rhs, int phicomp, int rhscomp) const {
+    std::vector<double> g((std::size_t)nblk_ * phi.extent(0) * phi.extent(1) * phi.extent(2));
+    phi.packGrown(phicomp, nblk_, g.data());
+    std::vector<double> r((std::size_t)nblk_ * ncell_);
+    rhs.packValid(rhscomp, nblk_, r.data());
+    check(sayaka_dev_upload(ctx_, SAYAKA_TMP, r.data()), "MacSolver::fixBC");
+    check(sayaka_dev_fix_bc(ctx_, g.data(), phi.numGrow(), SAYAKA_TMP), "MacSolver::fixBC");
+    check(sayaka_dev_download(ctx_, SAYAKA_TMP, r.data()), "MacSolver::fixBC");
+    rhs.unpackValid(rhscomp, nblk_, r.data());
+}
+
+void MacSolver::setRhs(const TreeData& rhs, int comp) {
+    std::vector<double> r((std::size_t)nblk_ * ncell_);
+    rhs.packValid(comp, nblk_, r.data());
+    setRhsPacked(r.data());
+}
+
+void MacSolver::setInitSol(const TreeData& sol, int comp) {
+    std::vector<double> r((std::size_t)nblk_ * ncell_);
+    sol.packValid(comp, nblk_, r.data());
+    setInitSolPacked(r.data());
+}
+
+void MacSolver::setZeroSol() { check(sayaka_dev_fill(ctx_, SAYAKA_SOL, 0.0), "MacSolver::setZeroSol"); }
+
+void MacSolver::getSol(TreeData& sol, int comp) const {
+    std::vector<double> r((std::size_t)nblk_ * ncell_);
+    getSolPacked(r.data());
+    sol.unpackValid(comp, nblk_, r.data());
+}
+
+int MacSolver::cg_solve() {
+    sayaka_solve_params p{};
+    p.tol_rel = tolRel_;
+    p.tol_abs = tolAbs_;
+    p.max_iter = maxIter_;
+    p.precond = precond_;
+    p.precision = precision_ == Mixed ? SAYAKA_MIXED : SAYAKA_FP64;
+    p.pre_smooth = preSmooth_;
+    p.post_smooth = postSmooth_;
+    p.hist_cap = 2 * maxIter_ + 2;
+    stats_.hist.assign(p.hist_cap, 0.0);
+    sayaka_solve_stats st{};
+    st.hist = stats_.hist.data();
+    const int rc = sayaka_dev_solve(ctx_, &p, &st);
+    check(rc, "MacSolver::cg_solve");
+    stats_.status = st.status;
+    stats_.iters = st.iters;
+    stats_.halfStep = st.half_step != 0;
+    stats_.bottomCalls = st.bottom_calls;
+    stats_.bottomIters = st.bottom_iters;
+    stats_.rnorm0 = st.rnorm0;
+    stats_.rnorm = st.rnorm;
+    stats_.hist.resize(st.nhist < p.hist_cap ? st.nhist : p.hist_cap);
+    if (verbose_)
+        std::printf("MacSolver::cg_solve: %s, iter=%d, rnorm=%e, rnorm0=%e\n",
+                    st.status == 0 ? "converged" : (st.status == 1 ? "breakdown" : "not converged"), st.iters,
+                    st.rnorm, st.rnorm0);
+    // the reference returns 1 on breakdown and 0 otherwise, converged or not
+    return st.status == 1 ? 1 : 0;
+}
+
+}  // namespace amd
+}  // namespace sayaka

@@ -1,0 +1,74 @@
+// tree_data.cpp -- host block data (TreeData, SayakaTreeData.h:86-306).
+#include "tree_data.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace sayaka {
+inline namespace amd {
+
+TreeData TreeData::MakeCellData(const AmrTree& tree, int ncomp, int ngrow) {
+    return MakeFaceData(tree, -1, ncomp, ngrow);
+}
+
+TreeData TreeData::MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow) {
+    if (ncomp < 1 || ngrow < 0) throw std::invalid_argument("TreeData: ncomp >= 1 and ngrow >= 0 required");
+    if (dir >= tree.ndim) throw std::invalid_argument("TreeData: face direction out of range");
+    TreeData t;
+    t.ndim_ = tree.ndim;
+    t.nb_ = tree.nb;
+    t.ngrow_ = ngrow;
+    t.ncomp_ = ncomp;
+    t.dir_ = dir;
+    t.nslot_ = tree.maxBlockNum;
+    t.kg_ = tree.ndim == 3 ? ngrow : 0;
+    t.lx_ = tree.nb + 2 * ngrow + (dir == 0);
+    t.ly_ = tree.nb + 2 * ngrow + (dir == 1);
+    t.lz_ = tree.ndim == 3 ? tree.nb + 2 * ngrow + (dir == 2) : 1;
+    t.compsize_ = (std::size_t)t.lx_ * t.ly_ * t.lz_;
+    t.blocksize_ = t.compsize_ * ncomp;
+    t.data_.assign(t.blocksize_ * t.nslot_, 0.0);
+    return t;
+}
+
+int TreeData::validExtent(int d) const {
+    if (d == 2 && ndim_ == 2) return 1;
+    return nb_ + (d == dir_ ? 1 : 0);
+}
+
+void TreeData::SetValue(double v) { std::fill(data_.begin(), data_.end(), v); }
+
+void TreeData::SetValue(double v, int comp) {
+    for (int b = 0; b < nslot_; b++)
+        std::fill(data_.begin() + (std::size_t)b * blocksize_ + (std::size_t)comp * compsize_,
+                  data_.begin() + (std::size_t)b * blocksize_ + (std::size_t)(comp + 1) * compsize_, v);
+}
+
+void TreeData::packValid(int comp, int nblk, double* out) const {
+    const int ex = validExtent(0), ey = validExtent(1), ez = validExtent(2);
+    std::size_t q = 0;
+    for (int b = 0; b < nblk; b++)
+        for (int k = 0; k < ez; k++)
+            for (int j = 0; j < ey; j++)
+                for (int i = 0; i < ex; i++) out[q++] = (*this)(b, i, j, k, comp);
+}
+
+void TreeData::unpackValid(int comp, int nblk, const double* in) {
+    const int ex = validExtent(0), ey = validExtent(1), ez = validExtent(2);
+    std::size_t q = 0;
+    for (int b = 0; b < nblk; b++)
+        for (int k = 0; k < ez; k++)
+            for (int j = 0; j < ey; j++)
+                for (int i = 0; i < ex; i++) (*this)(b, i, j, k, comp) = in[q++];
+}
+
+void TreeData::packGrown(int comp, int nblk, double* out) const {
+    for (int b = 0; b < nblk; b++)
+        std::memcpy(out + (std::size_t)b * compsize_,
+                    data_.data() + (std::size_t)b * blocksize_ + (std::size_t)comp * compsize_,
+                    compsize_ * sizeof(double));
+}
+
+}  // namespace amd
+}  // namespace sayaka
