@@ -112,6 +112,10 @@ int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, dou
  * the SOL field as initial guess; SOL receives the solution on convergence. */
 int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);
 
+/* Coarsest-level solve of the mixed-precision V-cycle (no reference counterpart: the FP64
+ * mode always uses the reference's 1e-6 relative + 1e-6 absolute, 2000 iterations). */
+int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* ctx, double tol_rel, double tol_abs, int max_iter);
+
 /* Set every value of a field to zero on the device (the reference's setZeroSol,
  * SayakaMacSolver.h:125-128, without a host copy). */
 int sayaka_dev_fill(sayaka_dev_ctx* ctx, int field, double value);

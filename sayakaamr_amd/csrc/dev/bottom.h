@@ -338,10 +338,15 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
             if (C::WLDS)
                 out[q] = bot_apply_cell<D, NB>(sh + blk * G::NTILE, wsh + blk * (D * G::NFACE), i, j, k, a.alpha,
                                                dvs[blk], vfs[threadIdx.x + q * NT]);
-            else
-                out[q] = cell_op<D, NB, false, double>(sh + blk * G::NTILE, a.w + gbs[blk] * (D * G::NFACE),
-                                                       a.wcf + gbs[blk] * (G::NF * G::NSLAB), i, j, k, bi[blk], 1,
-                                                       a, dvs[blk], vfs[threadIdx.x + q * NT], 0.0);
+            else {
+                double wr[6] = {0, 0, 0, 0, 0, 0}, wcr[6] = {0, 0, 0, 0, 0, 0};
+                const double* wb = a.w + gbs[blk] * (D * G::NFACE);
+#pragma unroll
+                for (int f = 0; f < G::NF; f++)
+                    wr[f] = wb[(f >> 1) * G::NFACE + G::fidx(f >> 1, i + (f == 1), j + (f == 3), k + (f == 5))];
+                out[q] = cell_op<D, NB, false, double>(sh + blk * G::NTILE, wr, wcr, i, j, k, bi[blk], 1, a.alpha,
+                                                       dvs[blk], vfs[threadIdx.x + q * NT], 0.0);
+            }
         }
         __syncthreads();   // tile free for the reduction staging
     };

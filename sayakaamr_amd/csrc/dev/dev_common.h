@@ -31,6 +31,8 @@ struct BlockInfo {
     int nbr_leaf[6];  // leaf flag of nbr[f] when nbr[f] >= 0
     int bct[6];       // math BC on wall faces (BCKind), -1 elsewhere
     int child[8];
+    int nfc[6][4];    // children of the same-level neighbour across face f that touch f,
+                      // indexed by their tangential sides s1 + 2*s2 (-1 when it has none)
 };
 
 struct BlockMetric {

@@ -286,54 +286,15 @@ __device__ inline int face_type(const BlockInfo& bi, int f, int m) {
     return FT_FF;
 }
 
-// Stage block b of field x and its one-cell face halo into LDS.  The halo value of a face
-// whose neighbour is coarser is the coarse leaf's cell that the reference's injection from
-// the parent's ghost resolves to (SayakaFillPatch.cpp:192-299, SayakaTreeData.h:41-44).
-template <int D, int NB, class T>
-__device__ inline void load_tile(T* tile, const T* __restrict__ x, const BlockInfo& bi, int b) {
-    using G = Geo<D, NB>;
-    for (int c = threadIdx.x; c < G::NCELL; c += blockDim.x) {
-        int i, j, k;
-        G::cell_ijk(c, i, j, k);
-        tile[G::tidx(i, j, k)] = x[b * G::NCELL + c];
-    }
-    for (int g = threadIdx.x; g < G::NGHOST; g += blockDim.x) {
-        const int f = g / G::NSLAB, p = g % G::NSLAB;
-        const int d = f >> 1, s = f & 1;
-        int a1, a2;
-        tang_axes<D>(d, a1, a2);
-        const int ext = d == 2 ? G::NBZ : NB;
-        int q[3] = {0, 0, 0};
-        q[a1] = p % NB;
-        if (D == 3) q[a2] = p / NB;
-        int gq[3] = {q[0], q[1], q[2]};
-        gq[d] = s ? ext : -1;
-        const int n = bi.nbr[f];
-        T v = T(0);
-        if (n >= 0) {
-            q[d] = s ? 0 : ext - 1;
-            v = x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
-        } else if (n > PHYS_BNDRY) {
-            const int cb = bi.cnbr[f];
-            const int wc = bi.which_child;
-            int r[3] = {0, 0, 0};
-            r[d] = s ? 0 : ext - 1;
-            r[a1] = (q[a1] >> 1) + side_of(wc, a1) * (NB / 2);
-            if (D == 3) r[a2] = (q[a2] >> 1) + side_of(wc, a2) * (NB / 2);
-            v = x[cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
-        }
-        tile[G::tidx(gq[0], gq[1], gq[2])] = v;
-    }
-}
-
 // Sum of the fine-side contributions w_f * phi_f over the 2^(d-1) fine faces that cover
 // one coarse face cell, in the order of restrictFaceBlockData_Average
 // (SayakaInterpPatch.cpp:659-687): higher tangential axis outer.  These are the values
 // the reference saves on FINE_CRSE faces (SayakaMacOperator.cpp:1446-1455, 1310-1319) and
-// sums into the coarse block's CRSE_FINE face (SayakaTower.cpp:360-413).
+// sums into the coarse block's CRSE_FINE face (SayakaTower.cpp:360-413).  The fine block
+// comes from the table (BlockInfo::nfc), so no dependent table load is needed.
 template <int D, int NB, class T>
-__device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* __restrict__ binfo,
-                           const T* __restrict__ x, const T* __restrict__ w) {
+__device__ inline T cf_sum(int f, int t1, int t2, const BlockInfo& bi, const T* __restrict__ x,
+                           const T* __restrict__ w) {
     using G = Geo<D, NB>;
     const int d = f >> 1, s = f & 1;
     int a1, a2;
@@ -341,10 +302,7 @@ __device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* _
     const int half = NB / 2;
     const int s1 = t1 >= half ? 1 : 0;
     const int s2 = (D == 3 && t2 >= half) ? 1 : 0;
-    int ch = (1 - s) << d;
-    ch |= s1 << a1;
-    if (D == 3) ch |= s2 << a2;
-    const int fb = binfo[nbrblk].child[ch];
+    const int fb = bi.nfc[f][s1 + 2 * s2];
     const int ext = d == 2 ? G::NBZ : NB;
     T acc = T(0);
     for (int q2 = 0; q2 <= (D == 3 ? 1 : 0); q2++) {
@@ -362,17 +320,60 @@ __device__ inline T cf_sum(int f, int t1, int t2, int nbrblk, const BlockInfo* _
     return acc;
 }
 
+// The one-cell face halo of a block into LDS (the block's own cells are already there).
+// Same-level neighbour: its boundary slab.  Coarser neighbour: the coarse leaf cell that the
+// reference's injection from the parent's ghost resolves to (SayakaFillPatch.cpp:192-299,
+// SayakaTreeData.h:41-44).  Finer neighbour on tower m (a coarse/fine face seen from the
+// coarse side): the sum of the fine w*phi over the face cell (cf_sum), which is all the
+// operator needs from that side.  Walls: 0 (walls enter through the weights).
+template <int D, int NB, class T>
+__device__ inline void load_halo(T* tile, const T* __restrict__ x, const T* __restrict__ w, const BlockInfo& bi,
+                                 int m) {
+    using G = Geo<D, NB>;
+    for (int g = threadIdx.x; g < G::NGHOST; g += blockDim.x) {
+        const int f = g / G::NSLAB, p = g % G::NSLAB;
+        const int d = f >> 1, s = f & 1;
+        int a1, a2;
+        tang_axes<D>(d, a1, a2);
+        const int ext = d == 2 ? G::NBZ : NB;
+        int q[3] = {0, 0, 0};
+        q[a1] = p % NB;
+        if (D == 3) q[a2] = p / NB;
+        int gq[3] = {q[0], q[1], q[2]};
+        gq[d] = s ? ext : -1;
+        const int n = bi.nbr[f];
+        T v = T(0);
+        if (n >= 0) {
+            if (bi.level < m && !bi.nbr_leaf[f]) {
+                v = cf_sum<D, NB, T>(f, q[a1], D == 3 ? q[a2] : 0, bi, x, w);
+            } else {
+                q[d] = s ? 0 : ext - 1;
+                v = x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
+            }
+        } else if (n > PHYS_BNDRY) {
+            const int cb = bi.cnbr[f];
+            const int wc = bi.which_child;
+            int r[3] = {0, 0, 0};
+            r[d] = s ? 0 : ext - 1;
+            r[a1] = (q[a1] >> 1) + side_of(wc, a1) * (NB / 2);
+            if (D == 3) r[a2] = (q[a2] >> 1) + side_of(wc, a2) * (NB / 2);
+            v = x[cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
+        }
+        tile[G::tidx(gq[0], gq[1], gq[2])] = v;
+    }
+}
+
 // One cell of the operator (MODE apply) or of a Gauss-Seidel update (MODE smooth):
 // mg_apply_block_usefill (SayakaMacOperator.cpp:1372-1458) and
 // mg_smooth_block_rbcycle_usefill (SayakaMacOperator.cpp:1232-1320) with the face
-// coefficients of cell_eval_coef (1115-1167) already folded into w / wcf.  wb and wcfb are
-// this block's own-face weights [D][NFACE] and coarse/fine weights [NF][NSLAB] (global or
-// LDS copies).
+// coefficients of cell_eval_coef (1115-1167) already folded into the weights: wr[f] is the
+// own weight of face f of this cell, wcr[f] the coarse-side weight of a coarse/fine face, and
+// the tile's halo holds the fine sum on such faces (load_halo).
 template <int D, int NB, bool SMOOTH, class T>
-__device__ inline T cell_op(const T* tile, const T* wb, const T* wcfb, int i, int j, int k, const BlockInfo& bi, int m,
-                            const StencilArgsT<T>& a, T dv, T vfv, T rhsv) {
+__device__ inline T cell_op(const T* tile, const T (&wr)[6], const T (&wcr)[6], int i, int j, int k,
+                            const BlockInfo& bi, int m, T alpha, T dv, T vfv, T rhsv) {
     using G = Geo<D, NB>;
-    T diag = a.alpha * dv;
+    T diag = alpha * dv;
     T offdiag = T(0);
     const T phic = tile[G::tidx(i, j, k)];
 #pragma unroll
@@ -381,27 +382,22 @@ __device__ inline T cell_op(const T* tile, const T* wb, const T* wcfb, int i, in
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int ext = d == 2 ? G::NBZ : NB;
         const bool bnd = s ? idx == ext - 1 : idx == 0;
-        const int fi = G::fidx(d, i + (f == 1), j + (f == 3), k + (f == 5));
         const int ni = G::tidx(i + (f == 1) - (f == 0), j + (f == 3) - (f == 2), k + (f == 5) - (f == 4));
         T off, offval;
         if (!bnd) {
-            off = wb[d * G::NFACE + fi];
+            off = wr[f];
             offval = off * tile[ni];
         } else {
             const int ft = face_type<D, NB>(bi, f, m);
             if (ft == FT_CF) {
-                int a1, a2;
-                tang_axes<D>(d, a1, a2);
-                const int ijk[3] = {i, j, k};
-                const int t1 = ijk[a1], t2 = D == 3 ? ijk[a2] : 0;
-                off = wcfb[f * G::NSLAB + t1 + NB * t2];
-                offval = cf_sum<D, NB, T>(f, t1, t2, bi.nbr[f], a.binfo, a.x, a.w);
+                off = wcr[f];
+                offval = tile[ni];
             } else if (ft == FT_BC) {
                 if (bi.bct[f] == BC_NEUMANN) {
                     off = T(0);
                     offval = T(0);
                 } else {
-                    off = wb[d * G::NFACE + fi];
+                    off = wr[f];
                     if (SMOOTH) {
                         offval = T(0);
                         off *= T(2);
@@ -410,7 +406,7 @@ __device__ inline T cell_op(const T* tile, const T* wb, const T* wcfb, int i, in
                     }
                 }
             } else {
-                off = wb[d * G::NFACE + fi];
+                off = wr[f];
                 offval = off * tile[ni];
             }
         }
@@ -425,20 +421,93 @@ __device__ inline T cell_op(const T* tile, const T* wb, const T* wcfb, int i, in
     }
 }
 
-template <int D, int NB, int MODE>
+// Everything one cell needs from HBM that does not depend on the block table: loaded at
+// kernel entry, so these loads overlap the table / halo round trips instead of following them.
+template <class T>
+struct CellIn {
+    T x, vf, rhs, dotv;
+    T w[6], wcf[6];
+};
+
+template <int D, int NB, int MODE, class T>
+__device__ inline void load_cell(CellIn<T>& in, const StencilArgsT<T>& a, int b, int c, bool active) {
+    using G = Geo<D, NB>;
+    int i, j, k;
+    G::cell_ijk(c, i, j, k);
+    const int gi = b * G::NCELL + c;
+    in.x = a.x[gi];
+    in.vf = in.rhs = in.dotv = T(0);
+#pragma unroll
+    for (int f = 0; f < 6; f++) in.w[f] = in.wcf[f] = T(0);
+    if (!active) return;
+    in.vf = a.vf[gi];
+    if (MODE == ST_SMOOTH || MODE == ST_RESID || MODE == ST_RESID_INIT) in.rhs = a.rhs[gi];
+    if (MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) in.dotv = a.dotv[gi];
+#pragma unroll
+    for (int f = 0; f < G::NF; f++) {
+        const int d = f >> 1, s = f & 1;
+        in.w[f] = a.w[(b * D + d) * G::NFACE + G::fidx(d, i + (f == 1), j + (f == 3), k + (f == 5))];
+        const int idx = d == 0 ? i : (d == 1 ? j : k);
+        const int ext = d == 2 ? G::NBZ : NB;
+        if (s ? idx == ext - 1 : idx == 0) {
+            int a1, a2;
+            tang_axes<D>(d, a1, a2);
+            const int ijk[3] = {i, j, k};
+            in.wcf[f] = a.wcf[(b * G::NF + f) * G::NSLAB + ijk[a1] + NB * (D == 3 ? ijk[a2] : 0)];
+        }
+    }
+}
+
+constexpr int BI_WORDS = sizeof(BlockInfo) / sizeof(int);
+
+// Operator / residual / smoother over one block list (one workgroup per block).  Dependent
+// memory round trips per workgroup: block id -> {table, own cells, weights, vf, rhs in
+// parallel} -> halo (needs the table) -> store.  FP64 modes with a fused dot product finish
+// with the grid reduction (reference order when rb.exact).
+template <int D, int NB, int MODE, class T>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
-k_stencil(StencilArgs a, RedBuf rb, int fin) {
+k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
     using G = Geo<D, NB>;
     constexpr int NT = G::THREADS;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;
-    __shared__ double tile[G::NTILE > 2 * G::NCELL + 2 ? G::NTILE : 2 * G::NCELL + 2];
+    constexpr bool REDUCE = MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT;
+    static_assert(!REDUCE || sizeof(T) == 8, "fused reductions are FP64 only");
+    __shared__ T tile[G::NTILE > 2 * G::NCELL + 2 ? G::NTILE : 2 * G::NCELL + 2];
     __shared__ double lds[3 * ((NT + 63) / 64)];
-    __shared__ BlockInfo bi;
+    __shared__ int biw[BI_WORDS];
     const int b = a.list[blockIdx.x];
-    if (threadIdx.x == 0) bi = a.binfo[b];
+    if ((int)threadIdx.x < BI_WORDS) biw[threadIdx.x] = reinterpret_cast<const int*>(a.binfo + b)[threadIdx.x];
+    // One cell per thread: prefetch everything at entry.  Several cells per thread (16^3
+    // blocks): prefetch the own values only, the rest per cell (the registers would spill).
+    constexpr bool PRE = CPT == 1;
+    CellIn<T> in[1];
+    T xo[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = threadIdx.x + q * NT;
+        if (c >= G::NCELL) continue;
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        const bool active = MODE != ST_SMOOTH || ((i + j + k) & 1) == a.colour;
+        if (PRE) {
+            load_cell<D, NB, MODE, T>(in[0], a, b, c, active);
+            xo[q] = in[0].x;
+        } else {
+            xo[q] = a.x[b * G::NCELL + c];
+        }
+    }
+    const T dv = (T)a.met[b].dv;
     __syncthreads();
-    const double dv = a.met[b].dv;
-    load_tile<D, NB, double>(tile, a.x, bi, b);
+    const BlockInfo& bi = *reinterpret_cast<const BlockInfo*>(biw);
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = threadIdx.x + q * NT;
+        if (c >= G::NCELL) continue;
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        tile[G::tidx(i, j, k)] = xo[q];
+    }
+    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
     __syncthreads();
     double q0[CPT], q1[CPT], mx = 0.0;
 #pragma unroll
@@ -449,37 +518,40 @@ k_stencil(StencilArgs a, RedBuf rb, int fin) {
         int i, j, k;
         G::cell_ijk(c, i, j, k);
         const int gi = b * G::NCELL + c;
-        const double vfv = a.vf[gi];
+        if (!PRE) {
+            const bool active = MODE != ST_SMOOTH || ((i + j + k) & 1) == a.colour;
+            if (!active) continue;
+            load_cell<D, NB, MODE, T>(in[0], a, b, c, true);
+        }
+        const T vfv = in[0].vf;
         if (MODE == ST_SMOOTH) {
             if (((i + j + k) & 1) != a.colour) continue;
-            const double u = cell_op<D, NB, true, double>(tile, a.w + b * (D * G::NFACE), a.wcf + b * (G::NF * G::NSLAB),
-                                                          i, j, k, bi, a.m, a, dv, vfv, a.rhs[gi]);
-            const_cast<double*>(a.x)[gi] = u;
+            const T u = cell_op<D, NB, true, T>(tile, in[0].w, in[0].wcf, i, j, k, bi, a.m, a.alpha, dv, vfv, in[0].rhs);
+            const_cast<T*>(a.x)[gi] = u;
         } else {
-            const double Lp = cell_op<D, NB, false, double>(tile, a.w + b * (D * G::NFACE),
-                                                            a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi, a.m, a, dv,
-                                                            vfv, 0.0);
+            const T Lp = cell_op<D, NB, false, T>(tile, in[0].w, in[0].wcf, i, j, k, bi, a.m, a.alpha, dv, vfv, T(0));
             if (MODE == ST_APPLY || MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) {
                 a.out[gi] = Lp;
-                if (MODE == ST_APPLY_DOT) q0[q] = a.dotv[gi] * Lp;   // rhat . v
+                if (MODE == ST_APPLY_DOT) q0[q] = in[0].dotv * Lp;   // rhat . v
                 if (MODE == ST_APPLY_DOT2) {
-                    q0[q] = Lp * a.dotv[gi];                          // t . s
+                    q0[q] = Lp * in[0].dotv;                          // t . s
                     q1[q] = Lp * Lp;                                  // t . t
                 }
             } else {
-                const double r = vfv > 0 ? a.rhs[gi] - Lp : 0.0;
+                const T r = vfv > 0 ? in[0].rhs - Lp : T(0);
                 a.out[gi] = r;
                 if (MODE == ST_RESID_INIT) {
                     a.out2[gi] = r;
-                    a.out3[gi] = 0.0;
+                    a.out3[gi] = T(0);
                     q0[q] = r * r;                                    // rhat . r with rhat = r
-                    mx = fmax(mx, fabs(r));
+                    mx = fmax(mx, fabs((double)r));
                 }
             }
         }
     }
-    if (MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT)
-        grid_reduce_cells<NT, G::NCELL, CPT>(q0, q1, mx, MODE == ST_APPLY_DOT2 ? 2 : 1, rb, fin, tile, lds);
+    if constexpr (REDUCE)
+        grid_reduce_cells<NT, G::NCELL, CPT>(q0, q1, mx, MODE == ST_APPLY_DOT2 ? 2 : 1, rb, fin,
+                                             reinterpret_cast<double*>(tile), lds);
 }
 
 // --------------------------------------------------------------- restriction / prolongation

@@ -33,7 +33,11 @@ struct MixedState {
     float* r32 = nullptr;
     float* u[MIX_MAXLV + 1] = {};
     float* f[MIX_MAXLV + 1] = {};
-    double bottom_tol_rel = 1.0e-6, bottom_tol_abs = 1.0e-6;
+    // The reference's bottom test (1e-6 relative + 1e-6 absolute) stops the coarse solve
+    // early once the outer residual is small, which leaves the cycle without a coarse
+    // correction late in the solve and can stall the outer BiCGSTAB (20 iterations instead of
+    // 4 on one C1 seed); a relative-only 1e-2 reduction keeps the cycle's quality constant.
+    double bottom_tol_rel = 1.0e-2, bottom_tol_abs = 0.0;
     int bottom_max_iter = 2000;
 };
 
@@ -82,37 +86,6 @@ k_mix(const int* list, int op, const BlockInfo* __restrict__ binfo, const float*
             case M_F64_TO_F32: dst[gi] = (float)src64[gi]; break;
             case M_F32_TO_F64: dst64[gi] = (double)src[gi]; break;
             default: break;
-        }
-    }
-}
-
-// FP32 sweep (SMOOTH) or residual r = f - A u (RESID) over one block list.
-template <int D, int NB, int MODE>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_stencil32(StencilArgsT<float> a) {
-    using G = Geo<D, NB>;
-    __shared__ float tile[G::NTILE];
-    __shared__ BlockInfo bi;
-    const int b = a.list[blockIdx.x];
-    if (threadIdx.x == 0) bi = a.binfo[b];
-    __syncthreads();
-    const float dv = (float)a.met[b].dv;
-    load_tile<D, NB, float>(tile, a.x, bi, b);
-    __syncthreads();
-    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
-        int i, j, k;
-        G::cell_ijk(c, i, j, k);
-        const int gi = b * G::NCELL + c;
-        const float vfv = a.vf[gi];
-        if (MODE == ST_SMOOTH) {
-            if (((i + j + k) & 1) != a.colour) continue;
-            const_cast<float*>(a.x)[gi] = cell_op<D, NB, true, float>(tile, a.w + b * (D * G::NFACE),
-                                                                          a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi,
-                                                                          a.m, a, dv, vfv, a.rhs[gi]);
-        } else {
-            const float Lp = cell_op<D, NB, false, float>(tile, a.w + b * (D * G::NFACE),
-                                                           a.wcf + b * (G::NF * G::NSLAB), i, j, k, bi, a.m, a, dv, vfv,
-                                                           0.0f);
-            a.out[gi] = vfv > 0.0f ? a.rhs[gi] - Lp : 0.0f;
         }
     }
 }

@@ -196,12 +196,12 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
     ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL,
                  a.nlist * AlgBytes<D, NB, double>::stencil(mode));
     switch (mode) {
-        case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_SMOOTH: hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_RESID_INIT: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID_INIT>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_SMOOTH: hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_RESID_INIT: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID_INIT, double>), grid, block, 0, c->stream, a, rb, fin); break;
         default: return fail(-2, "bad stencil mode");
     }
     HIPCHK(hipGetLastError());
@@ -406,7 +406,8 @@ struct Impl {
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL,
                              sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH));
-                hipLaunchKernelGGL((k_stencil32<D, NB, ST_SMOOTH>), dim3(sub.n), dim3(G::THREADS), 0, c->stream, a);
+                hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0, c->stream, a,
+                                   RedBuf{}, FIN_NONE);
             }
         }
         HIPCHK(hipGetLastError());
@@ -433,7 +434,8 @@ struct Impl {
             a.x = ms.u[l];
             a.rhs = ms.f[l];
             a.out = ms.r32;
-            hipLaunchKernelGGL((k_stencil32<D, NB, ST_RESID>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a);
+            hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID, float>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a,
+                               RedBuf{}, FIN_NONE);
             HIPCHK(hipGetLastError());
         }
         if ((e = mix(c, below, M_COPY, ms.f[l - 1], ms.r32, nullptr, nullptr))) return e;
@@ -775,6 +777,8 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
         bi.parent = parent[b];
         bi.which_child = which_child[b];
         for (int q = 0; q < 8; q++) bi.child[q] = child8[b * 8 + q];
+        for (int f = 0; f < 6; f++)
+            for (int q = 0; q < 4; q++) bi.nfc[f][q] = -1;
         for (int f = 0; f < 6; f++) {
             const int n = f < c->nf ? nbr6[b * 6 + f] : PHYS_BNDRY - 1;
             bi.nbr[f] = n;
@@ -782,7 +786,21 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
             bi.nbr_leaf[f] = 0;
             bi.bct[f] = -1;
             if (n >= nblk) return fail(-1, "neighbour id out of range");
-            if (n >= 0) bi.nbr_leaf[f] = node_type[n] == 1;
+            if (n >= 0) {
+                bi.nbr_leaf[f] = node_type[n] == 1;
+                if (!bi.nbr_leaf[f] && child8[n * 8] >= 0) {
+                    // children of the neighbour that touch face f, by tangential sides (s1, s2)
+                    const int d = f >> 1, side = f & 1;
+                    const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+                    for (int s2 = 0; s2 < (c->ndim == 3 ? 2 : 1); s2++)
+                        for (int s1 = 0; s1 < 2; s1++) {
+                            int ch = (1 - side) << d;
+                            ch |= s1 << a1;
+                            if (c->ndim == 3) ch |= s2 << a2;
+                            bi.nfc[f][s1 + 2 * s2] = child8[n * 8 + ch];
+                        }
+                }
+            }
             else if (n > PHYS_BNDRY) {
                 if (parent[b] < 0) return fail(-1, "coarse face on a root block");
                 const int cn = nbr6[parent[b] * 6 + f];
@@ -1034,6 +1052,14 @@ int sayaka_dev_fill(sayaka_dev_ctx* c, int field, double value) {
     });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* c, double tol_rel, double tol_abs, int max_iter) {
+    if (!c || tol_rel < 0 || tol_abs < 0 || max_iter < 1) return fail(-1, "bad argument");
+    c->mixed.bottom_tol_rel = tol_rel;
+    c->mixed.bottom_tol_abs = tol_abs;
+    c->mixed.bottom_max_iter = max_iter;
     return 0;
 }
 

@@ -54,6 +54,7 @@ _SIG = {
     "sayaka_dev_solve": ([vp, C.POINTER(SolveParams), C.POINTER(SolveStats)], i32),
     "sayaka_dev_sync": ([vp], i32),
     "sayaka_dev_fill": ([vp, i32, f64], i32),
+    "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
     "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                  C.POINTER(f64)], i32),
@@ -218,6 +219,9 @@ class MacSolverGPU:
     def zero(self, field):
         """setZeroSol on the device for any field."""
         _chk(self.lib.sayaka_dev_fill(self.h, field, 0.0), "fill")
+
+    def set_mixed_bottom(self, tol_rel, tol_abs, max_iter):
+        _chk(self.lib.sayaka_dev_set_mixed_bottom(self.h, tol_rel, tol_abs, max_iter), "set_mixed_bottom")
 
     def profile(self, kclass, enable=True):
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")
