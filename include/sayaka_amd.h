@@ -137,6 +137,37 @@ int sayaka_dev_profile(sayaka_dev_ctx* ctx, int kclass, int enable);
 int sayaka_dev_profile_read(sayaka_dev_ctx* ctx, int kclass, double* total_ms, long long* launches,
                             long long* cells, double* alg_bytes);
 
+/* ---- learned initial guess (no reference counterpart) ---------------------------------
+ * A per-block 3D CNN (12^3 window: the block and two same-level ghost cells per face; 5 input
+ * channels: scaled rhs, three cell-averaged B*a, volume fraction; four 3x3x3 convolutions
+ * 16-16-16-1 with ReLU; BF16 weights and activations, FP32 sums on the matrix cores) that maps
+ * each leaf block to an initial guess of its cells.  8^3 blocks in 3D only.
+ * Parameters: 4 layers x (27*16*16 weights [tap][ci][co] + 16 biases), FP32. */
+int sayaka_cnn_setup(sayaka_dev_ctx* ctx, int max_windows);
+int sayaka_cnn_set_params(sayaka_dev_ctx* ctx, const float* params);
+int sayaka_cnn_get_params(sayaka_dev_ctx* ctx, float* params);
+/* training store: windows of the current tree's leaves (ids on the host) with targets
+ * (SOL - shift) * t_scale; collect returns the store size */
+int sayaka_cnn_store_reset(sayaka_dev_ctx* ctx, long long capacity);
+long long sayaka_cnn_collect(sayaka_dev_ctx* ctx, const int* leaf_ids, int n, double s_rhs, double shift,
+                             double t_scale);
+int sayaka_cnn_store_read(sayaka_dev_ctx* ctx, long long first, int n, float* windows, float* targets);
+/* one Adam step (lr) on the MSE of the store windows idx[n]; loss of the batch before the step */
+int sayaka_cnn_train_step(sayaka_dev_ctx* ctx, const long long* idx, int n, float lr, float* loss);
+/* loss, parameter gradients and (optional) FP32 outputs [n][12^3] without an update */
+int sayaka_cnn_loss_grad(sayaka_dev_ctx* ctx, const long long* idx, int n, float* loss, float* grad, float* out);
+/* initial guess of every leaf block into `field` (output * out_scale; solid cells and
+ * non-leaf blocks zero) */
+int sayaka_cnn_guess(sayaka_dev_ctx* ctx, int field, double s_rhs, double out_scale);
+/* Inputs of the learned-guess workload for the current tree, generated on the device: solid
+ * sphere (centre, radius) with cut-cell fractions (nsub samples per edge), B = 1/rho
+ * (rho = 1 + (ratio-1) H_eps(-phi)), rhs = sum of nblob Gaussians (centres [nblob][3], widths,
+ * amplitudes) on fluid cells projected to zero mean (the removed mean in *mean_out).
+ * lo3 [nblk][3] and dx [nblk] describe the blocks. */
+int sayaka_gen_sphere_blobs(sayaka_dev_ctx* ctx, const double* lo3, const double* dx, const double* centre,
+                            double radius, double ratio, double eps, int nsub, int nblob, const double* blob_c,
+                            const double* blob_w, const double* blob_a, double* mean_out);
+
 /* Block until the context's stream is idle. */
 int sayaka_dev_sync(sayaka_dev_ctx* ctx);
 

@@ -37,6 +37,10 @@ void sy_tree_export(const sy_tree* t, int* level, int* parent, int* which_child,
                     int* nbr6, int* node_type, int* node_type2, double* lo3, double* hi3,
                     double* center3, double* length3, int* surr27, int* surrtype27,
                     int* surrbcref27);
+/* Refine level by level around the shell |x - centre| = r0 (blocks with a cell centre within
+ * factor * blockLength of it), as the demos' refinement loops do with their own criteria
+ * (Test7.h:411-418).  Returns numBlocks, -1 on pool overflow. */
+int sy_tree_build_shell(sy_tree* t, const double* centre, double r0, double factor);
 /* AmrTree::getTreeLevel (SayakaTree.h:317-318) */
 int sy_tree_level_count(const sy_tree* t, int level);
 void sy_tree_level_blocks(const sy_tree* t, int level, int* out);

@@ -38,6 +38,7 @@ _HOST_SIG = {
     "sy_tree_export": ([vp] + [vp] * 14, None),
     "sy_tree_level_count": ([vp, i32], i32),
     "sy_tree_level_blocks": ([vp, i32, vp], None),
+    "sy_tree_build_shell": ([vp, vp, f64, f64], i32),
     "sy_mac_last_error": ([], C.c_char_p),
     "sy_mac_create": ([vp, i32], vp),
     "sy_mac_destroy": ([vp], None),

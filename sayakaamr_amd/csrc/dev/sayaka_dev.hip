@@ -16,6 +16,8 @@
 #include "../../../include/sayaka_amd.h"
 #include "bottom.h"
 #include "coef.h"
+#include "cnn.h"
+#include "cnn_data.h"
 #include "kernels.h"
 #include "mixed.h"
 
@@ -81,6 +83,26 @@ struct sayaka_dev_ctx {
     int* h_bot_stats = nullptr;  // pinned
 
     MixedState mixed;
+    // learned initial guess (8^3 blocks in 3D): parameters, activations, training store
+    struct Cnn {
+        int cap = 0;                      // windows per forward / backward pass
+        float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+        __bf16* wf[cnn::NLAYER] = {};
+        __bf16* wb[cnn::NLAYER] = {};
+        __bf16* act[cnn::NLAYER] = {};    // act[0] input windows, act[l] output of layer l
+        __bf16 *g0 = nullptr, *g1 = nullptr;   // activation gradients (ping-pong)
+        float* out32 = nullptr;
+        float* tgt = nullptr;
+        float* loss = nullptr;
+        int* list = nullptr;              // leaf blocks of the current batch / guess
+        int list_cap = 0;
+        __bf16* store = nullptr;          // training windows [n][NV][C]
+        float* store_t = nullptr;         // training targets [n][512]
+        long long store_n = 0, store_cap = 0;
+        int step = 0;
+    } cnn;
+    double* d_gen = nullptr;              // generator scratch: block corners, sizes, partial sums
+    int gen_cap = 0;
     // per-kernel-class launch timing (HIP events on the context stream)
     struct ProfRec {
         hipEvent_t a, b;
@@ -95,6 +117,8 @@ struct sayaka_dev_ctx {
 };
 
 namespace {
+
+void cnn_free(sayaka_dev_ctx::Cnn& n);
 
 hipEvent_t pool_event(sayaka_dev_ctx* c) {
     if (!c->ev_pool.empty()) {
@@ -746,6 +770,8 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->h_scal) hipHostFree(c->h_scal);
     if (c->h_bot_stats) hipHostFree(c->h_bot_stats);
     mixed_free(c->mixed);
+    cnn_free(c->cnn);
+    if (c->d_gen) (void)hipFree(c->d_gen);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1099,6 +1125,323 @@ int sayaka_dev_profile_read(sayaka_dev_ctx* c, int kclass, double* total_ms, lon
 int sayaka_dev_sync(sayaka_dev_ctx* c) {
     if (!c) return fail(-1, "null ctx");
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"
+
+// ================================================================ learned initial guess
+namespace {
+
+constexpr int CNN_PL = cnn::NTAP * cnn::C * cnn::C + cnn::C;   // parameters per layer (padded)
+constexpr int CNN_NP = cnn::NLAYER * CNN_PL;
+
+void cnn_free(sayaka_dev_ctx::Cnn& n) {
+    void* ptrs[] = {n.p, n.g, n.m, n.v, n.g0, n.g1, n.out32, n.tgt, n.loss, n.list, n.store, n.store_t};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (int l = 0; l < cnn::NLAYER; l++) {
+        if (n.wf[l]) (void)hipFree(n.wf[l]);
+        if (n.wb[l]) (void)hipFree(n.wb[l]);
+        if (n.act[l]) (void)hipFree(n.act[l]);
+    }
+    n = sayaka_dev_ctx::Cnn{};
+}
+
+int cnn_pack(sayaka_dev_ctx* c) {
+    for (int l = 0; l < cnn::NLAYER; l++)
+        hipLaunchKernelGGL(cnn::k_cnn_pack, dim3(16), dim3(256), 0, c->stream, c->cnn.p + l * CNN_PL, c->cnn.wf[l],
+                           c->cnn.wb[l]);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int cnn_forward(sayaka_dev_ctx* c, int n) {
+    auto& N = c->cnn;
+    const dim3 g(n), b(cnn::NT);
+    const float* bias[4];
+    for (int l = 0; l < 4; l++) bias[l] = N.p + l * CNN_PL + cnn::NTAP * cnn::C * cnn::C;
+    hipLaunchKernelGGL((cnn::k_conv_fwd<true, false>), g, b, 0, c->stream, N.act[0], N.wf[0], bias[0], N.act[1], nullptr);
+    hipLaunchKernelGGL((cnn::k_conv_fwd<true, false>), g, b, 0, c->stream, N.act[1], N.wf[1], bias[1], N.act[2], nullptr);
+    hipLaunchKernelGGL((cnn::k_conv_fwd<true, false>), g, b, 0, c->stream, N.act[2], N.wf[2], bias[2], N.act[3], nullptr);
+    hipLaunchKernelGGL((cnn::k_conv_fwd<false, true>), g, b, 0, c->stream, N.act[3], N.wf[3], bias[3], nullptr, N.out32);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// loss and parameter gradients of the last forward pass (targets in N.tgt)
+int cnn_backward(sayaka_dev_ctx* c, int n) {
+    auto& N = c->cnn;
+    const dim3 g(n), b(cnn::NT);
+    HIPCHK(hipMemsetAsync(N.g, 0, CNN_NP * sizeof(float), c->stream));
+    HIPCHK(hipMemsetAsync(N.loss, 0, sizeof(float), c->stream));
+    float* dw[4];
+    float* db[4];
+    for (int l = 0; l < 4; l++) {
+        dw[l] = N.g + l * CNN_PL;
+        db[l] = dw[l] + cnn::NTAP * cnn::C * cnn::C;
+    }
+    const float scale = 1.0f / ((float)n * 512.0f);
+    hipLaunchKernelGGL(cnn::k_cnn_loss, g, b, 0, c->stream, N.out32, N.tgt, scale, N.g0, N.loss);
+    // layer 4 (no activation): dP4 = dY4 (g0)
+    hipLaunchKernelGGL((cnn::k_conv_bwd_w<false>), g, b, 0, c->stream, N.act[3], N.g0, N.g0, dw[3], db[3]);
+    hipLaunchKernelGGL((cnn::k_conv_bwd_data<false>), g, b, 0, c->stream, N.g0, N.g0, N.wb[3], N.g1);
+    // layer 3: dY3 in g1, Y3 = act[3]
+    hipLaunchKernelGGL((cnn::k_conv_bwd_w<true>), g, b, 0, c->stream, N.act[2], N.g1, N.act[3], dw[2], db[2]);
+    hipLaunchKernelGGL((cnn::k_conv_bwd_data<true>), g, b, 0, c->stream, N.g1, N.act[3], N.wb[2], N.g0);
+    // layer 2: dY2 in g0, Y2 = act[2]
+    hipLaunchKernelGGL((cnn::k_conv_bwd_w<true>), g, b, 0, c->stream, N.act[1], N.g0, N.act[2], dw[1], db[1]);
+    hipLaunchKernelGGL((cnn::k_conv_bwd_data<true>), g, b, 0, c->stream, N.g0, N.act[2], N.wb[1], N.g1);
+    // layer 1: dY1 in g1, Y1 = act[1]
+    hipLaunchKernelGGL((cnn::k_conv_bwd_w<true>), g, b, 0, c->stream, N.act[0], N.g1, N.act[1], dw[0], db[0]);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int cnn_fetch(sayaka_dev_ctx* c, const long long* idx, int n) {
+    auto& N = c->cnn;
+    for (int k = 0; k < n; k++)
+        if (idx[k] < 0 || idx[k] >= N.store_n) return fail(-1, "training index out of range");
+    long long* d_idx = reinterpret_cast<long long*>(N.list);   // list holds cap ints = cap/2 indices
+    if (n > N.cap / 2) return fail(-1, "batch larger than half the window capacity");
+    HIPCHK(hipMemcpyAsync(d_idx, idx, n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(cnn::k_cnn_fetch, dim3(n), dim3(cnn::NT), 0, c->stream, N.store, N.store_t, d_idx, N.act[0],
+                       N.tgt);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sayaka_cnn_setup(sayaka_dev_ctx* c, int max_windows) {
+    if (!c || max_windows < 2) return fail(-1, "bad argument");
+    if (c->ndim != 3 || c->nb != 8) return fail(-1, "the learned guess is defined for 8^3 blocks in 3D");
+    HIPCHK(hipSetDevice(c->device));
+    cnn_free(c->cnn);
+    auto& N = c->cnn;
+    N.cap = max_windows;
+    const size_t na = (size_t)max_windows * cnn::NV * cnn::C;
+    hipError_t e = hipSuccess;
+    for (float** q : {&N.p, &N.g, &N.m, &N.v})
+        if (e == hipSuccess) e = hipMalloc(q, CNN_NP * sizeof(float));
+    for (int l = 0; l < cnn::NLAYER && e == hipSuccess; l++) {
+        e = hipMalloc(&N.wf[l], cnn::WSZ * sizeof(__bf16));
+        if (e == hipSuccess) e = hipMalloc(&N.wb[l], cnn::WSZ * sizeof(__bf16));
+        if (e == hipSuccess) e = hipMalloc(&N.act[l], na * sizeof(__bf16));
+    }
+    if (e == hipSuccess) e = hipMalloc(&N.g0, na * sizeof(__bf16));
+    if (e == hipSuccess) e = hipMalloc(&N.g1, na * sizeof(__bf16));
+    if (e == hipSuccess) e = hipMalloc(&N.out32, (size_t)max_windows * cnn::NV * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&N.tgt, (size_t)max_windows * 512 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&N.loss, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&N.list, (size_t)max_windows * sizeof(int));
+    for (float* q : {N.p, N.m, N.v})
+        if (e == hipSuccess && q) e = hipMemset(q, 0, CNN_NP * sizeof(float));
+    if (e != hipSuccess) {
+        cnn_free(N);
+        return fail(-3, std::string("cnn allocation: ") + hipGetErrorString(e));
+    }
+    return cnn_pack(c);
+}
+
+int sayaka_cnn_set_params(sayaka_dev_ctx* c, const float* params) {
+    if (!c || !params || !c->cnn.p) return fail(-1, "cnn not set up");
+    auto& N = c->cnn;
+    HIPCHK(hipMemcpy(N.p, params, CNN_NP * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(N.m, 0, CNN_NP * sizeof(float)));
+    HIPCHK(hipMemset(N.v, 0, CNN_NP * sizeof(float)));
+    N.step = 0;
+    int e = cnn_pack(c);
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_cnn_get_params(sayaka_dev_ctx* c, float* params) {
+    if (!c || !params || !c->cnn.p) return fail(-1, "cnn not set up");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(params, c->cnn.p, CNN_NP * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sayaka_cnn_store_reset(sayaka_dev_ctx* c, long long cap) {
+    if (!c || !c->cnn.p || cap < 1) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    if (N.store) (void)hipFree(N.store);
+    if (N.store_t) (void)hipFree(N.store_t);
+    N.store = nullptr;
+    N.store_t = nullptr;
+    N.store_n = N.store_cap = 0;
+    HIPCHK(hipMalloc(&N.store, (size_t)cap * cnn::NV * cnn::C * sizeof(__bf16)));
+    HIPCHK(hipMalloc(&N.store_t, (size_t)cap * 512 * sizeof(float)));
+    N.store_cap = cap;
+    return 0;
+}
+
+// windows of leaf blocks (host ids) of the current tree from RHS and the coefficients, targets
+// (SOL - shift) * t_scale, appended to the training store; returns the store size
+long long sayaka_cnn_collect(sayaka_dev_ctx* c, const int* leaf_ids, int n, double s_rhs, double shift,
+                             double t_scale) {
+    if (!c || !leaf_ids || n < 1 || !c->cnn.store) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    if (N.store_n + n > N.store_cap) return fail(-1, "training store full");
+    if (n > N.cap) return fail(-1, "more windows than the capacity");
+    for (int k = 0; k < n; k++)
+        if (leaf_ids[k] < 0 || leaf_ids[k] >= c->nblk || !c->h_binfo[leaf_ids[k]].leaf) return fail(-1, "not a leaf block");
+    HIPCHK(hipMemcpyAsync(N.list, leaf_ids, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    cnn::GatherArgs g{N.list, c->d_binfo, c->d_field[SAYAKA_RHS], c->d_b, c->d_af, c->d_vf, s_rhs};
+    hipLaunchKernelGGL(cnn::k_cnn_gather, dim3(n), dim3(cnn::NT), 0, c->stream, g,
+                       N.store + (size_t)N.store_n * cnn::NV * cnn::C);
+    hipLaunchKernelGGL(cnn::k_cnn_target, dim3(n), dim3(256), 0, c->stream, N.list, c->d_field[SAYAKA_SOL], shift,
+                       t_scale, N.store_t + (size_t)N.store_n * 512);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    N.store_n += n;
+    return N.store_n;
+}
+
+int sayaka_cnn_train_step(sayaka_dev_ctx* c, const long long* idx, int n, float lr, float* loss) {
+    if (!c || !idx || n < 1 || !c->cnn.p) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    int e;
+    if ((e = cnn_fetch(c, idx, n)) || (e = cnn_forward(c, n)) || (e = cnn_backward(c, n))) return e;
+    N.step++;
+    cnn::AdamArgs a{N.p, N.g, N.m, N.v, CNN_NP, lr, 0.9f, 0.999f, 1e-8f,
+                    1.0f - powf(0.9f, (float)N.step), 1.0f - powf(0.999f, (float)N.step)};
+    hipLaunchKernelGGL(cnn::k_adam, dim3((CNN_NP + 255) / 256), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    if ((e = cnn_pack(c))) return e;
+    if (loss) {
+        HIPCHK(hipMemcpyAsync(loss, N.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *loss /= (float)n * 512.0f;
+    }
+    return 0;
+}
+
+// loss and gradients (no update) of the store windows idx; grad has the parameter layout
+int sayaka_cnn_loss_grad(sayaka_dev_ctx* c, const long long* idx, int n, float* loss, float* grad, float* out32) {
+    if (!c || !idx || n < 1 || !c->cnn.p) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    int e;
+    if ((e = cnn_fetch(c, idx, n)) || (e = cnn_forward(c, n)) || (e = cnn_backward(c, n))) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float l = 0.0f;
+    HIPCHK(hipMemcpy(&l, N.loss, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = l / ((float)n * 512.0f);
+    if (grad) HIPCHK(hipMemcpy(grad, N.g, CNN_NP * sizeof(float), hipMemcpyDeviceToHost));
+    if (out32) HIPCHK(hipMemcpy(out32, N.out32, (size_t)n * cnn::NV * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// copy store windows to the host (tests): [n][NV][C] as float
+int sayaka_cnn_store_read(sayaka_dev_ctx* c, long long first, int n, float* windows, float* targets) {
+    if (!c || !c->cnn.store || first < 0 || first + n > c->cnn.store_n) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    std::vector<__bf16> tmp((size_t)n * cnn::NV * cnn::C);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(tmp.data(), N.store + (size_t)first * cnn::NV * cnn::C, tmp.size() * sizeof(__bf16),
+                     hipMemcpyDeviceToHost));
+    if (windows)
+        for (size_t q = 0; q < tmp.size(); q++) windows[q] = (float)tmp[q];
+    if (targets)
+        HIPCHK(hipMemcpy(targets, N.store_t + (size_t)first * 512, (size_t)n * 512 * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// initial guess of every leaf block of the finest tower into `field` (other blocks zero)
+int sayaka_cnn_guess(sayaka_dev_ctx* c, int field, double s_rhs, double out_scale) {
+    if (!c || !c->cnn.p || !field_ok(field)) return fail(-1, "bad argument");
+    auto& N = c->cnn;
+    std::vector<int> leaves;
+    for (int b = 0; b < c->nblk; b++)
+        if (c->h_binfo[b].leaf) leaves.push_back(b);
+    int e = dispatch(c, [&](auto I) {
+        using Im = decltype(I);
+        VecArgs a = vec_args(c, c->all_blocks);
+        a.dst = c->d_field[field];
+        return launch_vec<Im::D_, Im::NB_>(c, c->all_blocks, V_ZERO, a, FIN_NONE);
+    });
+    if (e) return e;
+    for (size_t off = 0; off < leaves.size(); off += N.cap) {
+        const int n = (int)std::min<size_t>(N.cap, leaves.size() - off);
+        HIPCHK(hipMemcpyAsync(N.list, leaves.data() + off, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        cnn::GatherArgs g{N.list, c->d_binfo, c->d_field[SAYAKA_RHS], c->d_b, c->d_af, c->d_vf, s_rhs};
+        hipLaunchKernelGGL(cnn::k_cnn_gather, dim3(n), dim3(cnn::NT), 0, c->stream, g, N.act[0]);
+        if ((e = cnn_forward(c, n))) return e;
+        hipLaunchKernelGGL(cnn::k_cnn_emit, dim3(n), dim3(256), 0, c->stream, N.list, N.out32, c->d_vf, out_scale,
+                           c->d_field[field]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));   // N.list is reused by the next batch
+    }
+    return 0;
+}
+
+// Inputs of a solid sphere in a density-1 fluid (cut-cell fractions, B = 1/rho) and a
+// Gaussian-blob rhs projected to zero mean over the fluid leaves, generated on the device for
+// the current tree (lo3 [nblk][3], dx [nblk] from the tree).  Returns the removed mean.
+int sayaka_gen_sphere_blobs(sayaka_dev_ctx* c, const double* lo3, const double* dx, const double* centre, double radius,
+                            double ratio, double eps, int nsub, int nblob, const double* blob_c, const double* blob_w,
+                            const double* blob_a, double* mean_out) {
+    if (!c || !lo3 || !dx || !centre || nsub < 1 || nblob < 0 || nblob > cnn::MAXBLOB) return fail(-1, "bad argument");
+    if (c->ndim != 3) return fail(-1, "3D only");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->nblk;
+    if (c->gen_cap < n) {
+        if (c->d_gen) (void)hipFree(c->d_gen);
+        c->d_gen = nullptr;
+        HIPCHK(hipMalloc(&c->d_gen, (size_t)n * 6 * sizeof(double)));
+        c->gen_cap = n;
+    }
+    double* d_lo = c->d_gen;
+    double* d_dx = c->d_gen + 3 * n;
+    double* d_part = c->d_gen + 4 * n;
+    HIPCHK(hipMemcpyAsync(d_lo, lo3, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_dx, dx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    cnn::SphereArgs sa{d_lo, d_dx, {centre[0], centre[1], centre[2]}, radius, ratio, eps, nsub};
+    cnn::BlobArgs ba{};
+    ba.lo = d_lo;
+    ba.dx = d_dx;
+    ba.nblob = nblob;
+    for (int q = 0; q < nblob; q++) {
+        for (int d = 0; d < 3; d++) ba.c[q][d] = blob_c[3 * q + d];
+        ba.w[q] = blob_w[q];
+        ba.amp[q] = blob_a[q];
+    }
+    int e = 0;
+    if (c->nb == 8)
+        hipLaunchKernelGGL(cnn::k_gen_sphere<8>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
+    else if (c->nb == 16)
+        hipLaunchKernelGGL(cnn::k_gen_sphere<16>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
+    else
+        return fail(-1, "8^3 or 16^3 blocks");
+    HIPCHK(hipGetLastError());
+    // synchronise coarse/fine faces, average down, weights
+    e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
+    if (e) return e;
+    c->coef_version++;
+    double* rhs = c->d_field[SAYAKA_RHS];
+    if (c->nb == 8)
+        hipLaunchKernelGGL(cnn::k_gen_blobs<8>, dim3(n), dim3(256), 0, c->stream, ba, c->d_binfo, c->d_vf, rhs, d_part);
+    else
+        hipLaunchKernelGGL(cnn::k_gen_blobs<16>, dim3(n), dim3(256), 0, c->stream, ba, c->d_binfo, c->d_vf, rhs, d_part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(2 * n);
+    HIPCHK(hipMemcpyAsync(part.data(), d_part, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0, w = 0.0;
+    for (int b = 0; b < n; b++) {
+        s += part[2 * b];
+        w += part[2 * b + 1];
+    }
+    const double mean = w > 0 ? s / w : 0.0;
+    const int ncell = n * c->ncell;
+    hipLaunchKernelGGL(cnn::k_sub_mean, dim3(1024), dim3(256), 0, c->stream, rhs, c->d_vf, mean, ncell);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (mean_out) *mean_out = mean;
     return 0;
 }
 

@@ -1,4 +1,6 @@
 // capi_tree.cpp -- extern "C" entry points of the host tree (for ctypes and other FFIs).
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -79,6 +81,42 @@ void sy_tree_export(const sy_tree* h, int* level, int* parent, int* which_child,
         std::memcpy(surrtype27 + 27 * b, &s.types[0][0][0], 27 * sizeof(int));
         std::memcpy(surrbcref27 + 27 * b, &s.bcrefs[0][0][0], 27 * sizeof(int));
     }
+}
+
+// Refinement around a spherical shell, level by level from minAmrLevel: a leaf (or parent of
+// leaves) of level l < maxAmrLevel is flagged when one of its cell centres
+// x = lo + (i + 1/2) * dx lies within factor * blockLength of |x - c| = r0; one regrid per level
+// plus a final one (the same procedure as workloads.build_tree with shell_criterion).
+int sy_tree_build_shell(sy_tree* h, const double* centre, double r0, double factor) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    const int nb = t->nb, nz = t->ndim == 3 ? nb : 1;
+    for (int level = t->minAmrLevel; level <= t->maxAmrLevel; level++) {
+        std::fill(t->refineFlag.begin(), t->refineFlag.end(), 0);
+        if (level < t->maxAmrLevel)
+            for (int b = 0; b < t->numBlocks; b++) {
+                const AmrTreeNode& n = t->blocks[b];
+                if (n.refineLevel != level || !(n.nodeType == NodeIsLeaf || n.nodeType == NodeHasLeafChild)) continue;
+                double dx[3];
+                for (int d = 0; d < 3; d++) dx[d] = n.blockLength(d) / nb;
+                const double half = factor * n.blockLength(0);
+                bool hit = false;
+                for (int k = 0; k < nz && !hit; k++)
+                    for (int j = 0; j < nb && !hit; j++)
+                        for (int i = 0; i < nb && !hit; i++) {
+                            const double x = n.boundBox.vlo(0) + (i + 0.5) * dx[0];
+                            const double y = n.boundBox.vlo(1) + (j + 0.5) * dx[1];
+                            double r2 = (x - centre[0]) * (x - centre[0]) + (y - centre[1]) * (y - centre[1]);
+                            if (t->ndim == 3) {
+                                const double z = n.boundBox.vlo(2) + (k + 0.5) * dx[2];
+                                r2 += (z - centre[2]) * (z - centre[2]);
+                            }
+                            hit = std::fabs(std::sqrt(r2) - r0) < half;
+                        }
+                t->refineFlag[b] = hit ? 1 : 0;
+            }
+        if (t->regrid() != 0) return -1;
+    }
+    return t->numBlocks;
 }
 
 int sy_tree_level_count(const sy_tree* h, int l) {

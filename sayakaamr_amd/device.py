@@ -55,6 +55,16 @@ _SIG = {
     "sayaka_dev_sync": ([vp], i32),
     "sayaka_dev_fill": ([vp, i32, f64], i32),
     "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
+    "sayaka_cnn_setup": ([vp, i32], i32),
+    "sayaka_cnn_set_params": ([vp, vp], i32),
+    "sayaka_cnn_get_params": ([vp, vp], i32),
+    "sayaka_cnn_store_reset": ([vp, C.c_longlong], i32),
+    "sayaka_cnn_collect": ([vp, vp, i32, f64, f64, f64], C.c_longlong),
+    "sayaka_cnn_train_step": ([vp, vp, i32, C.c_float, C.POINTER(C.c_float)], i32),
+    "sayaka_cnn_loss_grad": ([vp, vp, i32, C.POINTER(C.c_float), vp, vp], i32),
+    "sayaka_cnn_store_read": ([vp, C.c_longlong, i32, vp, vp], i32),
+    "sayaka_cnn_guess": ([vp, i32, f64, f64], i32),
+    "sayaka_gen_sphere_blobs": ([vp, vp, vp, vp, f64, f64, f64, i32, i32, vp, vp, vp, C.POINTER(f64)], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
     "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                  C.POINTER(f64)], i32),
@@ -156,6 +166,12 @@ class MacSolverGPU:
         _chk(self.lib.sayaka_dev_get_coefs(self.h, N.ptr(vf), pa, pb, N.ptr(w), N.ptr(wcf)), "get_coefs")
         return vf, kb, ka, w, wcf
 
+    def get_vf(self):
+        """Volume fractions of every block (the solver's coefficient state)."""
+        vf = np.zeros((self.n, self.ncell))
+        _chk(self.lib.sayaka_dev_get_coefs(self.h, N.ptr(vf), None, None, None, None), "get_coefs")
+        return vf
+
     # --- fields ----------------------------------------------------------------------
     def upload(self, field, x):
         _chk(self.lib.sayaka_dev_upload(self.h, field, N.ptr(np.ascontiguousarray(x, dtype=np.float64))), "upload")
@@ -222,6 +238,66 @@ class MacSolverGPU:
 
     def set_mixed_bottom(self, tol_rel, tol_abs, max_iter):
         _chk(self.lib.sayaka_dev_set_mixed_bottom(self.h, tol_rel, tol_abs, max_iter), "set_mixed_bottom")
+
+    # --- learned initial guess (include/sayaka_amd.h, sayaka_cnn_*) -------------------
+    CNN_NP = 4 * (27 * 16 * 16 + 16)
+
+    def cnn_setup(self, max_windows):
+        _chk(self.lib.sayaka_cnn_setup(self.h, max_windows), "cnn_setup")
+
+    def cnn_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.CNN_NP
+        _chk(self.lib.sayaka_cnn_set_params(self.h, N.ptr(p)), "cnn_set_params")
+
+    def cnn_get_params(self):
+        p = np.zeros(self.CNN_NP, np.float32)
+        _chk(self.lib.sayaka_cnn_get_params(self.h, N.ptr(p)), "cnn_get_params")
+        return p
+
+    def cnn_store_reset(self, cap):
+        _chk(self.lib.sayaka_cnn_store_reset(self.h, int(cap)), "cnn_store_reset")
+
+    def cnn_collect(self, leaf_ids, s_rhs, shift, t_scale):
+        ids = np.ascontiguousarray(leaf_ids, dtype=np.int32)
+        return _chk(self.lib.sayaka_cnn_collect(self.h, N.ptr(ids), ids.size, s_rhs, shift, t_scale), "cnn_collect")
+
+    def cnn_train_step(self, idx, lr):
+        i = np.ascontiguousarray(idx, dtype=np.int64)
+        loss = C.c_float()
+        _chk(self.lib.sayaka_cnn_train_step(self.h, N.ptr(i), i.size, lr, C.byref(loss)), "cnn_train_step")
+        return loss.value
+
+    def cnn_loss_grad(self, idx, want_out=False):
+        i = np.ascontiguousarray(idx, dtype=np.int64)
+        loss = C.c_float()
+        g = np.zeros(self.CNN_NP, np.float32)
+        out = np.zeros((i.size, 12 ** 3), np.float32) if want_out else None
+        _chk(self.lib.sayaka_cnn_loss_grad(self.h, N.ptr(i), i.size, C.byref(loss), N.ptr(g), N.ptr(out)),
+             "cnn_loss_grad")
+        return loss.value, g, out
+
+    def cnn_store_read(self, first, n):
+        w = np.zeros((n, 12 ** 3, 16), np.float32)
+        t = np.zeros((n, 512), np.float32)
+        _chk(self.lib.sayaka_cnn_store_read(self.h, first, n, N.ptr(w), N.ptr(t)), "cnn_store_read")
+        return w, t
+
+    def cnn_guess(self, field, s_rhs, out_scale):
+        _chk(self.lib.sayaka_cnn_guess(self.h, field, s_rhs, out_scale), "cnn_guess")
+
+    def gen_sphere_blobs(self, tree, centre, radius, ratio, eps, nsub, blobs):
+        """Cut-cell sphere coefficients and a zero-mean Gaussian-blob rhs on the device."""
+        lo = np.ascontiguousarray(tree["lo"], dtype=np.float64)
+        dx = np.ascontiguousarray(tree["length"][:, 0] / self.nb, dtype=np.float64)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        bc = np.ascontiguousarray(blobs["c"], dtype=np.float64)
+        bw = np.ascontiguousarray(blobs["w"], dtype=np.float64)
+        ba = np.ascontiguousarray(blobs["a"], dtype=np.float64)
+        mean = f64()
+        _chk(self.lib.sayaka_gen_sphere_blobs(self.h, N.ptr(lo), N.ptr(dx), N.ptr(c), radius, ratio, eps, nsub,
+                                              bw.size, N.ptr(bc), N.ptr(bw), N.ptr(ba), C.byref(mean)), "gen")
+        return mean.value
 
     def profile(self, kclass, enable=True):
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")

@@ -63,6 +63,15 @@ class AmrTree:
             raise MemoryError("AmrTree::regrid: block pool overflow")
         return n
 
+    def build_shell(self, spec, centre, r0, factor):
+        """Create the tree and refine around the shell |x - centre| = r0 (native loop; same
+        result as workloads.build_tree with workloads.shell_criterion)."""
+        self.create(spec)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        if self.lib.sy_tree_build_shell(self.h, N.ptr(c), float(r0), float(factor)) < 0:
+            raise MemoryError("AmrTree::regrid: block pool overflow")
+        return self.export()
+
     def export(self) -> dict:
         n = self.numblocks
         t = dict(level=np.zeros(n, np.int32), parent=np.zeros(n, np.int32),
