@@ -44,7 +44,7 @@ struct SolveStats {
 
 class MacSolver {
    public:
-    enum Precision { FP64 = 0, Mixed = 1 };
+    enum Precision { FP64 = 0, Mixed = 1, FP64Fast = 2 };
 
     explicit MacSolver(const AmrTree& tree, int device = 0);
     ~MacSolver();

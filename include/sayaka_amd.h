@@ -32,8 +32,11 @@ enum sayaka_field {
 };
 
 enum sayaka_precision {
-    SAYAKA_FP64 = 0,   /* every kernel in FP64, V-cycle exactly as the reference */
-    SAYAKA_MIXED = 1   /* FP32 V(2,2)-cycle inside the FP64 BiCGSTAB, FP64 coarsest solve */
+    SAYAKA_FP64 = 0,      /* every kernel in FP64, V-cycle and summation order exactly as the
+                             reference: bit-identical results */
+    SAYAKA_MIXED = 1,     /* FP32 V(2,2)-cycle inside the FP64 BiCGSTAB, FP64 coarsest solve */
+    SAYAKA_FP64_FAST = 2  /* SAYAKA_FP64 with order-free (tree) sums in the inner products:
+                             same algorithm, results differ from the reference by rounding */
 };
 
 typedef struct {
@@ -115,6 +118,11 @@ int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_s
 /* Coarsest-level solve of the mixed-precision V-cycle (no reference counterpart: the FP64
  * mode always uses the reference's 1e-6 relative + 1e-6 absolute, 2000 iterations). */
 int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* ctx, double tol_rel, double tol_abs, int max_iter);
+
+/* Two-level preconditioner (Jacobi + one coarse unknown per level-1 block) for the mixed
+ * solve's coarsest BiCGSTAB: 1 = on (the default; used when the level-1 problem has at
+ * most 64 blocks), 0 = plain BiCGSTAB as in the FP64 mode.  No reference counterpart. */
+int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* ctx, int two_level);
 
 /* Set every value of a field to zero on the device (the reference's setZeroSol,
  * SayakaMacSolver.h:125-128, without a host copy). */

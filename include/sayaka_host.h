@@ -68,7 +68,8 @@ int sy_mac_set_init_sol(sy_mac* m, const double* sol);
 int sy_mac_set_zero_sol(sy_mac* m);
 int sy_mac_get_sol(sy_mac* m, double* sol);
 /* setTolerance / setMaxIteration / setCGUseMGPrecond / setPre-PostSmoothNum
- * (SayakaMacSolver.h:349-358); precision 0 = FP64, 1 = mixed */
+ * (SayakaMacSolver.h:349-358); precision 0 = FP64 (reference order),
+ * 1 = mixed, 2 = FP64 with order-free sums */
 int sy_mac_set_params(sy_mac* m, double tol_rel, double tol_abs, int max_iter, int precond, int precision,
                       int pre_smooth, int post_smooth);
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680) */

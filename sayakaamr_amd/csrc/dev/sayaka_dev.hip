@@ -114,6 +114,12 @@ struct sayaka_dev_ctx {
     std::vector<hipEvent_t> ev_pool;
     int coef_version = 0;
     int exact = 1;  // reference-order reductions (FP64 solves); the mixed solve clears it
+    // two-level preconditioner of the mixed solve's coarsest problem (bottom.h)
+    int* d_l1pos = nullptr;        // block id -> level-1 list position, -1 elsewhere
+    double* d_pc_jac = nullptr;    // [PC_MAXBLK][ncell]
+    double* d_pc_fsum = nullptr;   // [PC_MAXBLK][6]
+    float* d_pc_s0inv = nullptr;   // [PC_MAXBLK][PC_MAXBLK] (row-major nl x nl in use)
+    int pc_enable = 1, pc_version = -1, pc_ok = 0;
 };
 
 namespace {
@@ -309,9 +315,11 @@ struct Impl {
         HIPCHK(hipGetLastError());
         return 0;
     }
-    static int bottom(sayaka_dev_ctx* c, double* sol, const double* rhs, double tol_rel, double tol_abs, int max_iter) {
+    static int bottom(sayaka_dev_ctx* c, double* sol, const double* rhs, double tol_rel, double tol_abs, int max_iter,
+                      bool two_level = false) {
         ListRef lst = c->tw[1];
         if (lst.n <= 0) return 0;
+        two_level = two_level && lst.n <= PC_MAXBLK;
         BottomArgs A{};
         A.st = stencil_args(c, lst, 1);
         A.rhs = rhs;
@@ -323,13 +331,98 @@ struct Impl {
         A.tol_rel = tol_rel;
         A.tol_abs = tol_abs;
         A.max_iter = max_iter;
-        A.exact = c->exact;
+        A.exact = two_level ? 0 : c->exact;
+        A.s0inv = c->d_pc_s0inv;
+        A.jac = c->d_pc_jac;
+        A.l1pos = c->d_l1pos;
         void* args[] = {&A};
         // iterative and latency-bound: no per-launch byte count
         ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
         using BC = BotCfg<D, NB>;
         const int nwg = (lst.n + BC::BPW - 1) / BC::BPW;
-        HIPCHK(hipLaunchCooperativeKernel((const void*)k_bottom<D, NB>, dim3(nwg), dim3(BC::NT), args, 0, c->stream));
+        const void* kern = two_level ? (const void*)k_bottom<D, NB, true> : (const void*)k_bottom<D, NB, false>;
+        HIPCHK(hipLaunchCooperativeKernel(kern, dim3(nwg), dim3(BC::NT), args, 0, c->stream));
+        return 0;
+    }
+    // Two-level preconditioner data for the current coefficients and BCs (bottom.h): the
+    // per-cell 1 / diag(A) and per-face weight sums on the device, then the coarse matrix
+    // S0 = P^T (D A) P of the level-1 blocks assembled and inverted here (at most 64 x 64).
+    // The pure-Neumann S0 is singular (constants); its inverse is taken of S0 + (tr S0 / n^2)
+    // 1 1^T, which acts as a pseudo-inverse up to an added constant that A annihilates.
+    static int pc_prepare(sayaka_dev_ctx* c) {
+        ListRef lst = c->tw[1];
+        c->pc_ok = 0;
+        if (!c->pc_enable || lst.n <= 0 || lst.n > PC_MAXBLK) return 0;
+        const int nl = lst.n;
+        StencilArgs a = stencil_args(c, lst, 1);
+        hipLaunchKernelGGL((k_l1_coarse<D, NB>), dim3(nl), dim3(G::THREADS), 0, c->stream, a, c->d_pc_jac,
+                           c->d_pc_fsum);
+        HIPCHK(hipGetLastError());
+        std::vector<double> fs((size_t)nl * 6);
+        HIPCHK(hipMemcpyAsync(fs.data(), c->d_pc_fsum, fs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int* L = c->h_lists.data() + lst.off;
+        std::vector<int> pos(c->nblk, -1);
+        for (int q = 0; q < nl; q++) pos[L[q]] = q;
+        std::vector<double> S((size_t)nl * nl, 0.0);
+        bool singular = true;
+        for (int q = 0; q < nl; q++) {
+            const BlockInfo& bi = c->h_binfo[L[q]];
+            for (int f = 0; f < c->nf; f++) {
+                const double w = fs[(size_t)q * 6 + f];
+                const int n = bi.nbr[f];
+                if (n >= 0 && pos[n] >= 0) {
+                    S[(size_t)q * nl + q] += w;
+                    S[(size_t)q * nl + pos[n]] -= w;
+                } else if (n <= PHYS_BNDRY) {
+                    S[(size_t)q * nl + q] += w;   // Dirichlet / SimpleFill wall (0 on Neumann walls)
+                    if (w > 0) singular = false;
+                }
+            }
+        }
+        double tr = 0.0;
+        for (int q = 0; q < nl; q++)
+            for (int r = q + 1; r < nl; r++) {
+                const double m = 0.5 * (S[(size_t)q * nl + r] + S[(size_t)r * nl + q]);
+                S[(size_t)q * nl + r] = S[(size_t)r * nl + q] = m;
+            }
+        for (int q = 0; q < nl; q++) {
+            if (S[(size_t)q * nl + q] == 0.0) S[(size_t)q * nl + q] = 1.0;   // fully solid block
+            tr += S[(size_t)q * nl + q];
+        }
+        if (singular)
+            for (auto& v : S) v += tr / ((double)nl * nl);
+        // Gauss-Jordan with partial pivoting
+        std::vector<double> I((size_t)nl * nl, 0.0);
+        for (int q = 0; q < nl; q++) I[(size_t)q * nl + q] = 1.0;
+        for (int col = 0; col < nl; col++) {
+            int piv = col;
+            for (int r = col + 1; r < nl; r++)
+                if (std::fabs(S[(size_t)r * nl + col]) > std::fabs(S[(size_t)piv * nl + col])) piv = r;
+            if (S[(size_t)piv * nl + col] == 0.0) return 0;   // no preconditioner (plain coarse solve)
+            if (piv != col)
+                for (int k = 0; k < nl; k++) {
+                    std::swap(S[(size_t)piv * nl + k], S[(size_t)col * nl + k]);
+                    std::swap(I[(size_t)piv * nl + k], I[(size_t)col * nl + k]);
+                }
+            const double d = 1.0 / S[(size_t)col * nl + col];
+            for (int k = 0; k < nl; k++) {
+                S[(size_t)col * nl + k] *= d;
+                I[(size_t)col * nl + k] *= d;
+            }
+            for (int r = 0; r < nl; r++) {
+                if (r == col) continue;
+                const double m = S[(size_t)r * nl + col];
+                if (m == 0.0) continue;
+                for (int k = 0; k < nl; k++) {
+                    S[(size_t)r * nl + k] -= m * S[(size_t)col * nl + k];
+                    I[(size_t)r * nl + k] -= m * I[(size_t)col * nl + k];
+                }
+            }
+        }
+        std::vector<float> inv32(I.begin(), I.end());
+        HIPCHK(hipMemcpy(c->d_pc_s0inv, inv32.data(), inv32.size() * sizeof(float), hipMemcpyHostToDevice));
+        c->pc_ok = 1;
         return 0;
     }
     // mg_precond_level -> mg_relax_vcycle, one cycle from a zero guess
@@ -417,6 +510,11 @@ struct Impl {
             HIPCHK(hipGetLastError());
             ms.coef_version = c->coef_version;
         }
+        if (c->pc_version != c->coef_version) {
+            int e = pc_prepare(c);
+            if (e) return e;
+            c->pc_version = c->coef_version;
+        }
         return 0;
     }
     static int smooth32(sayaka_dev_ctx* c, int m, float* u, const float* f) {
@@ -444,7 +542,8 @@ struct Impl {
             double* resid = c->d_field[SAYAKA_RESID];
             double* corr = c->d_field[SAYAKA_CORR];
             if ((e = mix(c, c->tw[1], M_F32_TO_F64, nullptr, ms.f[1], resid, nullptr))) return e;
-            if ((e = bottom(c, corr, resid, ms.bottom_tol_rel, ms.bottom_tol_abs, ms.bottom_max_iter))) return e;
+            if ((e = bottom(c, corr, resid, ms.bottom_tol_rel, ms.bottom_tol_abs, ms.bottom_max_iter, c->pc_ok != 0)))
+                return e;
             return mix(c, c->tw[1], M_F64_TO_F32, ms.u[1], nullptr, nullptr, corr);
         }
         ListRef T = c->tw[l];
@@ -560,7 +659,7 @@ struct Impl {
             int saved;
             ~ExactScope() { c->exact = saved; }
         } exact_scope{c, c->exact};
-        c->exact = mixed ? 0 : 1;
+        c->exact = P->precision == SAYAKA_FP64 ? 1 : 0;
         int precond = P->precond;
         if (precond && L == 1) precond = 0;
         HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 4 * sizeof(int), c->stream));
@@ -736,7 +835,11 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipHostMalloc(&c->h_scal, S_NSCALAR * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_vpub, cells * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_rpub, cells * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->d_bot_part, (size_t)(max_blocks + 64) * 9 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_part, (size_t)(max_blocks + 64) * 12 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_l1pos, (size_t)max_blocks * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->d_pc_jac, (size_t)PC_MAXBLK * c->ncell * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_pc_fsum, (size_t)PC_MAXBLK * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_pc_s0inv, (size_t)PC_MAXBLK * PC_MAXBLK * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 4 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 4 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream);
@@ -759,7 +862,8 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_counter, c->d_scal,
-                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists};
+                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
+                    c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < SAYAKA_KCLASS_N; k++)
         for (auto& r : c->prof[k]) {
@@ -888,6 +992,12 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
     HIPCHK(hipMemcpy(c->d_binfo, c->h_binfo.data(), nblk * sizeof(BlockInfo), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_met, c->h_met.data(), nblk * sizeof(BlockMetric), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_lists, L.data(), L.size() * sizeof(int), hipMemcpyHostToDevice));
+    {
+        std::vector<int> l1pos(nblk, -1);
+        for (int q = 0; q < c->tw[1].n; q++) l1pos[L[c->tw[1].off + q]] = q;
+        HIPCHK(hipMemcpy(c->d_l1pos, l1pos.data(), nblk * sizeof(int), hipMemcpyHostToDevice));
+        c->pc_version = -1;
+    }
     // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32)
     c->alpha = 0.0;
     c->beta = 1.0;
@@ -1061,7 +1171,8 @@ int sayaka_dev_bottom(sayaka_dev_ctx* c, int sol, int rhs, double tol_rel, doubl
 int sayaka_dev_solve(sayaka_dev_ctx* c, const sayaka_solve_params* p, sayaka_solve_stats* st) {
     if (!c || !p || !st) return fail(-1, "null argument");
     if (p->max_iter < 1 || p->tol_rel < 0 || p->tol_abs < 0) return fail(-1, "bad solver parameters");
-    if (p->precision != SAYAKA_FP64 && p->precision != SAYAKA_MIXED) return fail(-1, "bad precision");
+    if (p->precision != SAYAKA_FP64 && p->precision != SAYAKA_MIXED && p->precision != SAYAKA_FP64_FAST)
+        return fail(-1, "bad precision");
     HIPCHK(hipSetDevice(c->device));
     return dispatch(c, [&](auto I) { return decltype(I)::solve(c, p, st); });
 }
@@ -1086,6 +1197,13 @@ int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* c, double tol_rel, double tol_ab
     c->mixed.bottom_tol_rel = tol_rel;
     c->mixed.bottom_tol_abs = tol_abs;
     c->mixed.bottom_max_iter = max_iter;
+    return 0;
+}
+
+int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* c, int two_level) {
+    if (!c) return fail(-1, "null ctx");
+    c->pc_enable = two_level ? 1 : 0;
+    c->pc_version = -1;
     return 0;
 }
 

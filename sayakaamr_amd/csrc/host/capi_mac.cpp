@@ -114,13 +114,13 @@ int sy_mac_get_sol(sy_mac* h, double* sol) {
 int sy_mac_set_params(sy_mac* h, double tol_rel, double tol_abs, int max_iter, int precond, int precision,
                       int pre_smooth, int post_smooth) {
     return guarded([&] {
-        if (max_iter < 1 || tol_rel < 0 || tol_abs < 0 || (precision != 0 && precision != 1))
+        if (max_iter < 1 || tol_rel < 0 || tol_abs < 0 || (precision < 0 || precision > 2))
             throw std::invalid_argument("bad solver parameters");
         MacSolver& m = *M(h);
         m.setTolerance(tol_rel, tol_abs);
         m.setMaxIteration(max_iter);
         m.setCGUseMGPrecond(precond);
-        m.setPrecision(precision ? MacSolver::Mixed : MacSolver::FP64);
+        m.setPrecision(static_cast<MacSolver::Precision>(precision));
         m.setPreSmoothNum(pre_smooth);
         m.setPostSmoothNum(post_smooth);
         return 0;

@@ -235,7 +235,7 @@ int MacSolver::cg_solve() {
     p.tol_abs = tolAbs_;
     p.max_iter = maxIter_;
     p.precond = precond_;
-    p.precision = precision_ == Mixed ? SAYAKA_MIXED : SAYAKA_FP64;
+    p.precision = precision_ == Mixed ? SAYAKA_MIXED : precision_ == FP64Fast ? SAYAKA_FP64_FAST : SAYAKA_FP64;
     p.pre_smooth = preSmooth_;
     p.post_smooth = postSmooth_;
     p.hist_cap = 2 * maxIter_ + 2;

@@ -17,7 +17,7 @@ from .workloads import NEUMANN
 i32, f64, vp = C.c_int, C.c_double, C.c_void_p
 
 RHS, SOL, RESID, CORR, TMP, X0, X, R, RHAT, P, PHAT, S, SHAT, V, T = range(15)
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_FAST = 0, 1, 2
 
 
 class SolveParams(C.Structure):
@@ -55,6 +55,7 @@ _SIG = {
     "sayaka_dev_sync": ([vp], i32),
     "sayaka_dev_fill": ([vp, i32, f64], i32),
     "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
+    "sayaka_dev_set_mixed_coarse": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
     "sayaka_cnn_get_params": ([vp, vp], i32),
@@ -238,6 +239,10 @@ class MacSolverGPU:
 
     def set_mixed_bottom(self, tol_rel, tol_abs, max_iter):
         _chk(self.lib.sayaka_dev_set_mixed_bottom(self.h, tol_rel, tol_abs, max_iter), "set_mixed_bottom")
+
+    def set_mixed_coarse(self, two_level: bool):
+        """Two-level preconditioner of the mixed solve's coarsest BiCGSTAB on / off."""
+        _chk(self.lib.sayaka_dev_set_mixed_coarse(self.h, int(bool(two_level))), "set_mixed_coarse")
 
     # --- learned initial guess (include/sayaka_amd.h, sayaka_cnn_*) -------------------
     CNN_NP = 4 * (27 * 16 * 16 + 16)

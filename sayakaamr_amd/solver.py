@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _native as N
 
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_FAST = 0, 1, 2
 
 
 class MacSolverError(RuntimeError):

@@ -81,6 +81,53 @@ def test_c1_mixed_precision(c1):
     assert err <= MIXED_REL_L2_TOL
 
 
+def test_c1_mixed_two_level_coarse_solve(c1):
+    """The mixed solve's coarsest BiCGSTAB with the two-level preconditioner (default) and
+    without it: both reach the outer tolerance with true residuals to match, agree with the
+    FP64 solution, and the preconditioned coarse solves take fewer iterations."""
+    g, spec, tree, inp, dev = c1
+    leaf = tree["node_type"] == 1
+    dev.zero(D.SOL)
+    dev.solve(precision=D.FP64)
+    ref = gauge(dev.download(D.SOL), tree, spec)
+    out = {}
+    try:
+        for two_level in (False, True):
+            dev.set_mixed_coarse(two_level)
+            dev.zero(D.SOL)
+            rm = dev.solve(tol_rel=1e-10, tol_abs=0.0, max_iter=200, precond=1, precision=D.MIXED)
+            assert rm["status"] == 0 and rm["rnorm"] < 1e-10 * rm["rnorm0"]
+            sol = dev.download(D.SOL)
+            assert true_residual(dev, spec, tree, sol) <= 2e-10 * rm["rnorm0"]
+            err = np.linalg.norm((gauge(sol, tree, spec) - ref)[leaf]) / np.linalg.norm(ref[leaf])
+            assert err <= MIXED_REL_L2_TOL
+            out[two_level] = rm["bottom_iters"] / rm["bottom_calls"]
+            print(f"[gpu] mixed two-level={two_level}: iters {rm['iters']}, coarse its/call {out[two_level]:.1f}, "
+                  f"rel L2 vs FP64 {err:.2e}", flush=True)
+    finally:
+        dev.set_mixed_coarse(True)
+    assert out[True] < 0.6 * out[False]
+
+
+def test_c1_fp64_fast_within_rounding(c1):
+    """Order-free sums: same algorithm, so the reference's tolerance is reached with the
+    iteration count within one; the solution agrees to the accuracy a 1e-10 relative
+    residual fixes for this operator (the reference-order mode is the bit-exact one)."""
+    g, spec, tree, inp, dev = c1
+    leaf = tree["node_type"] == 1
+    dev.zero(D.SOL)
+    dev.solve(precision=D.FP64)
+    ref = gauge(dev.download(D.SOL), tree, spec)
+    dev.zero(D.SOL)
+    rf = dev.solve(tol_rel=1e-10, tol_abs=0.0, max_iter=2000, precond=1, precision=D.FP64_FAST)
+    assert rf["status"] == 0 and abs(rf["iters"] - int(g["solve_iters"][0])) <= 1
+    sol = dev.download(D.SOL)
+    assert true_residual(dev, spec, tree, sol) <= 2e-10 * rf["rnorm0"]
+    err = np.linalg.norm((gauge(sol, tree, spec) - ref)[leaf]) / np.linalg.norm(ref[leaf])
+    print(f"[gpu] fp64-fast iters {rf['iters']} (reference {int(g['solve_iters'][0])}), rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-6
+
+
 def test_profile_counters(c1):
     g, spec, tree, inp, dev = c1
     for k in range(len(D.KCLASS_NAMES)):
