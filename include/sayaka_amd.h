@@ -124,6 +124,11 @@ int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* ctx, double tol_rel, double tol_
  * most 64 blocks), 0 = plain BiCGSTAB as in the FP64 mode.  No reference counterpart. */
 int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* ctx, int two_level);
 
+/* Smoother of the mixed solve's V-cycle: 1 = each colour in one launch over every
+ * sub-level of the tower, double-buffered (the default), 0 = one launch per sub-level and
+ * colour in place, in the reference's sub-level order.  No reference counterpart. */
+int sayaka_dev_set_mixed_smoother(sayaka_dev_ctx* ctx, int merged);
+
 /* Set every value of a field to zero on the device (the reference's setZeroSol,
  * SayakaMacSolver.h:125-128, without a host copy). */
 int sayaka_dev_fill(sayaka_dev_ctx* ctx, int field, double value);

@@ -31,6 +31,8 @@ struct MixedState {
     float* wcf32 = nullptr;
     float* vf32 = nullptr;
     float* r32 = nullptr;
+    float* t32 = nullptr;       // second buffer of the double-buffered smoother
+    int merged_smooth = 1;      // one launch per colour over all sub-levels of a tower
     float* u[MIX_MAXLV + 1] = {};
     float* f[MIX_MAXLV + 1] = {};
     // The reference's bottom test (1e-6 relative + 1e-6 absolute) stops the coarse solve
@@ -42,14 +44,14 @@ struct MixedState {
 };
 
 inline void mixed_free(MixedState& ms) {
-    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32};
+    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32, ms.t32};
     for (float* p : ptrs) if (p) (void)hipFree(p);
     for (int l = 0; l <= MIX_MAXLV; l++) {
         if (ms.u[l]) (void)hipFree(ms.u[l]);
         if (ms.f[l]) (void)hipFree(ms.f[l]);
         ms.u[l] = ms.f[l] = nullptr;
     }
-    ms.w32 = ms.wcf32 = ms.vf32 = ms.r32 = nullptr;
+    ms.w32 = ms.wcf32 = ms.vf32 = ms.r32 = ms.t32 = nullptr;
     ms.cap_cells = ms.cap_faces = ms.cap_cf = 0;
     ms.coef_version = -1;
 }

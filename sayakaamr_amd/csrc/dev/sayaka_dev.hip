@@ -73,7 +73,6 @@ struct sayaka_dev_ctx {
     double* d_wcf = nullptr;
 
     double* d_partials = nullptr;
-    unsigned* d_counter = nullptr;
     double* d_scal = nullptr;
     double* h_scal = nullptr;  // pinned
     double* d_bot_vpub = nullptr;
@@ -199,7 +198,7 @@ struct AlgBytes {
     static constexpr double prolong() { return 2 * C + C / (1 << D); }
 };
 
-RedBuf redbuf(sayaka_dev_ctx* c) { return RedBuf{c->d_partials, c->d_counter, c->d_scal, c->exact}; }
+RedBuf redbuf(sayaka_dev_ctx* c) { return RedBuf{c->d_partials, c->d_scal, c->exact}; }
 
 const int* dlist(sayaka_dev_ctx* c, ListRef r) { return c->d_lists + r.off; }
 
@@ -234,6 +233,9 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
         case ST_RESID_INIT: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID_INIT, double>), grid, block, 0, c->stream, a, rb, fin); break;
         default: return fail(-2, "bad stencil mode");
     }
+    if (mode == ST_APPLY_DOT || mode == ST_APPLY_DOT2 || mode == ST_RESID_INIT)
+        hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, rb, a.nlist,
+                           mode == ST_APPLY_DOT2 ? 2 : 1, fin);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -246,6 +248,9 @@ int launch_vec(sayaka_dev_ctx* c, ListRef lst, int op, const VecArgs& a, int fin
     ProfScope ps(c, SAYAKA_KCLASS_VEC, (long long)lst.n * G::NCELL, lst.n * AlgBytes<D, NB, double>::vec(op, extra));
     hipLaunchKernelGGL((k_vec<G::NCELL, G::THREADS>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, a, op,
                        redbuf(c), fin);
+    if (fin != FIN_NONE)
+        hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, redbuf(c), lst.n,
+                           (op == V_RUPDATE || op == V_DOT) ? 1 : 0, fin);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -491,6 +496,7 @@ struct Impl {
             HIPCHK(hipMalloc(&ms.wcf32, cf * sizeof(float)));
             HIPCHK(hipMalloc(&ms.vf32, cells * sizeof(float)));
             HIPCHK(hipMalloc(&ms.r32, cells * sizeof(float)));
+            HIPCHK(hipMalloc(&ms.t32, cells * sizeof(float)));
             for (int l = 1; l <= c->maxlevel; l++) {
                 HIPCHK(hipMalloc(&ms.u[l], cells * sizeof(float)));
                 HIPCHK(hipMalloc(&ms.f[l], cells * sizeof(float)));
@@ -518,6 +524,24 @@ struct Impl {
         return 0;
     }
     static int smooth32(sayaka_dev_ctx* c, int m, float* u, const float* f) {
+        if (c->mixed.merged_smooth) {
+            // both colours over the whole tower, u -> t32 -> u
+            ListRef T = c->tw[m];
+            if (T.n <= 0) return 0;
+            for (int rb = 0; rb < 2; rb++) {
+                StencilArgsT<float> a = args32(c, T, m);
+                a.x = rb == 0 ? u : c->mixed.t32;
+                a.out = rb == 0 ? c->mixed.t32 : u;
+                a.rhs = f;
+                a.colour = rb;
+                ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL,
+                             T.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH));
+                hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0, c->stream,
+                                   a, RedBuf{}, FIN_NONE);
+            }
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
         for (int l = m; l >= 1; l--) {
             ListRef sub = c->tw_sub[m][l];
             if (sub.n <= 0) break;
@@ -830,7 +854,6 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_w, faces * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_wcf, (size_t)max_blocks * c->nf * c->nslab * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_partials, (size_t)(max_blocks + 64) * 3 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->d_counter, sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&c->d_scal, S_NSCALAR * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_scal, S_NSCALAR * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_vpub, cells * sizeof(double));
@@ -842,7 +865,6 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_s0inv, (size_t)PC_MAXBLK * PC_MAXBLK * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 4 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 4 * sizeof(int), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_scal, 0, S_NSCALAR * sizeof(double), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_wcf, 0, (size_t)max_blocks * c->nf * c->nslab * sizeof(double), c->stream);
     for (int f = 0; f < SAYAKA_NFIELD && e == hipSuccess; f++) e = hipMemsetAsync(c->d_field[f], 0, cells * sizeof(double), c->stream);
@@ -861,7 +883,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
-    void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_counter, c->d_scal,
+    void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
     for (void* p : ptrs) if (p) hipFree(p);
@@ -1204,6 +1226,12 @@ int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* c, int two_level) {
     if (!c) return fail(-1, "null ctx");
     c->pc_enable = two_level ? 1 : 0;
     c->pc_version = -1;
+    return 0;
+}
+
+int sayaka_dev_set_mixed_smoother(sayaka_dev_ctx* c, int merged) {
+    if (!c) return fail(-1, "null ctx");
+    c->mixed.merged_smooth = merged ? 1 : 0;
     return 0;
 }
 

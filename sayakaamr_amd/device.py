@@ -56,6 +56,7 @@ _SIG = {
     "sayaka_dev_fill": ([vp, i32, f64], i32),
     "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
     "sayaka_dev_set_mixed_coarse": ([vp, i32], i32),
+    "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
     "sayaka_cnn_get_params": ([vp, vp], i32),
@@ -243,6 +244,11 @@ class MacSolverGPU:
     def set_mixed_coarse(self, two_level: bool):
         """Two-level preconditioner of the mixed solve's coarsest BiCGSTAB on / off."""
         _chk(self.lib.sayaka_dev_set_mixed_coarse(self.h, int(bool(two_level))), "set_mixed_coarse")
+
+    def set_mixed_smoother(self, merged: bool):
+        """Mixed V-cycle smoother: one launch per colour over the whole tower (double-buffered)
+        or one launch per sub-level and colour in place."""
+        _chk(self.lib.sayaka_dev_set_mixed_smoother(self.h, int(bool(merged))), "set_mixed_smoother")
 
     # --- learned initial guess (include/sayaka_amd.h, sayaka_cnn_*) -------------------
     CNN_NP = 4 * (27 * 16 * 16 + 16)
