@@ -67,6 +67,11 @@ C0_SPEC = TreeSpec(ndim=2, nb=8, minlevel=2, maxlevel=3, ndiv=(4, 4, 1),
                    lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 0.0))
 C1_SPEC = TreeSpec(ndim=3, nb=8, minlevel=2, maxlevel=4, ndiv=(4, 4, 4),
                    lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0))
+# Large snapshot: 16^3-cell blocks, root 4^3, tree levels 1-6 refined around an r = 0.15
+# sphere with the C1 shell rule (38 848 blocks, 34 000 leaves, 139 M leaf cells).
+C4_SPEC = TreeSpec(ndim=3, nb=16, minlevel=2, maxlevel=6, ndiv=(4, 4, 4),
+                   lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0))
+C4_RADIUS = 0.15
 
 
 def small_spec(ndim: int, maxlevel: int = 3, nb: int = 8) -> TreeSpec:
