@@ -3,7 +3,8 @@
 //
 // V(l, f_l) approximately solves A_l u_l = f_l on tower T(l) = leaves of levels < l plus
 // every block of level l (the reference's MG level, SayakaTower.h:121-129):
-//   l == 1 : FP64 BiCGSTAB on the level-1 blocks (the reference's bottom solve);
+//   l == 1 : FP64 BiCGSTAB on the level-1 blocks to a 1e-2 relative residual, with the
+//            two-level (Jacobi + one coarse unknown per block) preconditioner of bottom.h;
 //   else   : u_l = 0; nu1 red-black sweeps; r = f_l - A_l u_l;
 //            f_{l-1} = r on the leaves shared with T(l-1), and the volume-fraction weighted
 //            average of r from the children on the non-leaf blocks of level l-1;
@@ -11,7 +12,9 @@
 //            u_l += u_{l-1} on the shared leaves, += injection of u_{l-1} (zero where vf=0)
 //            on the blocks of level l; nu2 red-black sweeps.
 // Sweeps, residual, transfer and accumulation are FP32; the operator weights are the FP64
-// weights rounded once to FP32.  Each level keeps its own u and f (the towers share their
+// weights rounded once to FP32.  A sweep is two launches (one per colour) over the whole
+// tower, double-buffered (u -> t32 -> u), so blocks of every sub-level update together and
+// a coarse/fine neighbour is always read as it was before the launch.  Each level keeps its own u and f (the towers share their
 // coarse leaves, so one in-place array as in the reference cannot hold both corrections).
 #pragma once
 
