@@ -101,6 +101,14 @@ class MacSolver {
 
     // MacSolver::cg_solve (SayakaMacSolver.cpp:455-680)
     int cg_solve();
+
+    // MacSolver::mg_flux_level (SayakaMacSolver.h:275, SayakaMacOperator.cpp:161-239): the face
+    // flux -d(phi)/dx of direction dir on tower mg_level into flux (face data of dir): in-block
+    // and same-level faces over dx, the fine side of coarse/fine faces over 1.5 dx, the coarse
+    // side the average of the fine fluxes, walls 0.  The device context's X field carries phi;
+    // unlike the reference, phi's ghost cells are not filled as a side effect.
+    void mg_flux_level(int mg_level, int dir, TreeData& flux, TreeData& phi, int fluxcomp, int phicomp);
+    void fluxPacked(int mg_level, int dir, const double* phi, double* flux);
     const SolveStats& lastStats() const { return stats_; }
 
     const AmrTree& tree() const { return *tree_; }

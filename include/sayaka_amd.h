@@ -111,6 +111,11 @@ int sayaka_dev_precond(sayaka_dev_ctx* ctx, int m, int sol, int rhs);           
 int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, double tol_abs, int max_iter,
                       int* iters, int* status);      /* mg_solve_bottom */
 
+/* MacSolver::mg_flux_level (SayakaMacOperator.cpp:161-239): the face flux -d(phi)/dx of
+ * direction dir on tower m from the field phi, into flux_out [block][face] (host memory,
+ * nblk * (nb+1)*nb^(ndim-1) doubles; blocks outside tower m are 0). */
+int sayaka_dev_flux(sayaka_dev_ctx* ctx, int m, int dir, int phi, double* flux_out);
+
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
  * the SOL field as initial guess; SOL receives the solution on convergence. */
 int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);

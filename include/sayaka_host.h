@@ -67,6 +67,9 @@ int sy_mac_set_rhs(sy_mac* m, const double* rhs);
 int sy_mac_set_init_sol(sy_mac* m, const double* sol);
 int sy_mac_set_zero_sol(sy_mac* m);
 int sy_mac_get_sol(sy_mac* m, double* sol);
+/* mg_flux_level (SayakaMacSolver.h:275, SayakaMacOperator.cpp:161-239): face flux of
+ * direction dir on tower mg_level from packed phi into packed flux [blk][face] */
+int sy_mac_flux(sy_mac* m, int mg_level, int dir, const double* phi, double* flux);
 /* setTolerance / setMaxIteration / setCGUseMGPrecond / setPre-PostSmoothNum
  * (SayakaMacSolver.h:349-358); precision 0 = FP64 (reference order),
  * 1 = mixed, 2 = FP64 with order-free sums */

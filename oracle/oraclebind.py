@@ -62,6 +62,7 @@ def load():
         "oc_buf_get": ([_p, _i, _p], None),
         "oc_save_get": ([_p, _i, _p], None),
         "oc_fill": ([_p, _i, _i], None),
+        "oc_flux": ([_p, _i, _i, _p, _p], None),
         "oc_apply": ([_p, _i, _i, _i], None),
         "oc_residual": ([_p, _i, _i, _i, _i], None),
         "oc_smooth": ([_p, _i, _i, _i], None),
@@ -190,6 +191,12 @@ class Oracle:
     def save_get(self, d):
         out = np.zeros((self.n, self.nface[d]))
         self.lib.oc_save_get(self.h, d, _ptr(out))
+        return out
+
+    def flux(self, m, d, phi):
+        """MacSolver::mg_flux_level: face flux of direction d on tower m, [blk][face]."""
+        out = np.zeros((self.n, self.nface[d]))
+        self.lib.oc_flux(self.h, m, d, _ptr(np.ascontiguousarray(phi, dtype=np.float64)), _ptr(out))
         return out
 
     def fill(self, m, comp):

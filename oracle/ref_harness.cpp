@@ -417,6 +417,27 @@ void ref_smooth(int m, int solcomp, int rhscomp) {
     g_mac->mg_smooth_level(m, d, d, solcomp, rhscomp);
 }
 void ref_fill(int m, int comp) { g_mac->mg_fillbndry_level(m, g_mac->getDataBuf(), comp, 1, 1); }
+
+// MacSolver::mg_flux_level (SayakaMacOperator.cpp:161-239): phi (valid values, ghosts zero
+// before the reference fills them) in, the face flux of direction dir out for every block
+// (the flux array starts at zero).
+void ref_mac_flux(int m, int dir, const double* phi_valid, double* flux_faces) {
+    const int n = g_tree->numBlocks;
+    TreeData* phi = TreeData::CreateCellData(*g_tree, 1, 1);
+    std::vector<TreeData*> fl = TreeData::CreateFaceDataPArray(*g_tree, 1, 0);
+    const IndexBox gb = grown_cell_box(1);
+    std::vector<double> zero((size_t)n * gb.stride(), 0.0);
+    copy_in_box(*phi, 0, gb, zero.data(), n);
+    copy_in_box(*phi, 0, valid_box(), phi_valid, n);
+    std::vector<double> fz((size_t)n * face_box(dir).stride(), 0.0);
+    copy_in_box(*fl[dir], 0, face_box(dir), fz.data(), n);
+    quiet_begin();
+    g_mac->mg_flux_level(m, dir, *fl[dir], *phi, 0, 0);
+    quiet_end();
+    copy_out_box(*fl[dir], 0, face_box(dir), flux_faces, n);
+    delete phi;
+    TreeData::ReleaseDataPArray(fl);
+}
 void ref_restrict(int c, int comp) { g_mac->mg_restict_wgt(c, g_mac->getDataBuf(), comp); }
 void ref_prolong(int l, int comp) { g_mac->mg_prolong_wgt(l, g_mac->getDataBuf(), comp); }
 double ref_dot(int m, int a, int b) {

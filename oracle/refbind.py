@@ -68,6 +68,7 @@ def load(ndim: int):
         "ref_residual": ([_i, _i, _i, _i], None),
         "ref_smooth": ([_i, _i, _i], None),
         "ref_fill": ([_i, _i], None),
+        "ref_mac_flux": ([_i, _i, _p, _p], None),
         "ref_restrict": ([_i, _i], None),
         "ref_prolong": ([_i, _i], None),
         "ref_dot": ([_i, _i, _i], _d),
@@ -227,6 +228,14 @@ class RefMac:
     def buf_get(self, comp, n, ngrown):
         out = np.zeros((n, ngrown))
         self.lib.ref_buf_get(comp, _ptr(out))
+        return out
+
+    def flux(self, m, d, phi, nb):
+        """MacSolver::mg_flux_level on tower m, direction d: [blk][face]."""
+        n = phi.shape[0]
+        nd = self.t.ndim
+        out = np.zeros((n, (nb + 1) * nb ** (nd - 1)))
+        self.lib.ref_mac_flux(m, d, _ptr(np.ascontiguousarray(phi, dtype=np.float64)), _ptr(out))
         return out
 
     def save_get(self, d, n, nface):

@@ -649,6 +649,64 @@ void oc_fill(oc_ctx* c, int m, int comp) {
     }
 }
 
+static void valid_in(oc_ctx* c, int comp, const double* v);
+
+/* ------------------------------------------------------------------------ face flux */
+/* MacSolver::mg_flux_level, ghost-fill path (SayakaMacOperator.cpp:161-203), with
+ * mg_flux_block_usefill (SayakaMacOperator.cpp:929-1025): flux = -(phi(i) - phi(i-1)) / dh on
+ * the faces of direction d of every block of tower m, sub-level by sub-level from m down to 1;
+ * wall faces 0; the fine side of a coarse/fine face divides by 1.5 dh and reads the injected
+ * coarse ghost; the coarse side is left to syncSubLevelCrseFineFlux(FLUX_SYNC_AVG) after each
+ * finer sub-level.  phi: valid values [blk][cell]; out: [blk][face of d], zeroed first (the
+ * reference zeroes the level, SayakaMacOperator.cpp:176, and the sync also writes the
+ * restricted fine fluxes into the non-leaf parents' faces).  The oracle's OC_TMP component
+ * carries phi with its ghosts and is restored afterwards. */
+void oc_flux(oc_ctx* c, int m, int d, const double* phi, double* out) {
+    const tower_t* t = &c->tw[m];
+    const size_t gsz = (size_t)c->nblk * c->ngrown;
+    double* saved = (double*)malloc(gsz * sizeof(double));
+    memcpy(saved, G(c, OC_TMP, 0), gsz * sizeof(double));
+    memset(G(c, OC_TMP, 0), 0, gsz * sizeof(double));
+    valid_in(c, OC_TMP, phi);
+    const int nf = c->nface[d];
+    memset(out, 0, (size_t)c->nblk * nf * sizeof(double));
+    if (c->lvl_start[m + 1] - c->lvl_start[m] > 0) {
+        oc_fill(c, m, OC_TMP);
+        const int ii = d == 0, jj = d == 1, kk = d == 2;
+        const int ext = d == 2 ? c->nbz : c->nb;
+        for (int l = m; l >= 1; l--) {
+            for (int q = t->begin[l]; q < t->end[l]; q++) {
+                const int b = t->blocks[q];
+                const double dh = c->dx[b * 3 + d];
+                const double dh_fine_crse = dh * 1.5;
+                const double* ph = G(c, OC_TMP, b);
+                double* fo = out + (size_t)b * nf;
+                for (int k = 0; k < c->fd[d][2]; k++)
+                    for (int j = 0; j < c->fd[d][1]; j++)
+                        for (int i = 0; i < c->fd[d][0]; i++) {
+                            const int ijk = d == 0 ? i : (d == 1 ? j : k);
+                            const double dphi = ph[gidx(c, i, j, k)] - ph[gidx(c, i - ii, j - jj, k - kk)];
+                            const int side = ijk == 0 ? 0 : (ijk == ext ? 1 : -1);
+                            double v;
+                            if (side < 0) {
+                                v = -dphi / dh;
+                            } else {
+                                const int ft = t->face_type[b * 6 + 2 * d + side];
+                                if (ft == FBC) v = 0;
+                                else if (ft == FF) v = -dphi / dh;
+                                else if (ft == FC) v = -dphi / dh_fine_crse;
+                                else continue; /* CF: filled by the sync below */
+                            }
+                            fo[fidx(c, d, i, j, k)] = v;
+                        }
+            }
+            if (l > 1) sync_cf(c, m, l, l - 1, d, out, 1);
+        }
+    }
+    memcpy(G(c, OC_TMP, 0), saved, gsz * sizeof(double));
+    free(saved);
+}
+
 /* ------------------------------------------------------------------- operator, smoother */
 /* cell_eval_coef + IB/B scaling (SayakaMacOperator.cpp:1115-1167, 1256-1266, 1395-1405) */
 static void eval_coef(const oc_ctx* c, const tower_t* t, int b, int i, int j, int k, double off[6],

@@ -58,6 +58,9 @@ void oc_save_get(const oc_ctx* c, int dir, double* faces);
 
 /* operations on the buffer */
 void oc_fill(oc_ctx* c, int m, int comp);
+/* MacSolver::mg_flux_level (SayakaMacOperator.cpp:161-203, 929-1025): flux of direction d on
+ * tower m from valid phi [blk][cell] into out [blk][face] */
+void oc_flux(oc_ctx* c, int m, int d, const double* phi, double* out);
 void oc_apply(oc_ctx* c, int m, int outcomp, int incomp);
 void oc_residual(oc_ctx* c, int m, int rcomp, int xcomp, int rhscomp);
 void oc_smooth(oc_ctx* c, int m, int solcomp, int rhscomp);

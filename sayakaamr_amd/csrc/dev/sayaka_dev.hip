@@ -20,6 +20,7 @@
 #include "cnn_data.h"
 #include "kernels.h"
 #include "mixed.h"
+#include "flux.h"
 
 using namespace sydev;
 
@@ -119,6 +120,7 @@ struct sayaka_dev_ctx {
     double* d_pc_fsum = nullptr;   // [PC_MAXBLK][6]
     float* d_pc_s0inv = nullptr;   // [PC_MAXBLK][PC_MAXBLK] (row-major nl x nl in use)
     int pc_enable = 1, pc_version = -1, pc_ok = 0;
+    double* d_flux = nullptr;      // face flux of one direction, [cap][nface] (allocated on first use)
 };
 
 namespace {
@@ -656,6 +658,20 @@ struct Impl {
         return 0;
     }
 
+    // MacSolver::mg_flux_level (SayakaMacOperator.cpp:161-239) into d_flux (flux.h)
+    static int flux(sayaka_dev_ctx* c, int m, int dir, const double* x) {
+        if (!c->d_flux) HIPCHK(hipMalloc(&c->d_flux, (size_t)c->cap * c->nface * sizeof(double)));
+        HIPCHK(hipMemsetAsync(c->d_flux, 0, (size_t)c->nblk * c->nface * sizeof(double), c->stream));
+        ListRef T = c->tw[m];
+        if (T.n <= 0) return 0;
+        FluxArgs a{dlist(c, T), T.n, m, c->d_binfo, c->d_met, x, dir, c->d_flux};
+        ProfScope ps(c, SAYAKA_KCLASS_APPLY, (long long)T.n * G::NCELL, T.n * (AlgBytes<D, NB, double>::C +
+                                                                            (double)G::NFACE * sizeof(double)));
+        hipLaunchKernelGGL((k_flux<D, NB>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     static int dot(sayaka_dev_ctx* c, int m, const double* x, const double* y, double* dotv, double* mx) {
         VecArgs a = vec_args(c, c->tw[m]);
         a.src = x;
@@ -883,6 +899,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
+    if (c->d_flux) hipFree(c->d_flux);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
@@ -1157,6 +1174,18 @@ int sayaka_dev_prolong(sayaka_dev_ctx* c, int fl, int field) {
     if (!c || !field_ok(field)) return fail(-1, "bad argument");
     int e = dispatch(c, [&](auto I) { return decltype(I)::prolong(c, fl, c->d_field[field]); });
     if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_flux(sayaka_dev_ctx* c, int m, int dir, int phi, double* flux_out) {
+    if (!c || !field_ok(phi) || m < 1 || m > c->maxlevel || dir < 0 || dir >= c->ndim || !flux_out)
+        return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    int e = dispatch(c, [&](auto I) { return decltype(I)::flux(c, m, dir, c->d_field[phi]); });
+    if (e) return e;
+    HIPCHK(hipMemcpyAsync(flux_out, c->d_flux, (size_t)c->nblk * c->nface * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

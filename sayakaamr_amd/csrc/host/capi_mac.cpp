@@ -111,6 +111,14 @@ int sy_mac_get_sol(sy_mac* h, double* sol) {
     });
 }
 
+int sy_mac_flux(sy_mac* h, int mg_level, int dir, const double* phi, double* flux) {
+    return guarded([&] {
+        if (!phi || !flux) throw std::invalid_argument("sy_mac_flux: null array");
+        M(h)->fluxPacked(mg_level, dir, phi, flux);
+        return 0;
+    });
+}
+
 int sy_mac_set_params(sy_mac* h, double tol_rel, double tol_abs, int max_iter, int precond, int precision,
                       int pre_smooth, int post_smooth) {
     return guarded([&] {

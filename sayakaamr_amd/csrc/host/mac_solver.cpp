@@ -223,6 +223,19 @@ void MacSolver::setInitSol(const TreeData& sol, int comp) {
 
 void MacSolver::setZeroSol() { check(sayaka_dev_fill(ctx_, SAYAKA_SOL, 0.0), "MacSolver::setZeroSol"); }
 
+void MacSolver::fluxPacked(int mg_level, int dir, const double* phi, double* flux) {
+    check(sayaka_dev_upload(ctx_, SAYAKA_X, phi), "MacSolver::mg_flux_level");
+    check(sayaka_dev_flux(ctx_, mg_level, dir, SAYAKA_X, flux), "MacSolver::mg_flux_level");
+}
+
+void MacSolver::mg_flux_level(int mg_level, int dir, TreeData& flux, TreeData& phi, int fluxcomp, int phicomp) {
+    if (flux.faceDir() != dir) throw std::invalid_argument("mg_flux_level: flux must be face data of direction dir");
+    std::vector<double> p((std::size_t)nblk_ * ncell_), f((std::size_t)nblk_ * nface_);
+    phi.packValid(phicomp, nblk_, p.data());
+    fluxPacked(mg_level, dir, p.data(), f.data());
+    flux.unpackValid(fluxcomp, nblk_, f.data());
+}
+
 void MacSolver::getSol(TreeData& sol, int comp) const {
     std::vector<double> r((std::size_t)nblk_ * ncell_);
     getSolPacked(r.data());

@@ -56,6 +56,7 @@ _SIG = {
     "sayaka_dev_fill": ([vp, i32, f64], i32),
     "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
     "sayaka_dev_set_mixed_coarse": ([vp, i32], i32),
+    "sayaka_dev_flux": ([vp, i32, i32, i32, vp], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -202,6 +203,13 @@ class MacSolverGPU:
 
     def prolong(self, fl, field):
         _chk(self.lib.sayaka_dev_prolong(self.h, fl, field), "prolong")
+
+    def flux(self, m, d, phi):
+        """MacSolver::mg_flux_level: face flux of direction d on tower m from field phi,
+        [blk][face] (0 outside the tower)."""
+        out = np.zeros((self.n, self.nface))
+        _chk(self.lib.sayaka_dev_flux(self.h, m, d, phi, N.ptr(out)), "flux")
+        return out
 
     def dot(self, m, a, b):
         d = f64()

@@ -101,6 +101,14 @@ class MacSolver:
         self._chk(self.lib.sy_mac_get_sol(self.h, N.ptr(out)), "getSol")
         return out
 
+    def mg_flux_level(self, mg_level, d, phi):
+        """MacSolver::mg_flux_level: face flux of direction d on tower mg_level from phi
+        [blk][cell]; returns [blk][face] (0 outside the tower)."""
+        out = np.zeros((self.numBlocks, (self.nb + 1) * self.nb ** (self.ndim - 1)))
+        p = np.ascontiguousarray(phi, dtype=np.float64)
+        self._chk(self.lib.sy_mac_flux(self.h, mg_level, d, N.ptr(p), N.ptr(out)), "mg_flux_level")
+        return out
+
     def setTolerance(self, rel, abs_):
         self._params.update(tol_rel=rel, tol_abs=abs_)
 

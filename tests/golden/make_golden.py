@@ -5,7 +5,8 @@ sources by oracle/build_ref.sh) in this container; the .npz files are data only 
 and the reference's outputs) and are what tests/ check the oracle and the product tree
 against on machines where the reference is absent.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py         (operators, solves, regrid, C1 summary)
+    python tests/golden/make_golden.py flux    (face fluxes: tests/golden/{c0,s3}_flux.npz)
 """
 from __future__ import annotations
 
@@ -199,7 +200,45 @@ def regrid_sequence():
     print("regrid_sequence: 6 steps")
 
 
+def case_flux(name, spec, crit, inp_fn, bc, seed=7):
+    """MacSolver::mg_flux_level on the tower of every MG level and every direction, for the
+    seeded input of case_ops (the same tree, coefficients and op_x) and for the case's own
+    solution."""
+    rt = R.RefTree(spec.ndim)
+    tree = W.build_tree(rt, spec, crit)
+    n = tree["level"].shape[0]
+    nb, nd = spec.nb, spec.ndim
+    inp = inp_fn(tree, spec)
+    mac = R.RefMac(rt)
+    mac.prepare()
+    if inp.get("vf") is not None:
+        mac.set_ib_frac(inp["vf"], inp["af"])
+    if inp.get("b") is not None:
+        mac.set_coefb(inp["b"])
+    mac.set_bc([W.WALL], [bc], [0.0])
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, nb ** nd))
+    out = {"op_x": x}
+    for m in range(1, spec.maxlevel + 1):
+        for d in range(nd):
+            out[f"flux_m{m}_d{d}"] = mac.flux(m, d, x, nb)
+    mac.set_rhs(inp["rhs"])
+    mac.set_zero_sol()
+    mac.set_params(1e-10, 0.0, 2000, 1)
+    mac.cg_solve()
+    sol = mac.get_sol(n, nb ** nd)
+    out["sol"] = sol
+    for d in range(nd):
+        out[f"flux_sol_d{d}"] = mac.flux(spec.maxlevel, d, sol, nb)
+    np.savez_compressed(os.path.join(HERE, f"{name}_flux.npz"), **out)
+    print(f"{name}_flux: {n} blocks, wrote {len(out)} arrays")
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "flux":
+        case_flux("c0", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
+        case_flux("s3", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs, W.NEUMANN)
+        sys.exit(0)
     case_ops("c0_ops", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
     case_ops("s3_ops", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs, W.NEUMANN)
     regrid_sequence()
