@@ -104,7 +104,8 @@ def test_projection_through_mg_flux_level():
     unit coefficients the operator is exactly the divergence of that flux (in-block and
     same-level faces over dx, fine side of coarse/fine faces over 1.5 dx, coarse side the
     average of the fine fluxes), so the corrected velocity is divergence-free on every leaf cell
-    of the 3D octree to the solver's residual.  A flux with a wrong sign, divisor or coarse/fine
+    of the 3D octree to the solver's residual (1e-10 relative; the reference's algorithm stalls
+    before 1e-12 on this tree).  A flux with a wrong sign, divisor or coarse/fine
     average leaves an O(1) divergence at the faces where it is wrong."""
     spec = W.C1_SPEC
     tree = AmrTree(3, max_blocks=6000)
@@ -121,7 +122,7 @@ def test_projection_through_mg_flux_level():
         mac.setBC([W.WALL], [W.NEUMANN])
         mac.setRhs(rhs)
         mac.setZeroSol()
-        mac.setTolerance(1e-12, 0.0)
+        mac.setTolerance(1e-10, 0.0)
         assert mac.cg_solve() == 0 and mac.stats["status"] == 0
         p = mac.getSol()
         flux = [mac.mg_flux_level(L, d, p) for d in range(3)]
@@ -131,4 +132,5 @@ def test_projection_through_mg_flux_level():
     div0 = np.abs(rhs[leaf]).max()
     div1 = W.divergence_rhs(t, spec, [u[d] - flux[d] for d in range(3)], one_c, one_f)
     print(f"[gpu] projection: |div u| {div0:.3e} -> {np.abs(div1[leaf]).max():.3e}", flush=True)
-    assert np.abs(div1[leaf]).max() <= 1e-9 * div0
+    # the residual of a 1e-10 solve, plus rounding of the divergence sums
+    assert np.abs(div1[leaf]).max() <= 1e-8 * div0
