@@ -68,6 +68,10 @@ struct AmrTree {
 
     std::vector<AmrTreeNode> blocks;
     std::vector<int> refineFlag, coarsenFlag, newChildFlag, stayFlag;
+    // after regrid(): for every block, the id it had before the regrid, -1 for a block the
+    // regrid created (the reference moves the registered state data along the same way,
+    // SayakaRegrid.cpp:530-591, and fills the new blocks in initNewBlockDataAfterRegrid)
+    std::vector<int> regridOrigin;
     std::vector<SurroundingBlocks> surrBlocks;
     std::vector<std::vector<int>> levels;  // levels[l] = ids of level-l blocks, ascending
 

@@ -116,6 +116,14 @@ int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, dou
  * nblk * (nb+1)*nb^(ndim-1) doubles; blocks outside tower m are 0). */
 int sayaka_dev_flux(sayaka_dev_ctx* ctx, int m, int dir, int phi, double* flux_out);
 
+/* Regrid data movement of the device fields, after sayaka_dev_set_topology with the regridded
+ * tree: block b takes the values block origin[b] had before (origin from sy_tree_regrid_origin;
+ * nold = block count before the regrid), and blocks with origin -1 are filled from their
+ * parents by injection, coarsest level first -- the reference's packing of registered state
+ * data and AmrTree::initNewBlockDataAfterRegrid with the default injection
+ * (SayakaRegrid.cpp:530-591, 987-1105). */
+int sayaka_dev_remap(sayaka_dev_ctx* ctx, const int* origin, int nold, const int* fields, int nfields);
+
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
  * the SOL field as initial guess; SOL receives the solution on convergence. */
 int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);

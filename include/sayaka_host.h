@@ -32,6 +32,10 @@ int sy_tree_finest(const sy_tree* t);
 int sy_tree_set_flags(sy_tree* t, const int* refine, const int* coarsen, int n);
 /* AmrTree::regrid (SayakaRegrid.cpp:26-125).  Returns numBlocks, -1 on pool overflow. */
 int sy_tree_regrid(sy_tree* t);
+/* after sy_tree_regrid: per block its id before the regrid, -1 for a block the regrid created
+ * (the data movement of SayakaRegrid.cpp:530-591; no reference accessor, the reference moves its
+ * registered state data itself) */
+void sy_tree_regrid_origin(const sy_tree* t, int* origin);
 /* Flat copy of AmrTreeNode and SurroundingBlocks (SayakaTree.h:52-152, 336-526) */
 void sy_tree_export(const sy_tree* t, int* level, int* parent, int* which_child, int* child8,
                     int* nbr6, int* node_type, int* node_type2, double* lo3, double* hi3,

@@ -418,6 +418,42 @@ void ref_smooth(int m, int solcomp, int rhscomp) {
 }
 void ref_fill(int m, int comp) { g_mac->mg_fillbndry_level(m, g_mac->getDataBuf(), comp, 1, 1); }
 
+// A one-component cell state registered with the tree the way the reference's demos do it
+// (Test6.h:596-660): a FillPatch with its default injection prolongation and a BCPatch with
+// Neumann on the wall code physbc, so that regrid packs the data along with the blocks
+// (SayakaRegrid.cpp:530-591) and initNewBlockDataAfterRegrid (987-1105) fills new blocks.
+void ref_state_create(int ngrow, int physbc) {
+    TreeData* st = TreeData::CreateCellData(*g_tree, 1, ngrow);
+    BCPatch* bc = new BCPatch();
+    bc->defineByData(*st);
+    for (int q = 0; q < SurroundingIndex::NumSurr; q++) {
+        SurroundingIndex s(q);
+        bc->boundaryRegister(0).setBCMap(s.ipos(), s.jpos(), s.kpos(), physbc, 0);
+    }
+    FillPatch* fp = new FillPatch(*g_tree, *st);
+    fp->setBC(*bc);
+    TreeStateData tsd;
+    tsd.new_data = st;
+    tsd.bc_patch = bc;
+    tsd.fill_new_patch = fp;
+    g_tree->treeStateData.push_back(tsd);
+    IndexBox gb = valid_box();
+    gb.extend(ngrow);
+    std::vector<double> zero((size_t)g_tree->maxBlockNum * gb.stride(), 0.0);
+    copy_in_box(*st, 0, gb, zero.data(), g_tree->maxBlockNum);
+}
+void ref_state_set(const double* valid) {
+    copy_in_box(*g_tree->treeStateData.back().new_data, 0, valid_box(), valid, g_tree->numBlocks);
+}
+void ref_state_get(double* valid) {
+    copy_out_box(*g_tree->treeStateData.back().new_data, 0, valid_box(), valid, g_tree->numBlocks);
+}
+void ref_tree_init_new_data() {
+    quiet_begin();
+    g_tree->initNewBlockDataAfterRegrid();
+    quiet_end();
+}
+
 // MacSolver::mg_flux_level (SayakaMacOperator.cpp:161-239): phi (valid values, ghosts zero
 // before the reference fills them) in, the face flux of direction dir out for every block
 // (the flux array starts at zero).

@@ -69,6 +69,10 @@ def load(ndim: int):
         "ref_smooth": ([_i, _i, _i], None),
         "ref_fill": ([_i, _i], None),
         "ref_mac_flux": ([_i, _i, _p, _p], None),
+        "ref_state_create": ([_i, _i], None),
+        "ref_state_set": ([_p], None),
+        "ref_state_get": ([_p], None),
+        "ref_tree_init_new_data": ([], None),
         "ref_restrict": ([_i, _i], None),
         "ref_prolong": ([_i, _i], None),
         "ref_dot": ([_i, _i, _i], _d),
@@ -115,6 +119,21 @@ class RefTree:
 
     def regrid(self):
         return self.lib.ref_tree_regrid()
+
+    # a registered one-component cell state (injection FillPatch, Neumann walls)
+    def state_create(self, ngrow, physbc):
+        self.lib.ref_state_create(ngrow, physbc)
+
+    def state_set(self, v):
+        self.lib.ref_state_set(_ptr(np.ascontiguousarray(v, dtype=np.float64)))
+
+    def state_get(self, ncell):
+        out = np.zeros((self.lib.ref_tree_numblocks(), ncell))
+        self.lib.ref_state_get(_ptr(out))
+        return out
+
+    def init_new_data(self):
+        self.lib.ref_tree_init_new_data()
 
     def export(self):
         n = self.lib.ref_tree_numblocks()

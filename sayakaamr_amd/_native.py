@@ -35,6 +35,7 @@ _HOST_SIG = {
     "sy_tree_finest": ([vp], i32),
     "sy_tree_set_flags": ([vp, vp, vp, i32], i32),
     "sy_tree_regrid": ([vp], i32),
+    "sy_tree_regrid_origin": ([vp, vp], None),
     "sy_tree_export": ([vp] + [vp] * 14, None),
     "sy_tree_level_count": ([vp, i32], i32),
     "sy_tree_level_blocks": ([vp, i32, vp], None),

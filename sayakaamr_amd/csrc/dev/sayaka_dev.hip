@@ -21,6 +21,7 @@
 #include "kernels.h"
 #include "mixed.h"
 #include "flux.h"
+#include "remap.h"
 
 using namespace sydev;
 
@@ -121,6 +122,9 @@ struct sayaka_dev_ctx {
     float* d_pc_s0inv = nullptr;   // [PC_MAXBLK][PC_MAXBLK] (row-major nl x nl in use)
     int pc_enable = 1, pc_version = -1, pc_ok = 0;
     double* d_flux = nullptr;      // face flux of one direction, [cap][nface] (allocated on first use)
+    // regrid data movement (allocated on first use): old field copy, origin map, new-block list
+    double* d_remap_tmp = nullptr;
+    int* d_remap_idx = nullptr;
 };
 
 namespace {
@@ -672,6 +676,22 @@ struct Impl {
         return 0;
     }
 
+    // regrid data movement of one field (flux.h): gather by origin, then inject the new blocks
+    // level by level; idx = [origin (nblk)][new blocks sorted by level]; lv[l] their ranges
+    static int remap(sayaka_dev_ctx* c, double* x, int nold, const int* d_origin, const int* d_new,
+                     const std::vector<ListRef>& lv) {
+        HIPCHK(hipMemcpyAsync(c->d_remap_tmp, x, (size_t)nold * G::NCELL * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+        hipLaunchKernelGGL((k_remap_gather<D, NB>), dim3(c->nblk), dim3(G::THREADS), 0, c->stream, d_origin,
+                           c->d_remap_tmp, x);
+        for (const ListRef& r : lv)
+            if (r.n > 0)
+                hipLaunchKernelGGL((k_inject<D, NB>), dim3(r.n), dim3(G::THREADS), 0, c->stream, d_new + r.off,
+                                   c->d_binfo, x);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     static int dot(sayaka_dev_ctx* c, int m, const double* x, const double* y, double* dotv, double* mx) {
         VecArgs a = vec_args(c, c->tw[m]);
         a.src = x;
@@ -900,6 +920,8 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
     if (c->d_flux) hipFree(c->d_flux);
+    if (c->d_remap_tmp) hipFree(c->d_remap_tmp);
+    if (c->d_remap_idx) hipFree(c->d_remap_idx);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
@@ -1174,6 +1196,39 @@ int sayaka_dev_prolong(sayaka_dev_ctx* c, int fl, int field) {
     if (!c || !field_ok(field)) return fail(-1, "bad argument");
     int e = dispatch(c, [&](auto I) { return decltype(I)::prolong(c, fl, c->d_field[field]); });
     if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_remap(sayaka_dev_ctx* c, const int* origin, int nold, const int* fields, int nfields) {
+    if (!c || !origin || !fields || nfields < 0 || nold < 1 || nold > c->cap) return fail(-1, "bad argument");
+    for (int q = 0; q < nfields; q++)
+        if (!field_ok(fields[q])) return fail(-1, "bad field");
+    const int n = c->nblk;
+    std::vector<int> idx(origin, origin + n);
+    std::vector<ListRef> lv;
+    for (int l = 1; l <= c->maxlevel; l++) {
+        ListRef r{(int)idx.size(), 0};
+        for (int b = 0; b < n; b++) {
+            if (origin[b] >= nold) return fail(-1, "origin beyond the old block count");
+            if (origin[b] < 0 && c->h_binfo[b].level == l) {
+                if (c->h_binfo[b].parent < 0) return fail(-1, "a new block without a parent");
+                idx.push_back(b);
+            }
+        }
+        r.n = (int)idx.size() - r.off;
+        lv.push_back(r);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->d_remap_tmp) HIPCHK(hipMalloc(&c->d_remap_tmp, (size_t)c->cap * c->ncell * sizeof(double)));
+    if (!c->d_remap_idx) HIPCHK(hipMalloc(&c->d_remap_idx, (size_t)2 * c->cap * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(c->d_remap_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    for (int q = 0; q < nfields; q++) {
+        int e = dispatch(c, [&](auto I) {
+            return decltype(I)::remap(c, c->d_field[fields[q]], nold, c->d_remap_idx, c->d_remap_idx, lv);
+        });
+        if (e) return e;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

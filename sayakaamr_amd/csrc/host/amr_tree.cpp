@@ -149,6 +149,8 @@ int AmrTree::regrid() {
         refine_save[i] = refineFlag[i];
     }
     std::fill(newChildFlag.begin(), newChildFlag.end(), 0);
+    regridOrigin.assign(maxTreeBlockNum, -1);
+    for (int i = 0; i < numBlocks; i++) regridOrigin[i] = i;
     checkCoarsen();
     // a leaf is not coarsened when its parent is being refined
     for (int i = 0; i < numBlocks; i++)
@@ -357,6 +359,7 @@ int AmrTree::coarsenBlocks(int oldBlockNum) {
             if (ipack != i) {
                 blocks[ipack] = blocks[i];
                 newChildFlag[ipack] = newChildFlag[i];
+                regridOrigin[ipack] = regridOrigin[i];
             }
             newloc[i] = ipack++;
         } else {

@@ -57,6 +57,11 @@ int sy_tree_regrid(sy_tree* h) {
     return t->regrid() == 0 ? t->numBlocks : -1;
 }
 
+void sy_tree_regrid_origin(const sy_tree* h, int* origin) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    for (int i = 0; i < t->numBlocks; i++) origin[i] = i < (int)t->regridOrigin.size() ? t->regridOrigin[i] : i;
+}
+
 void sy_tree_export(const sy_tree* h, int* level, int* parent, int* which_child, int* child8, int* nbr6,
                     int* node_type, int* node_type2, double* lo3, double* hi3, double* center3,
                     double* length3, int* surr27, int* surrtype27, int* surrbcref27) {

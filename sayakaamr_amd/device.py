@@ -57,6 +57,7 @@ _SIG = {
     "sayaka_dev_set_mixed_bottom": ([vp, f64, f64, i32], i32),
     "sayaka_dev_set_mixed_coarse": ([vp, i32], i32),
     "sayaka_dev_flux": ([vp, i32, i32, i32, vp], i32),
+    "sayaka_dev_remap": ([vp, vp, i32, vp, i32], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -203,6 +204,13 @@ class MacSolverGPU:
 
     def prolong(self, fl, field):
         _chk(self.lib.sayaka_dev_prolong(self.h, fl, field), "prolong")
+
+    def remap(self, origin, nold, fields):
+        """After prepare() with a regridded tree: move fields to the new numbering (origin
+        from AmrTree.regrid_origin) and inject the new blocks from their parents."""
+        o = np.ascontiguousarray(origin, dtype=np.int32)
+        f = np.ascontiguousarray(fields, dtype=np.int32)
+        _chk(self.lib.sayaka_dev_remap(self.h, N.ptr(o), int(nold), N.ptr(f), f.size), "remap")
 
     def flux(self, m, d, phi):
         """MacSolver::mg_flux_level: face flux of direction d on tower m from field phi,

@@ -63,6 +63,12 @@ class AmrTree:
             raise MemoryError("AmrTree::regrid: block pool overflow")
         return n
 
+    def regrid_origin(self):
+        """Per block, its id before the last regrid (-1 for blocks that regrid created)."""
+        out = np.zeros(self.numblocks, np.int32)
+        self.lib.sy_tree_regrid_origin(self.h, N.ptr(out))
+        return out
+
     def build_shell(self, spec, centre, r0, factor):
         """Create the tree and refine around the shell |x - centre| = r0 (native loop; same
         result as workloads.build_tree with workloads.shell_criterion)."""

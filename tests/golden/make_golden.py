@@ -7,6 +7,7 @@ against on machines where the reference is absent.
 
     python tests/golden/make_golden.py         (operators, solves, regrid, C1 summary)
     python tests/golden/make_golden.py flux    (face fluxes: tests/golden/{c0,s3}_flux.npz)
+    python tests/golden/make_golden.py remap   (regrid data movement: regrid_remap{2,3}.npz)
 """
 from __future__ import annotations
 
@@ -200,6 +201,47 @@ def regrid_sequence():
     print("regrid_sequence: 6 steps")
 
 
+def regrid_remap(ndim):
+    """The regrid sequence of a moving sphere (3D, 4^3 blocks) / ring (2D, 8^2 blocks) with a
+    registered cell state: before every regrid the state holds a distinct value per cell
+    (step * 1e7 + block * ncell + cell, so that a copied value names its source), after regrid
+    + initNewBlockDataAfterRegrid the state the reference produced (packed survivors,
+    injected new blocks)."""
+    spec = W.small_spec(ndim, 4, nb=4 if ndim == 3 else 8)
+    rt = R.RefTree(ndim)
+    nc = spec.nb ** ndim
+    if ndim == 3:
+        crit_at = lambda x: W.shell_criterion(centre=(x, 0.5, 0.5), r0=0.18)
+    else:
+        crit_at = lambda x: W.ring_criterion(centre=(x, 0.5), r0=0.2, half=0.04)
+    W.build_tree(rt, spec, crit_at(0.35))
+    rt.state_create(spec.ngrow, W.WALL)
+    out = {}
+    for step in range(6):
+        crit = crit_at(0.35 + 0.05 * step)
+        t = rt.export()
+        nblk = t["level"].shape[0]
+        pre = (step * 1e7 + np.arange(nblk * nc, dtype=np.float64)).reshape(nblk, nc)
+        rt.state_set(pre)
+        leaf_or_parent = (t["node_type"] == 1) | (t["node_type"] == 2)
+        hit = np.zeros(nblk, bool)
+        cand = np.nonzero(leaf_or_parent)[0]
+        hit[cand] = crit(t, spec.nb, spec.ndim, cand)
+        refine = (hit & (t["node_type"] == 1)).astype(np.int32)
+        coarsen = ((~hit) & (t["node_type"] == 1) & (t["level"] > spec.minlevel)).astype(np.int32)
+        rt.set_refine_flags(refine)
+        rt.set_coarsen_flags(coarsen)
+        rt.regrid()
+        rt.init_new_data()
+        out[f"step{step}_pre"] = pre
+        out[f"step{step}_refine"] = refine
+        out[f"step{step}_coarsen"] = coarsen
+        out[f"step{step}_post"] = rt.state_get(nc)
+        out.update(tree_arrays(f"step{step}_", rt.export()))
+    np.savez_compressed(os.path.join(HERE, f"regrid_remap{ndim}.npz"), **out)
+    print(f"regrid_remap{ndim}: 6 steps")
+
+
 def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     """MacSolver::mg_flux_level on the tower of every MG level and every direction, for the
     seeded input of case_ops (the same tree, coefficients and op_x) and for the case's own
@@ -235,6 +277,10 @@ def case_flux(name, spec, crit, inp_fn, bc, seed=7):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "remap":
+        regrid_remap(3)
+        regrid_remap(2)
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "flux":
         case_flux("c0", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
         case_flux("s3", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs, W.NEUMANN)
