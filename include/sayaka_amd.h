@@ -142,6 +142,10 @@ int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* ctx, int two_level);
  * colour in place, in the reference's sub-level order.  No reference counterpart. */
 int sayaka_dev_set_mixed_smoother(sayaka_dev_ctx* ctx, int merged);
 
+/* Copy one field into another on the device (TreeData::Copy of the valid region,
+ * SayakaTreeData.cpp:267-290). */
+int sayaka_dev_copy(sayaka_dev_ctx* ctx, int dst, int src);
+
 /* Set every value of a field to zero on the device (the reference's setZeroSol,
  * SayakaMacSolver.h:125-128, without a host copy). */
 int sayaka_dev_fill(sayaka_dev_ctx* ctx, int field, double value);

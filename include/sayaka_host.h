@@ -45,6 +45,9 @@ void sy_tree_export(const sy_tree* t, int* level, int* parent, int* which_child,
  * factor * blockLength of it), as the demos' refinement loops do with their own criteria
  * (Test7.h:411-418).  Returns numBlocks, -1 on pool overflow. */
 int sy_tree_build_shell(sy_tree* t, const double* centre, double r0, double factor);
+/* refine / coarsen flags of the current leaves for the shell |x - centre| = r0 (the rule of
+ * sy_tree_build_shell); returns the number of flagged blocks; follow with sy_tree_regrid */
+int sy_tree_flag_shell(sy_tree* t, const double* centre, double r0, double factor);
 /* AmrTree::getTreeLevel (SayakaTree.h:317-318) */
 int sy_tree_level_count(const sy_tree* t, int level);
 void sy_tree_level_blocks(const sy_tree* t, int level, int* out);

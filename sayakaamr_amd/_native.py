@@ -36,6 +36,7 @@ _HOST_SIG = {
     "sy_tree_set_flags": ([vp, vp, vp, i32], i32),
     "sy_tree_regrid": ([vp], i32),
     "sy_tree_regrid_origin": ([vp, vp], None),
+    "sy_tree_flag_shell": ([vp, vp, f64, f64], i32),
     "sy_tree_export": ([vp] + [vp] * 14, None),
     "sy_tree_level_count": ([vp, i32], i32),
     "sy_tree_level_blocks": ([vp, i32, vp], None),

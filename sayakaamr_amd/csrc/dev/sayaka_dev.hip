@@ -131,6 +131,10 @@ namespace {
 
 void cnn_free(sayaka_dev_ctx::Cnn& n);
 
+__global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 hipEvent_t pool_event(sayaka_dev_ctx* c) {
     if (!c->ev_pool.empty()) {
         hipEvent_t e = c->ev_pool.back();
@@ -1086,13 +1090,21 @@ int sayaka_dev_set_coefs(sayaka_dev_ctx* c, double alpha, double beta, const dou
     c->beta = beta;
     const size_t cells = (size_t)c->nblk * c->ncell;
     const size_t nfd = (size_t)c->nface;
-    std::vector<double> ones(std::max(cells, (size_t)c->nblk * c->ndim * nfd), 1.0);
+    const size_t faces = (size_t)c->nblk * c->ndim * nfd;
+    // unit coefficients are filled on the device (a regridded tree resets them every step)
     if (vf) HIPCHK(hipMemcpy(c->d_vf, vf, cells * sizeof(double), hipMemcpyHostToDevice));
-    else HIPCHK(hipMemcpy(c->d_vf, ones.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+    else hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->d_vf, cells, 1.0);
     // host arrays are [dir][block][face]; device arrays are [block][dir][face]
-    std::vector<double> tmp((size_t)c->nblk * c->ndim * nfd);
+    std::vector<double> tmp;
     for (int which = 0; which < 2; which++) {
         const double* const* src = which == 0 ? b : af;
+        bool any = false;
+        for (int d = 0; d < c->ndim; d++) any = any || (src && src[d]);
+        if (!any) {
+            hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, which == 0 ? c->d_b : c->d_af, faces, 1.0);
+            continue;
+        }
+        tmp.resize(faces);
         for (int blk = 0; blk < c->nblk; blk++)
             for (int d = 0; d < c->ndim; d++) {
                 double* o = tmp.data() + ((size_t)blk * c->ndim + d) * nfd;
@@ -1281,6 +1293,16 @@ int sayaka_dev_solve(sayaka_dev_ctx* c, const sayaka_solve_params* p, sayaka_sol
         return fail(-1, "bad precision");
     HIPCHK(hipSetDevice(c->device));
     return dispatch(c, [&](auto I) { return decltype(I)::solve(c, p, st); });
+}
+
+int sayaka_dev_copy(sayaka_dev_ctx* c, int dst, int src) {
+    if (!c || !field_ok(dst) || !field_ok(src)) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    if (dst != src)
+        HIPCHK(hipMemcpyAsync(c->d_field[dst], c->d_field[src], (size_t)c->nblk * c->ncell * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int sayaka_dev_fill(sayaka_dev_ctx* c, int field, double value) {

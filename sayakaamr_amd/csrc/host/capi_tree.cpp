@@ -124,6 +124,59 @@ int sy_tree_build_shell(sy_tree* h, const double* centre, double r0, double fact
     return t->numBlocks;
 }
 
+// Refine / coarsen flags for the shell |x - centre| = r0 on the current tree: a leaf is hit
+// when one of its cell centres lies within factor * (its block edge) of the shell (the rule
+// of sy_tree_build_shell); hit leaves below maxAmrLevel are flagged for refinement, leaves
+// that are not hit and above minAmrLevel for coarsening (regrid keeps only complete sibling
+// sets and the 2:1 balance).  Blocks whose box
+// cannot reach the band are rejected before their cells are visited.  Returns the number of
+// flagged blocks.
+int sy_tree_flag_shell(sy_tree* h, const double* centre, double r0, double factor) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    const int nb = t->nb, nz = t->ndim == 3 ? nb : 1;
+    std::fill(t->refineFlag.begin(), t->refineFlag.end(), 0);
+    std::fill(t->coarsenFlag.begin(), t->coarsenFlag.end(), 0);
+    int nflag = 0;
+    for (int b = 0; b < t->numBlocks; b++) {
+        const AmrTreeNode& n = t->blocks[b];
+        if (n.nodeType != NodeIsLeaf) continue;
+        const double half = factor * n.blockLength(0);
+        double dmin2 = 0.0, dmax2 = 0.0;
+        for (int d = 0; d < t->ndim; d++) {
+            const double lo = n.boundBox.vlo(d) - centre[d], hi = n.boundBox.vhi(d) - centre[d];
+            const double near = lo > 0 ? lo : (hi < 0 ? -hi : 0.0);
+            const double far = std::max(std::fabs(lo), std::fabs(hi));
+            dmin2 += near * near;
+            dmax2 += far * far;
+        }
+        bool hit = false;
+        if (std::sqrt(dmin2) <= r0 + half && std::sqrt(dmax2) >= r0 - half) {
+            double dx[3];
+            for (int d = 0; d < 3; d++) dx[d] = n.blockLength(d) / nb;
+            for (int k = 0; k < nz && !hit; k++)
+                for (int j = 0; j < nb && !hit; j++)
+                    for (int i = 0; i < nb && !hit; i++) {
+                        const double x = n.boundBox.vlo(0) + (i + 0.5) * dx[0];
+                        const double y = n.boundBox.vlo(1) + (j + 0.5) * dx[1];
+                        double r2 = (x - centre[0]) * (x - centre[0]) + (y - centre[1]) * (y - centre[1]);
+                        if (t->ndim == 3) {
+                            const double z = n.boundBox.vlo(2) + (k + 0.5) * dx[2];
+                            r2 += (z - centre[2]) * (z - centre[2]);
+                        }
+                        hit = std::fabs(std::sqrt(r2) - r0) < half;
+                    }
+        }
+        if (hit && n.refineLevel < t->maxAmrLevel) {
+            t->refineFlag[b] = 1;
+            nflag++;
+        } else if (!hit && n.refineLevel > t->minAmrLevel) {
+            t->coarsenFlag[b] = 1;
+            nflag++;
+        }
+    }
+    return nflag;
+}
+
 int sy_tree_level_count(const sy_tree* h, int l) {
     const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
     return (l >= 1 && l <= t->maxAmrLevel) ? (int)t->levels[l].size() : 0;

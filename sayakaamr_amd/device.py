@@ -58,6 +58,7 @@ _SIG = {
     "sayaka_dev_set_mixed_coarse": ([vp, i32], i32),
     "sayaka_dev_flux": ([vp, i32, i32, i32, vp], i32),
     "sayaka_dev_remap": ([vp, vp, i32, vp, i32], i32),
+    "sayaka_dev_copy": ([vp, i32, i32], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -249,6 +250,10 @@ class MacSolverGPU:
 
     def sync(self):
         _chk(self.lib.sayaka_dev_sync(self.h), "sync")
+
+    def copy(self, dst, src):
+        """TreeData::Copy on the device: field dst = field src."""
+        _chk(self.lib.sayaka_dev_copy(self.h, dst, src), "copy")
 
     def zero(self, field):
         """setZeroSol on the device for any field."""

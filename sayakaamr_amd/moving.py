@@ -1,0 +1,137 @@
+"""Moving-boundary run on the device: a solid sphere translating through the domain, the tree
+regridded around it every step, the previous step's pressure carried to the new tree and used
+as the initial guess of the next solve.
+
+Per step (the regrid / projection cycle of the reference's flow demo, Test7.h:411-418, 612-653):
+  1. flag + regrid on the host tree (AmrTree::regrid, SayakaRegrid.cpp:26-976): leaves within
+     factor * (block edge) of the moved shell are refined, the others coarsened, up to three
+     passes per step; the block origins of the passes are composed;
+  2. the new tables to the device (sayaka_dev_set_topology);
+  3. the pressure to the new numbering (sayaka_dev_remap: survivors moved, new blocks injected
+     from their parents -- the reference's packing and initNewBlockDataAfterRegrid);
+  4. cut-cell coefficients of the moved sphere (density ratio 1000) and a Gaussian-blob rhs,
+     generated on the device (sayaka_gen_sphere_blobs);
+  5. the mixed-precision solve to 1e-10 |b|_inf from the carried pressure, and, for comparison,
+     from zero.
+
+    python -m sayakaamr_amd.moving --steps 10
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import device as D
+from . import workloads as W
+from .tree import AmrTree
+
+
+def compose(first, second):
+    """Origins of two regrids in a row: block b after the second came from block
+    first[second[b]] before the first (-1: created by either)."""
+    out = np.full(second.shape[0], -1, np.int32)
+    m = second >= 0
+    out[m] = first[second[m]]
+    return out
+
+
+def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=1000.0, compare_zero=True,
+        max_blocks=45000, passes=3, log=None):
+    log = log or (lambda m: print(f"[moving] {m}", file=sys.stderr, flush=True))
+    tb = AmrTree(spec.ndim, max_blocks=max_blocks)
+    tree = tb.build_shell(spec, (x0, 0.5, 0.5), radius, W.SHELL_FACTOR)
+    dev = D.MacSolverGPU(spec.ndim, spec.nb, max_blocks, device=0)
+    rng = np.random.default_rng(7)
+    blobs = dict(c=rng.uniform(0, 1, (8, 3)), w=rng.uniform(0.02, 0.1, 8), a=rng.normal(0, 1, 8))
+    dxf = (spec.hi[0] - spec.lo[0]) / spec.ndiv[0] / spec.nb / 2 ** (spec.maxlevel - 1)
+    rows = []
+    try:
+        for step in range(steps + 1):
+            centre = (x0 + shift * step, 0.5, 0.5)
+            row = dict(step=step)
+            t0 = time.perf_counter()
+            if step > 0:
+                nold = tree["level"].shape[0]
+                origin = np.arange(nold, dtype=np.int32)
+                for _ in range(passes):
+                    tb.flag_shell(centre, radius, W.SHELL_FACTOR)
+                    n0 = tb.numblocks
+                    tb.regrid()
+                    o = tb.regrid_origin()
+                    origin = compose(origin, o)
+                    if tb.numblocks == n0 and np.array_equal(o, np.arange(n0)):
+                        break
+                tree = tb.export()
+            t1 = time.perf_counter()
+            dev.prepare(tree, spec)
+            dev.set_bc([W.WALL], [W.NEUMANN])
+            dev.sync()
+            t2 = time.perf_counter()
+            if step > 0:
+                dev.remap(origin, nold, [D.SOL])
+            t3 = time.perf_counter()
+            dev.gen_sphere_blobs(tree, centre, radius, ratio, 1.5 * dxf, 4, blobs)
+            L = int(tree["level"].max())
+            bmax = dev.dot(L, D.RHS, D.RHS)[1]
+            dev.sync()
+            t4 = time.perf_counter()
+            tol = 1e-10 * bmax
+            if step == 0:
+                dev.zero(D.SOL)
+            if compare_zero and step > 0:
+                dev.copy(D.TMP, D.SOL)
+                dev.zero(D.SOL)
+                dev.sync()
+                tz = time.perf_counter()
+                rz = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=200, precision=D.MIXED)
+                dev.sync()
+                row.update(zero_ms=(time.perf_counter() - tz) * 1e3, zero_iters=rz["iters"])
+                dev.copy(D.SOL, D.TMP)
+            dev.sync()
+            t5 = time.perf_counter()
+            r = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=200, precision=D.MIXED)
+            dev.sync()
+            t6 = time.perf_counter()
+            leaf = tree["node_type"] == 1
+            row.update(blocks=int(tree["level"].shape[0]), leaf_cells=int(leaf.sum()) * spec.nb ** spec.ndim,
+                       regrid_ms=(t1 - t0) * 1e3, tables_ms=(t2 - t1) * 1e3, remap_ms=(t3 - t2) * 1e3,
+                       coef_ms=(t4 - t3) * 1e3, solve_ms=(t6 - t5) * 1e3, iters=r["iters"], status=r["status"],
+                       rel_resid=float(r["rnorm"] / bmax))
+            rows.append(row)
+            log(json.dumps({k: (round(v, 2) if isinstance(v, float) else v) for k, v in row.items()}))
+            if r["status"] != 0:
+                raise RuntimeError(f"step {step}: solve status {r['status']}")
+    finally:
+        dev.close()
+    return rows
+
+
+def summarize(rows):
+    s = [r for r in rows if r["step"] > 0]
+    out = {k: float(np.mean([r[k] for r in s])) for k in ("regrid_ms", "tables_ms", "remap_ms", "coef_ms",
+                                                            "solve_ms", "iters")}
+    if s and "zero_ms" in s[0]:
+        out["zero_guess_solve_ms"] = float(np.mean([r["zero_ms"] for r in s]))
+        out["zero_guess_iters"] = float(np.mean([r["zero_iters"] for r in s]))
+    out["step_ms"] = out["regrid_ms"] + out["tables_ms"] + out["remap_ms"] + out["coef_ms"] + out["solve_ms"]
+    out["steps"] = len(s)
+    out["leaf_cells_mean"] = float(np.mean([r["leaf_cells"] for r in s]))
+    out["max_rel_resid"] = float(max(r["rel_resid"] for r in s))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--no-zero", action="store_true", help="skip the zero-guess comparison solve")
+    args = ap.parse_args(argv)
+    rows = run(steps=args.steps, compare_zero=not args.no_zero)
+    print(json.dumps(summarize(rows)))
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,11 @@ class AmrTree:
             raise MemoryError("AmrTree::regrid: block pool overflow")
         return n
 
+    def flag_shell(self, centre, r0, factor):
+        """Refine / coarsen flags of the current leaves for the shell |x - centre| = r0."""
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        return self.lib.sy_tree_flag_shell(self.h, N.ptr(c), float(r0), float(factor))
+
     def regrid_origin(self):
         """Per block, its id before the last regrid (-1 for blocks that regrid created)."""
         out = np.zeros(self.numblocks, np.int32)

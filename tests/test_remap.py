@@ -69,3 +69,10 @@ def test_injection_closed_form():
                         jc = (j + ((wc >> 1) & 1) * nb) // 2
                         kc = (k + ((wc >> 2) & 1) * nb) // 2 if ndim == 3 else 0
                         assert f[k, j, i] == parent[(kc * nb + jc) * nb + ic], (ndim, wc, i, j, k, h)
+
+
+def test_compose_origins():
+    from sayakaamr_amd import moving as MV
+    first = np.array([0, 1, -1, 3, 4], np.int32)      # block 2 new in the first regrid
+    second = np.array([0, 2, 3, -1, 4], np.int32)     # block 3 new in the second
+    assert np.array_equal(MV.compose(first, second), [0, -1, 3, -1, 4])
