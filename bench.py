@@ -19,12 +19,15 @@ Reported (one JSON line on stdout):
                    (zero guess vs guess-with-fallback, mixed solver; FP64 beside it);
   hbm_regime       the 16^3-block, 139 M-cell snapshot: operator and smoother GB/s where the
                    fields no longer fit in the caches, and its mixed solve;
+  moving           the moving-boundary cycle on the same 16^3-block tree: per step host regrid,
+                   tables, device remap of the pressure, device coefficients and the mixed
+                   solve warm-started from the carried pressure (zero-guess solve beside it);
   roofline         the dominant HBM kernel class of the headline solve, measured with HIP
                    events on the solver's stream inside the timed region; `traffic` from the
                    committed rocprofv3 counter summary (profiles/c1_fp64_smooth_traffic.md);
   cpu_baseline     the compiled reference solver (or the C oracle) on 1 core.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-cpu] [--no-learned] [--no-large]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-cpu] [--no-learned] [--no-large] [--no-moving]
     python bench.py --pmc-workload          (the short program the counter passes profile)
     python bench.py --summarize-pmc DIR...  (turn rocprofv3 --pmc csv output into the summary)
 """
@@ -408,6 +411,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 139 M-cell snapshot")
+    ap.add_argument("--no-moving", action="store_true", help="skip the moving-boundary cycle")
+    ap.add_argument("--moving-steps", type=int, default=10)
     ap.add_argument("--pmc-workload", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -471,6 +476,16 @@ def main():
     if learned:
         log(f"learned: {learned}")
     large = None if args.no_large else large_leg(D, W)
+    moving = None
+    if not args.no_moving:
+        from sayakaamr_amd import moving as MV
+
+        moving = MV.summarize(MV.run(steps=args.moving_steps, log=lambda m: log(f"moving: {m}")))
+        moving = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in moving.items()}
+        moving["workload"] = ("16^3 blocks, root 4^3, levels 1-6 around an r=0.15 sphere moving +0.01/step in x "
+                              "from x=0.5, regrid every step (up to 3 passes), density ratio 1000, blob rhs, "
+                              "mixed solve to 1e-10 |b|_inf warm-started from the carried pressure")
+        log(f"moving: {moving}")
 
     # V-cycle rate: preconditioner time / V-cycles of the FP64 solve
     nvc = 2 * r64["iters"] - (1 if r64["half_step"] else 0)
@@ -517,6 +532,7 @@ def main():
                   "seeds": mixed_rows, "kernels": classes_mixed},
         "learned": learned,
         "hbm_regime": large,
+        "moving": moving,
         "roofline": {"bound": "hbm", "kernel": roof_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                      "launches": prof["launches"], "avg_launch_us": round(avg_launch_us, 3),
