@@ -197,6 +197,33 @@ int sayaka_cnn_guess(sayaka_dev_ctx* ctx, int field, double s_rhs, double out_sc
 int sayaka_gen_sphere_blobs(sayaka_dev_ctx* ctx, const double* lo3, const double* dx, const double* centre,
                             double radius, double ratio, double eps, int nsub, int nblob, const double* blob_c,
                             const double* blob_w, const double* blob_a, double* mean_out);
+/* The sphere's coefficients of sayaka_gen_sphere_blobs alone (vf, af, B; coarse/fine faces
+ * synchronised), leaving the rhs as it is. */
+int sayaka_gen_sphere(sayaka_dev_ctx* ctx, const double* lo3, const double* dx, const double* centre, double radius,
+                      double ratio, double eps, int nsub);
+
+/* Projection of a face velocity, on the device (the projection step of the reference's flow
+ * demo, Test7.h:580-657, with dt = 1 and p's sign flipped: rhs = div u, solve, u -= B flux with
+ * mg_flux_level's flux,
+ * fill_vel_all's coarse/fine synchronisation).  No single reference function: the demo drives
+ * TreeData face fields and MacSolver from the host.
+ *
+ * sayaka_gen_velocity: a seeded face velocity u ~ U(-1,1) for the current tree, keyed on (seed,
+ * level, global face index, direction) so that a face shared by two blocks of one level gets one
+ * value (lo3 [nblk][3] block corners, dlo3 the domain's lower corner, root3 the root block edge);
+ * zero_walls = 1 sets it to 0 on the domain walls.  Invalidated by sayaka_dev_set_topology.
+ * sayaka_gen_divergence_rhs: synchronises the velocity's coarse/fine faces (coarse side = average
+ * of the fine faces) and writes rhs = div(a u) / (dv vf) on fluid cells (0 on solid cells) minus
+ * its volume-weighted mean over the fluid leaf cells (returned in *mean_out).
+ * sayaka_dev_project: u -= B flux(phi) on every face of the finest tower (flux as
+ * sayaka_dev_flux), faces re-synchronised; *div_before / *div_after = max |div u - mean| over
+ * the tower before and after.
+ * sayaka_dev_get_velocity: the face velocity to the host, [nblk][ndim][(nb+1) nb^(ndim-1)]. */
+int sayaka_gen_velocity(sayaka_dev_ctx* ctx, const double* lo3, const double* dlo3, const double* root3,
+                        unsigned long long seed, int zero_walls);
+int sayaka_gen_divergence_rhs(sayaka_dev_ctx* ctx, double* mean_out);
+int sayaka_dev_project(sayaka_dev_ctx* ctx, int phi, double* div_before, double* div_after);
+int sayaka_dev_get_velocity(sayaka_dev_ctx* ctx, double* out);
 
 /* Block until the context's stream is idle. */
 int sayaka_dev_sync(sayaka_dev_ctx* ctx);

@@ -22,6 +22,7 @@
 #include "mixed.h"
 #include "flux.h"
 #include "remap.h"
+#include "projection.h"
 
 using namespace sydev;
 
@@ -125,6 +126,10 @@ struct sayaka_dev_ctx {
     // regrid data movement (allocated on first use): old field copy, origin map, new-block list
     double* d_remap_tmp = nullptr;
     int* d_remap_idx = nullptr;
+    // projection (allocated on first use): face velocity [cap][D][nface], divergence [cap][ncell]
+    double* d_vel = nullptr;
+    double* d_div = nullptr;
+    int vel_valid = 0;             // d_vel generated for the current tree
 };
 
 namespace {
@@ -617,27 +622,33 @@ struct Impl {
         return mix(c, c->tw[L], M_F32_TO_F64, nullptr, ms.u[L], phat64, nullptr);
     }
 
-    static int set_coefs(sayaka_dev_ctx* c) {
+    // coarse/fine synchronisation of one face field [blk][D][nface] on the finest tower (the
+    // coarse side of every coarse/fine face gets the average of the fine faces), then the faces
+    // of non-leaf blocks averaged from their children, coarsest last (coef.h)
+    static void sync_faces(sayaka_dev_ctx* c, double* fd) {
         const int L = c->finest;
-        for (int which = 0; which < 2; which++) {
-            double* fd = which == 0 ? c->d_b : c->d_af;
-            for (int d = 0; d < D; d++) {
-                for (int l = L; l >= 2; l--) {
-                    ListRef fine = c->tw_sub[L][l], crse = c->tw_sub[L][l - 1];
-                    if (c->tw[L].n == 0 || fine.n <= 0 || crse.n <= 0) continue;
-                    hipLaunchKernelGGL((k_sync_fine<D, NB>), dim3(fine.n), dim3(64), 0, c->stream, dlist(c, fine),
-                                       c->d_binfo, fd, d);
-                    hipLaunchKernelGGL((k_sync_crse<D, NB>), dim3(crse.n), dim3(64), 0, c->stream, dlist(c, crse),
-                                       c->d_binfo, fd, d, L);
-                }
-                for (int l = L - 1; l >= 1; l--) {
-                    ListRef nl = c->lv_nonleaf[l];
-                    if (nl.n <= 0) continue;
-                    hipLaunchKernelGGL((k_restrict_face<D, NB>), dim3(nl.n), dim3(256), 0, c->stream, dlist(c, nl),
-                                       c->d_binfo, fd, d);
-                }
+        for (int d = 0; d < D; d++) {
+            for (int l = L; l >= 2; l--) {
+                ListRef fine = c->tw_sub[L][l], crse = c->tw_sub[L][l - 1];
+                if (c->tw[L].n == 0 || fine.n <= 0 || crse.n <= 0) continue;
+                hipLaunchKernelGGL((k_sync_fine<D, NB>), dim3(fine.n), dim3(64), 0, c->stream, dlist(c, fine),
+                                   c->d_binfo, fd, d);
+                hipLaunchKernelGGL((k_sync_crse<D, NB>), dim3(crse.n), dim3(64), 0, c->stream, dlist(c, crse),
+                                   c->d_binfo, fd, d, L);
+            }
+            for (int l = L - 1; l >= 1; l--) {
+                ListRef nl = c->lv_nonleaf[l];
+                if (nl.n <= 0) continue;
+                hipLaunchKernelGGL((k_restrict_face<D, NB>), dim3(nl.n), dim3(256), 0, c->stream, dlist(c, nl),
+                                   c->d_binfo, fd, d);
             }
         }
+    }
+
+    static int set_coefs(sayaka_dev_ctx* c) {
+        const int L = c->finest;
+        sync_faces(c, c->d_b);
+        sync_faces(c, c->d_af);
         for (int l = L - 1; l >= 1; l--) {
             ListRef nl = c->lv_nonleaf[l];
             if (nl.n <= 0) continue;
@@ -694,6 +705,60 @@ struct Impl {
                                    c->d_binfo, x);
         HIPCHK(hipGetLastError());
         return 0;
+    }
+
+    // seeded face velocity of every block into d_vel (projection.h); d_lo = [nblk][3] corners
+    static int gen_velocity(sayaka_dev_ctx* c, const double* d_lo, const double* dlo, const double* root,
+                            unsigned long long seed, int zero_walls) {
+        VelArgs a{d_lo, {0, 0, 0}, {1, 1, 1}, seed, zero_walls};
+        for (int d = 0; d < 3; d++) {
+            a.dlo[d] = dlo[d];
+            a.root[d] = root[d];
+        }
+        hipLaunchKernelGGL((k_face_velocity<D, NB>), dim3(c->nblk), dim3(256), 0, c->stream, a, c->d_binfo, c->d_vel);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    // out = div(a u) / (dv vf) on every block minus its volume-weighted mean over the fluid
+    // cells of the leaves; d_part = 2 * nblk doubles of scratch
+    static int divergence(sayaka_dev_ctx* c, double* out, double* d_part, double* mean_out) {
+        const ListRef all = c->all_blocks;
+        hipLaunchKernelGGL((k_divergence<D, NB>), dim3(all.n), dim3(G::THREADS), 0, c->stream, dlist(c, all),
+                           c->d_binfo, c->d_met, c->d_af, c->d_vel, c->d_vf, out, d_part);
+        HIPCHK(hipGetLastError());
+        std::vector<double> part(2 * (size_t)all.n);
+        HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double s = 0.0, w = 0.0;
+        for (int q = 0; q < all.n; q++) {
+            s += part[2 * q];
+            w += part[2 * q + 1];
+        }
+        const double mean = w > 0 ? s / w : 0.0;
+        hipLaunchKernelGGL(cnn::k_sub_mean, dim3(1024), dim3(256), 0, c->stream, out, c->d_vf, mean, c->nblk * G::NCELL);
+        HIPCHK(hipGetLastError());
+        if (mean_out) *mean_out = mean;
+        return 0;
+    }
+
+    // u -= B flux(phi) on the faces of the finest tower, faces re-synchronised; the largest
+    // |divergence| over the tower before and after (means removed)
+    static int project(sayaka_dev_ctx* c, const double* phi, double* d_part, double* before, double* after) {
+        const int L = c->finest;
+        const ListRef T = c->tw[L];
+        int e;
+        if ((e = divergence(c, c->d_div, d_part, nullptr))) return e;
+        if ((e = dot(c, L, c->d_div, c->d_div, nullptr, before))) return e;
+        for (int d = 0; d < D; d++) {
+            if ((e = flux(c, L, d, phi))) return e;
+            hipLaunchKernelGGL((k_vel_correct<D, NB>), dim3(T.n), dim3(256), 0, c->stream, dlist(c, T), c->d_b,
+                               c->d_flux, c->d_vel, d);
+            HIPCHK(hipGetLastError());
+        }
+        sync_faces(c, c->d_vel);
+        if ((e = divergence(c, c->d_div, d_part, nullptr))) return e;
+        return dot(c, L, c->d_div, c->d_div, nullptr, after);
     }
 
     static int dot(sayaka_dev_ctx* c, int m, const double* x, const double* y, double* dotv, double* mx) {
@@ -926,6 +991,8 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_flux) hipFree(c->d_flux);
     if (c->d_remap_tmp) hipFree(c->d_remap_tmp);
     if (c->d_remap_idx) hipFree(c->d_remap_idx);
+    if (c->d_vel) hipFree(c->d_vel);
+    if (c->d_div) hipFree(c->d_div);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
@@ -1063,6 +1130,7 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
         HIPCHK(hipMemcpy(c->d_l1pos, l1pos.data(), nblk * sizeof(int), hipMemcpyHostToDevice));
         c->pc_version = -1;
     }
+    c->vel_valid = 0;  // the face velocity belongs to the previous tree
     // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32)
     c->alpha = 0.0;
     c->beta = 1.0;
@@ -1631,15 +1699,9 @@ int sayaka_cnn_guess(sayaka_dev_ctx* c, int field, double s_rhs, double out_scal
     return 0;
 }
 
-// Inputs of a solid sphere in a density-1 fluid (cut-cell fractions, B = 1/rho) and a
-// Gaussian-blob rhs projected to zero mean over the fluid leaves, generated on the device for
-// the current tree (lo3 [nblk][3], dx [nblk] from the tree).  Returns the removed mean.
-int sayaka_gen_sphere_blobs(sayaka_dev_ctx* c, const double* lo3, const double* dx, const double* centre, double radius,
-                            double ratio, double eps, int nsub, int nblob, const double* blob_c, const double* blob_w,
-                            const double* blob_a, double* mean_out) {
-    if (!c || !lo3 || !dx || !centre || nsub < 1 || nblob < 0 || nblob > cnn::MAXBLOB) return fail(-1, "bad argument");
-    if (c->ndim != 3) return fail(-1, "3D only");
-    HIPCHK(hipSetDevice(c->device));
+// generator scratch for the current tree: block corners [nblk][3], sizes [nblk], partial sums
+// [2 nblk]; lo3 (and dx when given) uploaded
+static int gen_scratch(sayaka_dev_ctx* c, const double* lo3, const double* dx) {
     const int n = c->nblk;
     if (c->gen_cap < n) {
         if (c->d_gen) (void)hipFree(c->d_gen);
@@ -1647,33 +1709,52 @@ int sayaka_gen_sphere_blobs(sayaka_dev_ctx* c, const double* lo3, const double* 
         HIPCHK(hipMalloc(&c->d_gen, (size_t)n * 6 * sizeof(double)));
         c->gen_cap = n;
     }
-    double* d_lo = c->d_gen;
-    double* d_dx = c->d_gen + 3 * n;
+    HIPCHK(hipMemcpyAsync(c->d_gen, lo3, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (dx) HIPCHK(hipMemcpyAsync(c->d_gen + 3 * n, dx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// cut-cell fractions and B of the solid sphere, coarse/fine faces synchronised, weights
+static int gen_sphere_coefs(sayaka_dev_ctx* c, const double* lo3, const double* dx, const double* centre,
+                            double radius, double ratio, double eps, int nsub) {
+    if (c->ndim != 3) return fail(-1, "3D only");
+    if (c->nb != 8 && c->nb != 16) return fail(-1, "8^3 or 16^3 blocks");
+    HIPCHK(hipSetDevice(c->device));
+    int e = gen_scratch(c, lo3, dx);
+    if (e) return e;
+    const int n = c->nblk;
+    cnn::SphereArgs sa{c->d_gen, c->d_gen + 3 * n, {centre[0], centre[1], centre[2]}, radius, ratio, eps, nsub};
+    if (c->nb == 8)
+        hipLaunchKernelGGL(cnn::k_gen_sphere<8>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
+    else
+        hipLaunchKernelGGL(cnn::k_gen_sphere<16>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
+    HIPCHK(hipGetLastError());
+    e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
+    if (e) return e;
+    c->coef_version++;
+    return 0;
+}
+
+// Inputs of a solid sphere in a density-1 fluid (cut-cell fractions, B = 1/rho) and a
+// Gaussian-blob rhs projected to zero mean over the fluid leaves, generated on the device for
+// the current tree (lo3 [nblk][3], dx [nblk] from the tree).  Returns the removed mean.
+int sayaka_gen_sphere_blobs(sayaka_dev_ctx* c, const double* lo3, const double* dx, const double* centre, double radius,
+                            double ratio, double eps, int nsub, int nblob, const double* blob_c, const double* blob_w,
+                            const double* blob_a, double* mean_out) {
+    if (!c || !lo3 || !dx || !centre || nsub < 1 || nblob < 0 || nblob > cnn::MAXBLOB) return fail(-1, "bad argument");
+    int e = gen_sphere_coefs(c, lo3, dx, centre, radius, ratio, eps, nsub);
+    if (e) return e;
+    const int n = c->nblk;
     double* d_part = c->d_gen + 4 * n;
-    HIPCHK(hipMemcpyAsync(d_lo, lo3, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d_dx, dx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    cnn::SphereArgs sa{d_lo, d_dx, {centre[0], centre[1], centre[2]}, radius, ratio, eps, nsub};
     cnn::BlobArgs ba{};
-    ba.lo = d_lo;
-    ba.dx = d_dx;
+    ba.lo = c->d_gen;
+    ba.dx = c->d_gen + 3 * n;
     ba.nblob = nblob;
     for (int q = 0; q < nblob; q++) {
         for (int d = 0; d < 3; d++) ba.c[q][d] = blob_c[3 * q + d];
         ba.w[q] = blob_w[q];
         ba.amp[q] = blob_a[q];
     }
-    int e = 0;
-    if (c->nb == 8)
-        hipLaunchKernelGGL(cnn::k_gen_sphere<8>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
-    else if (c->nb == 16)
-        hipLaunchKernelGGL(cnn::k_gen_sphere<16>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
-    else
-        return fail(-1, "8^3 or 16^3 blocks");
-    HIPCHK(hipGetLastError());
-    // synchronise coarse/fine faces, average down, weights
-    e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
-    if (e) return e;
-    c->coef_version++;
     double* rhs = c->d_field[SAYAKA_RHS];
     if (c->nb == 8)
         hipLaunchKernelGGL(cnn::k_gen_blobs<8>, dim3(n), dim3(256), 0, c->stream, ba, c->d_binfo, c->d_vf, rhs, d_part);
@@ -1694,6 +1775,86 @@ int sayaka_gen_sphere_blobs(sayaka_dev_ctx* c, const double* lo3, const double* 
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (mean_out) *mean_out = mean;
+    return 0;
+}
+
+int sayaka_gen_sphere(sayaka_dev_ctx* c, const double* lo3, const double* dx, const double* centre, double radius,
+                      double ratio, double eps, int nsub) {
+    if (!c || !lo3 || !dx || !centre || nsub < 1) return fail(-1, "bad argument");
+    int e = gen_sphere_coefs(c, lo3, dx, centre, radius, ratio, eps, nsub);
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int vel_alloc(sayaka_dev_ctx* c) {
+    if (!c->d_vel) HIPCHK(hipMalloc(&c->d_vel, (size_t)c->cap * c->ndim * c->nface * sizeof(double)));
+    if (!c->d_div) HIPCHK(hipMalloc(&c->d_div, (size_t)c->cap * c->ncell * sizeof(double)));
+    return 0;
+}
+
+int sayaka_gen_velocity(sayaka_dev_ctx* c, const double* lo3, const double* dlo3, const double* root3,
+                        unsigned long long seed, int zero_walls) {
+    if (!c || !lo3 || !dlo3 || !root3 || c->nblk < 1) return fail(-1, "bad argument");
+    for (int d = 0; d < c->ndim; d++)
+        if (!(root3[d] > 0)) return fail(-1, "root block edge must be positive");
+    HIPCHK(hipSetDevice(c->device));
+    int e = vel_alloc(c);
+    if (!e) e = gen_scratch(c, lo3, nullptr);
+    if (!e)
+        e = dispatch(c, [&](auto I) {
+            return decltype(I)::gen_velocity(c, c->d_gen, dlo3, root3, seed, zero_walls ? 1 : 0);
+        });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->vel_valid = 1;
+    return 0;
+}
+
+static int need_vel(sayaka_dev_ctx* c) {
+    if (!c->vel_valid) return fail(-1, "no face velocity for this tree: call sayaka_gen_velocity");
+    return 0;
+}
+
+int sayaka_gen_divergence_rhs(sayaka_dev_ctx* c, double* mean_out) {
+    if (!c) return fail(-1, "bad argument");
+    int e = need_vel(c);
+    if (e) return e;
+    HIPCHK(hipSetDevice(c->device));
+    e = dispatch(c, [&](auto I) {
+        using Im = decltype(I);
+        Im::sync_faces(c, c->d_vel);
+        return Im::divergence(c, c->d_field[SAYAKA_RHS], c->d_gen + 4 * c->nblk, mean_out);
+    });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_project(sayaka_dev_ctx* c, int phi, double* div_before, double* div_after) {
+    if (!c || !field_ok(phi) || c->finest < 1) return fail(-1, "bad argument");
+    int e = need_vel(c);
+    if (e) return e;
+    HIPCHK(hipSetDevice(c->device));
+    double b = 0, a = 0;
+    e = dispatch(c, [&](auto I) {
+        return decltype(I)::project(c, c->d_field[phi], c->d_gen + 4 * c->nblk, &b, &a);
+    });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (div_before) *div_before = b;
+    if (div_after) *div_after = a;
+    return 0;
+}
+
+int sayaka_dev_get_velocity(sayaka_dev_ctx* c, double* out) {
+    if (!c || !out) return fail(-1, "bad argument");
+    int e = need_vel(c);
+    if (e) return e;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(out, c->d_vel, (size_t)c->nblk * c->ndim * c->nface * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -70,6 +70,11 @@ _SIG = {
     "sayaka_cnn_store_read": ([vp, C.c_longlong, i32, vp, vp], i32),
     "sayaka_cnn_guess": ([vp, i32, f64, f64], i32),
     "sayaka_gen_sphere_blobs": ([vp, vp, vp, vp, f64, f64, f64, i32, i32, vp, vp, vp, C.POINTER(f64)], i32),
+    "sayaka_gen_sphere": ([vp, vp, vp, vp, f64, f64, f64, i32], i32),
+    "sayaka_gen_velocity": ([vp, vp, vp, vp, C.c_ulonglong, i32], i32),
+    "sayaka_gen_divergence_rhs": ([vp, C.POINTER(f64)], i32),
+    "sayaka_dev_project": ([vp, i32, C.POINTER(f64), C.POINTER(f64)], i32),
+    "sayaka_dev_get_velocity": ([vp, vp], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
     "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                  C.POINTER(f64)], i32),
@@ -330,6 +335,44 @@ class MacSolverGPU:
         _chk(self.lib.sayaka_gen_sphere_blobs(self.h, N.ptr(lo), N.ptr(dx), N.ptr(c), radius, ratio, eps, nsub,
                                               bw.size, N.ptr(bc), N.ptr(bw), N.ptr(ba), C.byref(mean)), "gen")
         return mean.value
+
+    def gen_sphere(self, tree, centre, radius, ratio, eps, nsub):
+        """The sphere's cut-cell coefficients alone (gen_sphere_blobs without the rhs)."""
+        lo = np.ascontiguousarray(tree["lo"], dtype=np.float64)
+        dx = np.ascontiguousarray(tree["length"][:, 0] / self.nb, dtype=np.float64)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        _chk(self.lib.sayaka_gen_sphere(self.h, N.ptr(lo), N.ptr(dx), N.ptr(c), radius, ratio, eps, nsub), "gen")
+
+    def gen_velocity(self, tree, spec, seed, zero_walls=True):
+        """Seeded face velocity of the current tree on the device (workloads.face_velocity's
+        definition; 0 on the domain walls when zero_walls)."""
+        lo = np.ascontiguousarray(tree["lo"], dtype=np.float64)
+        dlo = np.zeros(3)
+        root = np.ones(3)
+        for a in range(spec.ndim):
+            dlo[a] = spec.lo[a]
+            root[a] = (spec.hi[a] - spec.lo[a]) / max(spec.ndiv[a], 1)
+        _chk(self.lib.sayaka_gen_velocity(self.h, N.ptr(lo), N.ptr(dlo), N.ptr(root), int(seed),
+                                          int(bool(zero_walls))), "gen_velocity")
+
+    def gen_divergence_rhs(self):
+        """RHS = div(a u) / (dv vf) minus its mean over the fluid leaves, after the coarse/fine
+        face sync of u; returns the removed mean."""
+        mean = f64()
+        _chk(self.lib.sayaka_gen_divergence_rhs(self.h, C.byref(mean)), "gen_divergence_rhs")
+        return mean.value
+
+    def project(self, phi=SOL):
+        """u -= B flux(phi) on the finest tower; (max |div u| before, after), means removed."""
+        b, a = f64(), f64()
+        _chk(self.lib.sayaka_dev_project(self.h, phi, C.byref(b), C.byref(a)), "project")
+        return b.value, a.value
+
+    def get_velocity(self):
+        """Face velocity, a list of ndim arrays [nblk][faces]."""
+        out = np.empty((self.n, self.ndim, self.nface))
+        _chk(self.lib.sayaka_dev_get_velocity(self.h, N.ptr(out)), "get_velocity")
+        return [np.ascontiguousarray(out[:, d]) for d in range(self.ndim)]
 
     def profile(self, kclass, enable=True):
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")

@@ -2,17 +2,22 @@
 regridded around it every step, the previous step's pressure carried to the new tree and used
 as the initial guess of the next solve.
 
-Per step (the regrid / projection cycle of the reference's flow demo, Test7.h:411-418, 612-653):
+Per step (the regrid / projection cycle of the reference's flow demo, Test7.h:411-418, 580-657):
   1. flag + regrid on the host tree (AmrTree::regrid, SayakaRegrid.cpp:26-976): leaves within
      factor * (block edge) of the moved shell are refined, the others coarsened, up to three
      passes per step; the block origins of the passes are composed;
   2. the new tables to the device (sayaka_dev_set_topology);
   3. the pressure to the new numbering (sayaka_dev_remap: survivors moved, new blocks injected
      from their parents -- the reference's packing and initNewBlockDataAfterRegrid);
-  4. cut-cell coefficients of the moved sphere (density ratio 1000) and a Gaussian-blob rhs,
-     generated on the device (sayaka_gen_sphere_blobs);
+  4. cut-cell coefficients of the moved sphere (density ratio 1000), generated on the device
+     (sayaka_gen_sphere), and the rhs: by default the divergence of a seeded face velocity
+     resampled on the new tree (sayaka_gen_velocity + sayaka_gen_divergence_rhs, walls at 0;
+     the same seed every step, so the field only moves with the tree and the sphere), or
+     Gaussian blobs (rhs="blobs", sayaka_gen_sphere_blobs);
   5. the mixed-precision solve to 1e-10 |b|_inf from the carried pressure, and, for comparison,
-     from zero.
+     from zero;
+  6. (projection rhs) the velocity correction u -= B flux(p) with coarse/fine faces
+     re-synchronised (sayaka_dev_project), and the divergence left on the leaves.
 
     python -m sayakaamr_amd.moving --steps 10
 """
@@ -40,7 +45,9 @@ def compose(first, second):
 
 
 def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=1000.0, compare_zero=True,
-        max_blocks=45000, passes=3, log=None):
+        max_blocks=45000, passes=3, rhs="projection", seed=1, log=None):
+    if rhs not in ("projection", "blobs"):
+        raise ValueError(f"rhs must be 'projection' or 'blobs', not {rhs!r}")
     log = log or (lambda m: print(f"[moving] {m}", file=sys.stderr, flush=True))
     tb = AmrTree(spec.ndim, max_blocks=max_blocks)
     tree = tb.build_shell(spec, (x0, 0.5, 0.5), radius, W.SHELL_FACTOR)
@@ -74,7 +81,12 @@ def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=
             if step > 0:
                 dev.remap(origin, nold, [D.SOL])
             t3 = time.perf_counter()
-            dev.gen_sphere_blobs(tree, centre, radius, ratio, 1.5 * dxf, 4, blobs)
+            if rhs == "blobs":
+                dev.gen_sphere_blobs(tree, centre, radius, ratio, 1.5 * dxf, 4, blobs)
+            else:
+                dev.gen_sphere(tree, centre, radius, ratio, 1.5 * dxf, 4)
+                dev.gen_velocity(tree, spec, seed, zero_walls=True)
+                dev.gen_divergence_rhs()
             L = int(tree["level"].max())
             bmax = dev.dot(L, D.RHS, D.RHS)[1]
             dev.sync()
@@ -96,13 +108,16 @@ def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=
             r = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=200, precision=D.MIXED)
             dev.sync()
             t6 = time.perf_counter()
+            if rhs == "projection":
+                div0, div1 = dev.project(D.SOL)
+                row.update(project_ms=(time.perf_counter() - t6) * 1e3, div_before=div0, div_after=div1)
             leaf = tree["node_type"] == 1
             row.update(blocks=int(tree["level"].shape[0]), leaf_cells=int(leaf.sum()) * spec.nb ** spec.ndim,
                        regrid_ms=(t1 - t0) * 1e3, tables_ms=(t2 - t1) * 1e3, remap_ms=(t3 - t2) * 1e3,
                        coef_ms=(t4 - t3) * 1e3, solve_ms=(t6 - t5) * 1e3, iters=r["iters"], status=r["status"],
                        rel_resid=float(r["rnorm"] / bmax))
             rows.append(row)
-            log(json.dumps({k: (round(v, 2) if isinstance(v, float) else v) for k, v in row.items()}))
+            log(json.dumps({k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in row.items()}))
             if r["status"] != 0:
                 raise RuntimeError(f"step {step}: solve status {r['status']}")
     finally:
@@ -118,6 +133,10 @@ def summarize(rows):
         out["zero_guess_solve_ms"] = float(np.mean([r["zero_ms"] for r in s]))
         out["zero_guess_iters"] = float(np.mean([r["zero_iters"] for r in s]))
     out["step_ms"] = out["regrid_ms"] + out["tables_ms"] + out["remap_ms"] + out["coef_ms"] + out["solve_ms"]
+    if s and "project_ms" in s[0]:
+        out["project_ms"] = float(np.mean([r["project_ms"] for r in s]))
+        out["step_ms"] += out["project_ms"]
+        out["max_div_ratio"] = float(max(r["div_after"] / r["div_before"] for r in rows))
     out["steps"] = len(s)
     out["leaf_cells_mean"] = float(np.mean([r["leaf_cells"] for r in s]))
     out["max_rel_resid"] = float(max(r["rel_resid"] for r in s))
@@ -128,8 +147,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--no-zero", action="store_true", help="skip the zero-guess comparison solve")
+    ap.add_argument("--rhs", choices=("projection", "blobs"), default="projection")
     args = ap.parse_args(argv)
-    rows = run(steps=args.steps, compare_zero=not args.no_zero)
+    rows = run(steps=args.steps, compare_zero=not args.no_zero, rhs=args.rhs)
     print(json.dumps(summarize(rows)))
 
 

@@ -1,6 +1,7 @@
 """The moving-boundary cycle on the GPU (sayakaamr_amd.moving): host regrid around a translating
-sphere, tables, device remap of the pressure, device coefficients, mixed solve -- every step
-converges to the absolute target from the carried pressure, the carried pressure equals the
+sphere, tables, device remap of the pressure, device coefficients and rhs, mixed solve, velocity
+correction -- every step converges to the absolute target from the carried pressure and leaves
+a divergence-free velocity, the carried pressure equals the
 previous solution moved by the regrid (checked against the numpy restatement of the remap), and
 the tree stays a valid 2:1-balanced octree that follows the sphere."""
 from __future__ import annotations
@@ -19,13 +20,17 @@ pytestmark = pytest.mark.gpu
 SPEC = W.TreeSpec(ndim=3, nb=8, minlevel=2, maxlevel=4, ndiv=(4, 4, 4), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0))
 
 
-def test_moving_cycle_converges_every_step():
-    rows = MV.run(steps=3, spec=SPEC, radius=0.2, x0=0.45, shift=0.02, max_blocks=8000,
+@pytest.mark.parametrize("rhs", ["projection", "blobs"])
+def test_moving_cycle_converges_every_step(rhs):
+    rows = MV.run(steps=3, spec=SPEC, radius=0.2, x0=0.45, shift=0.02, max_blocks=8000, rhs=rhs,
                   log=lambda m: print(f"[gpu] {m}", flush=True))
     assert len(rows) == 4
     for r in rows:
         assert r["status"] == 0 and r["rel_resid"] <= 1e-10
         assert r["iters"] <= 20
+        if rhs == "projection":
+            # the corrected velocity is divergence-free to the solve's residual
+            assert r["div_after"] <= 1e-8 * r["div_before"]
     assert rows[1]["blocks"] > 0 and "zero_iters" in rows[1]
 
 
