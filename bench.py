@@ -20,8 +20,10 @@ Reported (one JSON line on stdout):
   hbm_regime       the 16^3-block, 139 M-cell snapshot: operator and smoother GB/s where the
                    fields no longer fit in the caches, and its mixed solve;
   moving           the moving-boundary cycle on the same 16^3-block tree: per step host regrid,
-                   tables, device remap of the pressure, device coefficients and the mixed
-                   solve warm-started from the carried pressure (zero-guess solve beside it);
+                   tables, device remap of the pressure, device coefficients and projection rhs
+                   (divergence of a seeded face velocity), the mixed solve warm-started from the
+                   carried pressure (zero-guess solve beside it) and the velocity correction,
+                   with the divergence it leaves;
   roofline         the dominant HBM kernel class of the headline solve, measured with HIP
                    events on the solver's stream inside the timed region; `traffic` from the
                    committed rocprofv3 counter summary (profiles/c1_fp64_smooth_traffic.md);
@@ -483,8 +485,9 @@ def main():
         moving = MV.summarize(MV.run(steps=args.moving_steps, log=lambda m: log(f"moving: {m}")))
         moving = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in moving.items()}
         moving["workload"] = ("16^3 blocks, root 4^3, levels 1-6 around an r=0.15 sphere moving +0.01/step in x "
-                              "from x=0.5, regrid every step (up to 3 passes), density ratio 1000, blob rhs, "
-                              "mixed solve to 1e-10 |b|_inf warm-started from the carried pressure")
+                              "from x=0.5, regrid every step (up to 3 passes), density ratio 1000, rhs = div u of "
+                              "a seeded face velocity (walls 0), mixed solve to 1e-10 |b|_inf warm-started from "
+                              "the carried pressure, u -= B grad p, max |div u| after / before")
         log(f"moving: {moving}")
 
     # V-cycle rate: preconditioner time / V-cycles of the FP64 solve
