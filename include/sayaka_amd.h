@@ -19,6 +19,8 @@
 #ifndef SAYAKA_AMD_H
 #define SAYAKA_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -123,6 +125,35 @@ int sayaka_dev_flux(sayaka_dev_ctx* ctx, int m, int dir, int phi, double* flux_o
  * data and AmrTree::initNewBlockDataAfterRegrid with the default injection
  * (SayakaRegrid.cpp:530-591, 987-1105). */
 int sayaka_dev_remap(sayaka_dev_ctx* ctx, const int* origin, int nold, const int* fields, int nfields);
+
+/* Device memory of the context's GPU for caller-held data such as a state with ghost layers
+ * (no reference counterpart: the reference's TreeData lives in host memory).  memcpy copies
+ * host -> device when to_device is non-zero, device -> host otherwise, and returns when done. */
+int sayaka_dev_alloc(sayaka_dev_ctx* ctx, size_t bytes, void** out);
+int sayaka_dev_free(sayaka_dev_ctx* ctx, void* ptr);
+int sayaka_dev_memcpy(sayaka_dev_ctx* ctx, void* dst, const void* src, size_t bytes, int to_device);
+
+/* The tree's 3^d surrounding blocks for the general state fill, after every
+ * sayaka_dev_set_topology: surr27[b][27] = SurroundingBlocks::blocks and bcref27[b][27] =
+ * SurroundingBlocks::bcrefs of block b, both in the reference's [ii+1][jj+1][kk+1] order
+ * (AmrTree::surrBlocks, SayakaTree.h:336-526, built by SayakaRegrid.cpp:655-945; block id >= 0
+ * same level, -1 coarser, <= -20 physical code). */
+int sayaka_dev_set_surroundings(sayaka_dev_ctx* ctx, int nblk, const int* surr27, const int* bcref27);
+
+/* FillPatch::fillBoundary(scomp, nc, nlayer) (SayakaFillPatch.cpp:17-368) on a cell state held
+ * on the device (state: device pointer, [ncomp][blocks_alloc][grown box], the block grown by
+ * ngrow layers, i fastest, 2D blocks not grown in z): levels coarsest first, every block; same-
+ * level surroundings (faces, edges, corners) copied, coarse ones prolonged from the parent
+ * (prolong[c] per filled component: 0 injection, 1 centred linear, 2 MC-limited centred linear;
+ * InterpPatch::ProlongMethod, SayakaInterpPatch.cpp:157-391), physical ones filled by the BC of
+ * BCPatch::fillBasicCellBlockBC (SayakaBoundaryPatch.cpp:139-229) over all ngrow layers.  The
+ * BC register of component scomp+c maps physical code physbc[h] at surrounding q to
+ * (mathbc, bcval)[(c*27 + q)*nphys + h] (BCRegister, SayakaBoundaryPatch.h:62-140; math BC 0
+ * Neumann, 1 Dirichlet, 2 SimpleFill).  Returns -2 when a physical code of the tree has no
+ * mapping (the reference exits).  Needs sayaka_dev_set_surroundings for the current tree. */
+int sayaka_dev_state_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blocks_alloc, int ngrow, int scomp,
+                          int nc, int nlayer, const int* prolong, int nphys, const int* physbc, const int* mathbc,
+                          const double* bcval);
 
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
  * the SOL field as initial guess; SOL receives the solution on convergence. */

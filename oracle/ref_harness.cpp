@@ -474,6 +474,38 @@ void ref_mac_flux(int m, int dir, const double* phi_valid, double* flux_faces) {
     delete phi;
     TreeData::ReleaseDataPArray(fl);
 }
+// FillPatch::fillBoundary (SayakaFillPatch.cpp:17-30) on an ncomp-component cell state with
+// ngrow ghost layers.  data: [comp][block][grown box], in and out.  Per component: a
+// prolongation method (InterpPatch::ProlongMethod, SayakaInterpPatch.h) and a BC register
+// mapping each of the nphys physical codes to (math BC, value) on every surrounding position
+// (mathbc / bcval: [comp][nphys]).
+void ref_fill_state(int ncomp, int ngrow, double* data, const int* prolong, int nphys, const int* physbc,
+                    const int* mathbc, const double* bcval, int scomp, int nc, int nlayer) {
+    const int n = g_tree->numBlocks;
+    TreeData* st = TreeData::CreateCellData(*g_tree, ncomp, ngrow);
+    const IndexBox gb = grown_cell_box(ngrow);
+    const size_t per = (size_t)n * gb.stride();
+    for (int c = 0; c < ncomp; c++) copy_in_box(*st, c, gb, data + c * per, n);
+    BCPatch bc;
+    bc.defineByData(*st);
+    for (int c = 0; c < ncomp; c++)
+        for (int q = 0; q < SurroundingIndex::NumSurr; q++) {
+            SurroundingIndex s(q);
+            for (int t = 0; t < nphys; t++) {
+                bc.boundaryRegister(c).setBCMap(s.ipos(), s.jpos(), s.kpos(), physbc[t], mathbc[c * nphys + t]);
+                bc.boundaryRegister(c).setBCVal(s.ipos(), s.jpos(), s.kpos(), physbc[t], bcval[c * nphys + t]);
+            }
+        }
+    FillPatch fp(*g_tree, *st);
+    fp.setBC(bc);
+    for (int c = 0; c < ncomp; c++) fp.setProlongation(c, prolong[c]);
+    quiet_begin();
+    fp.fillBoundary(scomp, nc, nlayer);
+    quiet_end();
+    for (int c = 0; c < ncomp; c++) copy_out_box(*st, c, gb, data + c * per, n);
+    delete st;
+}
+
 void ref_restrict(int c, int comp) { g_mac->mg_restict_wgt(c, g_mac->getDataBuf(), comp); }
 void ref_prolong(int l, int comp) { g_mac->mg_prolong_wgt(l, g_mac->getDataBuf(), comp); }
 double ref_dot(int m, int a, int b) {

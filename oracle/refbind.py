@@ -73,6 +73,7 @@ def load(ndim: int):
         "ref_state_set": ([_p], None),
         "ref_state_get": ([_p], None),
         "ref_tree_init_new_data": ([], None),
+        "ref_fill_state": ([_i, _i, _p, _p, _i, _p, _p, _p, _i, _i, _i], None),
         "ref_restrict": ([_i, _i], None),
         "ref_prolong": ([_i, _i], None),
         "ref_dot": ([_i, _i, _i], _d),
@@ -134,6 +135,18 @@ class RefTree:
 
     def init_new_data(self):
         self.lib.ref_tree_init_new_data()
+
+    # FillPatch::fillBoundary on a [comp][block][grown] state (a fresh copy is returned)
+    def fill_state(self, data, ngrow, prolong, physbc, mathbc, bcval, scomp, nc, nlayer):
+        d = np.array(data, dtype=np.float64, order="C", copy=True)
+        ncomp = d.shape[0]
+        pr = np.ascontiguousarray(prolong, dtype=np.int32)
+        pb = np.ascontiguousarray(physbc, dtype=np.int32)
+        mb = np.ascontiguousarray(mathbc, dtype=np.int32).reshape(ncomp, pb.size)
+        bv = np.ascontiguousarray(bcval, dtype=np.float64).reshape(ncomp, pb.size)
+        self.lib.ref_fill_state(ncomp, ngrow, _ptr(d), _ptr(pr), pb.size, _ptr(pb), _ptr(mb), _ptr(bv),
+                                scomp, nc, nlayer)
+        return d
 
     def export(self):
         n = self.lib.ref_tree_numblocks()

@@ -23,6 +23,7 @@
 #include "flux.h"
 #include "remap.h"
 #include "projection.h"
+#include "statefill.h"
 
 using namespace sydev;
 
@@ -130,6 +131,15 @@ struct sayaka_dev_ctx {
     double* d_vel = nullptr;
     double* d_div = nullptr;
     int vel_valid = 0;             // d_vel generated for the current tree
+    // general state fill (statefill.h): the tree's surrounding tables [blk][27] (host copy kept
+    // to check BC maps) and the BC tables of the last call
+    int* d_surr = nullptr;
+    int* d_bcref = nullptr;
+    std::vector<int> h_surr;
+    int surr_n = 0;
+    int* d_fill_int = nullptr;     // physical codes, then [nc][27][nphys] math BC types
+    double* d_fill_val = nullptr;  // [nc][27][nphys] BC values
+    size_t fill_icap = 0, fill_vcap = 0;
 };
 
 namespace {
@@ -707,6 +717,19 @@ struct Impl {
         return 0;
     }
 
+    // general ghost fill of a grown cell state (statefill.h): one launch per tree level,
+    // coarsest first (FillPatch::fillBoundary, SayakaFillPatch.cpp:17-30)
+    static int state_fill(sayaka_dev_ctx* c, FillArgs a, size_t lds_bytes) {
+        for (int l = 1; l <= c->maxlevel; l++) {
+            const ListRef lst = c->lv_all[l];
+            if (lst.n <= 0) continue;
+            a.list = dlist(c, lst);
+            hipLaunchKernelGGL((k_state_fill<D, NB>), dim3(lst.n), dim3(FILL_NT), lds_bytes, c->stream, a);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // seeded face velocity of every block into d_vel (projection.h); d_lo = [nblk][3] corners
     static int gen_velocity(sayaka_dev_ctx* c, const double* d_lo, const double* dlo, const double* root,
                             unsigned long long seed, int zero_walls) {
@@ -993,6 +1016,10 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_remap_idx) hipFree(c->d_remap_idx);
     if (c->d_vel) hipFree(c->d_vel);
     if (c->d_div) hipFree(c->d_div);
+    if (c->d_surr) hipFree(c->d_surr);
+    if (c->d_bcref) hipFree(c->d_bcref);
+    if (c->d_fill_int) hipFree(c->d_fill_int);
+    if (c->d_fill_val) hipFree(c->d_fill_val);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
@@ -1275,6 +1302,134 @@ int sayaka_dev_restrict(sayaka_dev_ctx* c, int cl, int field) {
 int sayaka_dev_prolong(sayaka_dev_ctx* c, int fl, int field) {
     if (!c || !field_ok(field)) return fail(-1, "bad argument");
     int e = dispatch(c, [&](auto I) { return decltype(I)::prolong(c, fl, c->d_field[field]); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_alloc(sayaka_dev_ctx* c, size_t bytes, void** out) {
+    if (!c || !out || bytes == 0) return fail(-1, "bad argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(out, bytes));
+    return 0;
+}
+
+int sayaka_dev_free(sayaka_dev_ctx* c, void* p) {
+    if (!c) return fail(-1, "bad argument");
+    if (!p) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(p));
+    return 0;
+}
+
+int sayaka_dev_memcpy(sayaka_dev_ctx* c, void* dst, const void* src, size_t bytes, int to_device) {
+    if (!c || !dst || !src) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_set_surroundings(sayaka_dev_ctx* c, int nblk, const int* surr27, const int* bcref27) {
+    if (!c || !surr27 || !bcref27) return fail(-1, "bad argument");
+    if (nblk != c->nblk) return fail(-1, "block count differs from the topology");
+    for (int e = 0; e < nblk * 27; e++) {
+        const int v = surr27[e];
+        if (v >= nblk || bcref27[e] < -1 || bcref27[e] >= 27) return fail(-1, "surrounding table out of range");
+    }
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->d_surr) {
+        HIPCHK(hipMalloc(&c->d_surr, (size_t)c->cap * 27 * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_bcref, (size_t)c->cap * 27 * sizeof(int)));
+    }
+    c->h_surr.assign(surr27, surr27 + (size_t)nblk * 27);
+    HIPCHK(hipMemcpyAsync(c->d_surr, surr27, (size_t)nblk * 27 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bcref, bcref27, (size_t)nblk * 27 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->surr_n = nblk;
+    return 0;
+}
+
+int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks_alloc, int ngrow, int scomp, int nc,
+                          int nlayer, const int* prolong, int nphys, const int* physbc, const int* mathbc,
+                          const double* bcval) {
+    if (!c || !state || ncomp < 1 || ngrow < 1 || scomp < 0 || nc < 0 || scomp + nc > ncomp)
+        return fail(-1, "bad argument");
+    if (nlayer <= 0 || nc == 0) return 0;   // the reference returns at once for nlayer <= 0
+    if (nlayer > ngrow) return fail(-1, "nlayer > ngrow");
+    if (nc > FILL_MAXCOMP) return fail(-1, "more than 16 components in one fill");
+    if (c->surr_n != c->nblk || c->nblk == 0) return fail(-1, "surrounding tables not set for this topology");
+    if (blocks_alloc < c->nblk) return fail(-1, "blocks_alloc below the tree's block count");
+    if (nphys < 0 || (nphys > 0 && (!physbc || !mathbc || !bcval)) || !prolong) return fail(-1, "bad argument");
+    const int nz = c->ndim == 3 ? c->nb : 1, gz = c->ndim == 3 ? ngrow : 0;
+    const long long gx = c->nb + 2 * ngrow;
+    const long long ntile = gx * gx * (nz + 2 * gz);
+    const size_t lds = (size_t)ntile * sizeof(double);
+    if (lds > 64 * 1024) return fail(-1, "grown block above 64 KiB (ngrow too large for this block size)");
+    for (int q = 0; q < nc; q++) {
+        if (prolong[q] < 0 || prolong[q] > 2) return fail(-1, "unknown prolongation method");
+        // centred slopes read one coarse cell beyond the mapped one (the reference exits,
+        // SayakaInterpPatch.cpp:233-236)
+        if (prolong[q] > 0 && ngrow < 2) return fail(-1, "centred-linear prolongation needs ngrow >= 2");
+    }
+    // every (surrounding, physical code) the tree holds needs a BC for every component
+    // (the reference exits on an unknown mapping, SayakaBoundaryPatch.h:101-110)
+    std::vector<char> seen((size_t)27 * (nphys + 1), 0);
+    for (int b = 0; b < c->nblk; b++)
+        for (int q = 0; q < 27; q++) {
+            const int v = c->h_surr[(size_t)b * 27 + q];
+            if (v > -20) continue;
+            int h = -1;
+            for (int t = 0; t < nphys; t++)
+                if (physbc[t] == v) h = t;
+            if (h < 0) return fail(-2, "no BC mapping for physical code " + std::to_string(v));
+            if (seen[(size_t)q * (nphys + 1) + h]) continue;
+            seen[(size_t)q * (nphys + 1) + h] = 1;
+            for (int cc = 0; cc < nc; cc++) {
+                const int t = mathbc[((size_t)cc * 27 + q) * nphys + h];
+                if (t < 0 || t > 2)
+                    return fail(-2, "no BC mapping for physical code " + std::to_string(v) + " at surrounding " +
+                                        std::to_string(q) + " of component " + std::to_string(scomp + cc));
+            }
+        }
+    HIPCHK(hipSetDevice(c->device));
+    const size_t ni = (size_t)nphys + (size_t)nc * 27 * nphys, nv = (size_t)nc * 27 * nphys;
+    if (ni > c->fill_icap) {
+        if (c->d_fill_int) HIPCHK(hipFree(c->d_fill_int));
+        HIPCHK(hipMalloc(&c->d_fill_int, (ni + 1) * sizeof(int)));
+        c->fill_icap = ni;
+    }
+    if (nv > c->fill_vcap) {
+        if (c->d_fill_val) HIPCHK(hipFree(c->d_fill_val));
+        HIPCHK(hipMalloc(&c->d_fill_val, (nv + 1) * sizeof(double)));
+        c->fill_vcap = nv;
+    }
+    if (nphys > 0) {
+        std::vector<int> hi(ni);
+        std::copy(physbc, physbc + nphys, hi.begin());
+        std::copy(mathbc, mathbc + nv, hi.begin() + nphys);
+        HIPCHK(hipMemcpyAsync(c->d_fill_int, hi.data(), ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_fill_val, bcval, nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // the host vector goes out of scope
+    }
+    FillArgs a{};
+    a.binfo = c->d_binfo;
+    a.surr = c->d_surr;
+    a.bcref = c->d_bcref;
+    a.physcode = c->d_fill_int;
+    a.bctype = c->d_fill_int + nphys;
+    a.bcval = c->d_fill_val;
+    a.state = state;
+    a.comp_stride = (long long)blocks_alloc * ntile;
+    a.ngrow = ngrow;
+    a.nlayer = nlayer;
+    a.scomp = scomp;
+    a.nc = nc;
+    a.nphys = nphys;
+    for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
+    int e = dispatch(c, [&](auto I) { return decltype(I)::state_fill(c, a, lds); });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;

@@ -59,6 +59,11 @@ _SIG = {
     "sayaka_dev_flux": ([vp, i32, i32, i32, vp], i32),
     "sayaka_dev_remap": ([vp, vp, i32, vp, i32], i32),
     "sayaka_dev_copy": ([vp, i32, i32], i32),
+    "sayaka_dev_set_surroundings": ([vp, i32, vp, vp], i32),
+    "sayaka_dev_alloc": ([vp, C.c_size_t, C.POINTER(vp)], i32),
+    "sayaka_dev_free": ([vp, vp], i32),
+    "sayaka_dev_memcpy": ([vp, vp, vp, C.c_size_t, i32], i32),
+    "sayaka_dev_state_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -111,6 +116,40 @@ def _parr(arrs):
     keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
     pa = (vp * 3)(*([None if a is None else a.ctypes.data for a in keep] + [None] * (3 - len(keep))))
     return keep, pa
+
+
+class DeviceState:
+    """A cell state with ghost layers in device memory, [ncomp][blocks][grown box] doubles (the
+    device counterpart of a grown TreeData, SayakaTreeData.h:86-306)."""
+
+    def __init__(self, solver, ncomp, ngrow, blocks):
+        self.s, self.ncomp, self.ngrow, self.blocks = solver, ncomp, ngrow, blocks
+        nb, nd = solver.nb, solver.ndim
+        self.shape = (ncomp, blocks) + ((nb + 2 * ngrow,) * 3 if nd == 3 else (1, nb + 2 * ngrow, nb + 2 * ngrow))
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = vp()
+        _chk(solver.lib.sayaka_dev_alloc(solver.h, self.nbytes, C.byref(p)), "alloc")
+        self.ptr = p.value
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64).reshape(self.shape)
+        _chk(self.s.lib.sayaka_dev_memcpy(self.s.h, C.c_void_p(self.ptr), a.ctypes.data_as(vp), self.nbytes, 1),
+             "memcpy")
+
+    def download(self):
+        out = np.empty(self.shape)
+        _chk(self.s.lib.sayaka_dev_memcpy(self.s.h, out.ctypes.data_as(vp), C.c_void_p(self.ptr), self.nbytes, 0),
+             "memcpy")
+        return out
+
+    def fill(self, nlayer, prolong, bcmap, scomp=0, nc=None):
+        """FillPatch::fillBoundary(scomp, nc, nlayer) on this state."""
+        self.s.state_fill(self.ptr, self.ncomp, self.blocks, self.ngrow, nlayer, prolong, bcmap, scomp, nc)
+
+    def close(self):
+        if self.ptr and getattr(self.s, "h", None):
+            self.s.lib.sayaka_dev_free(self.s.h, C.c_void_p(self.ptr))
+        self.ptr = None
 
 
 class MacSolverGPU:
@@ -217,6 +256,34 @@ class MacSolverGPU:
         o = np.ascontiguousarray(origin, dtype=np.int32)
         f = np.ascontiguousarray(fields, dtype=np.int32)
         _chk(self.lib.sayaka_dev_remap(self.h, N.ptr(o), int(nold), N.ptr(f), f.size), "remap")
+
+    # --- FillPatch::fillBoundary on a device-resident cell state ------------------------
+    def set_surroundings(self, tree: dict):
+        """The tree's SurroundingBlocks tables (after prepare, for the current tree)."""
+        s = np.ascontiguousarray(tree["surr"], dtype=np.int32)
+        r = np.ascontiguousarray(tree["surr_bcref"], dtype=np.int32)
+        _chk(self.lib.sayaka_dev_set_surroundings(self.h, self.n, N.ptr(s), N.ptr(r)), "set_surroundings")
+
+    def state_fill(self, state_ptr, ncomp, blocks_alloc, ngrow, nlayer, prolong, bcmap, scomp=0, nc=None):
+        """Ghost fill of the device state at state_ptr ([ncomp][blocks_alloc][grown box] doubles,
+        e.g. DeviceState.ptr).
+        prolong[c]: method of component scomp+c (0 injection, 1 centred linear, 2 limited);
+        bcmap[c]: {physical code: (math BC, value)} of component scomp+c, the same on every
+        surrounding position (BCRegister filled position by position the way the drivers do)."""
+        nc = ncomp - scomp if nc is None else nc
+        codes = sorted({int(k) for m in bcmap[:nc] for k in m})
+        nphys = len(codes)
+        mb = np.full((nc, 27, max(nphys, 1)), -1, np.int32)
+        bv = np.zeros((nc, 27, max(nphys, 1)))
+        for c in range(nc):
+            for h, code in enumerate(codes):
+                if code in bcmap[c]:
+                    mb[c, :, h], bv[c, :, h] = bcmap[c][code]
+        pr = np.ascontiguousarray(prolong[:nc], dtype=np.int32)
+        pc = np.array(codes if codes else [0], dtype=np.int32)
+        _chk(self.lib.sayaka_dev_state_fill(self.h, C.c_void_p(int(state_ptr)), ncomp, blocks_alloc, ngrow, scomp,
+                                            nc, nlayer, N.ptr(pr), nphys, N.ptr(pc), N.ptr(mb), N.ptr(bv)),
+             "state_fill")
 
     def flux(self, m, d, phi):
         """MacSolver::mg_flux_level: face flux of direction d on tower m from field phi,

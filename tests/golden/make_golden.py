@@ -8,6 +8,7 @@ against on machines where the reference is absent.
     python tests/golden/make_golden.py         (operators, solves, regrid, C1 summary)
     python tests/golden/make_golden.py flux    (face fluxes: tests/golden/{c0,s3}_flux.npz)
     python tests/golden/make_golden.py remap   (regrid data movement: regrid_remap{2,3}.npz)
+    python tests/golden/make_golden.py fill    (general state fill: fill{2,3}.npz)
 """
 from __future__ import annotations
 
@@ -242,6 +243,30 @@ def regrid_remap(ndim):
     print(f"regrid_remap{ndim}: 6 steps")
 
 
+def state_fill(name):
+    """FillPatch::fillBoundary of the reference on the case's tree (tests/golden_cases.py
+    FILL_CASES): the tree tables and, per nlayer, the ghost cells of every component after the
+    fill (the inputs are rebuilt from their hash by golden_cases.fill_input)."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import golden_cases as GC
+    spec = GC.fill_spec(name)
+    nd, nb = spec.ndim, spec.nb
+    rt = R.RefTree(nd)
+    tree = W.build_tree(rt, spec, GC.fill_criterion(name))
+    n = tree["level"].shape[0]
+    data = GC.fill_input(name, n)
+    comps = GC.FILL_COMPS[name]
+    ncomp = data.shape[0]
+    mask = GC.ghost_mask(nd, nb, GC.FILL_NGROW)
+    out = tree_arrays("tree_", tree)
+    for nlayer in GC.FILL_CASES[name][5]:
+        res = rt.fill_state(data.reshape(ncomp, n, -1), GC.FILL_NGROW, comps["prolong"], GC.FILL_WALLS,
+                            comps["mathbc"], comps["bcval"], 0, ncomp, nlayer).reshape(data.shape)
+        out[f"ghosts_nlayer{nlayer}"] = res[:, :, mask]
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: {n} blocks, {ncomp} components")
+
+
 def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     """MacSolver::mg_flux_level on the tower of every MG level and every direction, for the
     seeded input of case_ops (the same tree, coefficients and op_x) and for the case's own
@@ -277,6 +302,10 @@ def case_flux(name, spec, crit, inp_fn, bc, seed=7):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "fill":
+        state_fill("fill2")
+        state_fill("fill3")
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "remap":
         regrid_remap(3)
         regrid_remap(2)
