@@ -71,7 +71,7 @@ struct BotCfg {
     static constexpr int CPT = WCELL / NT;                             // cells per thread
     static constexpr int WGHOST = BPW * G::NGHOST;
     static constexpr int GPT = (WGHOST + NT - 1) / NT;                 // ghost slots per thread
-    static constexpr int NRED = 2 * BPW * (G::NCELL + 1);              // two sums staged
+    static constexpr int NRED = 2 * BPW * (G::NCELL + 2);              // two sums staged
     static constexpr int NTIL = BPW * G::NTILE;
     // face weights of the workgroup's blocks staged in LDS when they fit beside the rest
     static constexpr bool WLDS = BPW * D * G::NFACE * 8 <= 64 * 1024;
@@ -165,72 +165,71 @@ struct Bottom {
     // Per-block sums of the staged terms (nsum of them) for the blocks of this workgroup,
     // published with the workgroup's max; a grid barrier; then every workgroup reads all
     // partials and forms the totals (in block order when exact).  a0/a1 are the thread's
-    // running sums in fast mode.
+    // running sums in fast mode.  Both sides of the barrier run in wave 0 alone after one
+    // workgroup barrier, so the other waves only wait once on each side.
     __device__ static void reduce(cooperative_groups::grid_group& grid, const BottomArgs& A, double a0, double a1,
                                   int nsum, double mxl, double* part, double* red, double* wl, int nvalid,
                                   double& s0, double& s1, double& mx) {
         const int w0 = blockIdx.x * BPW;
         const int nl = A.st.nlist;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         mxl = wave_max(mxl);
-        if ((threadIdx.x & 63) == 0) wl[threadIdx.x >> 6] = mxl;
-        if (A.exact) {
-            __syncthreads();
-            const int l = threadIdx.x;
-            if (l < BPW * nsum) {
-                const double sv = lane_chain<G::NCELL>(red, l);
-                const int blk = l % BPW;
-                if (blk < nvalid) part[PS * (w0 + blk) + l / BPW] = sv;
-            }
-        } else {
+        if (lane == 0) wl[wv] = mxl;
+        if (!A.exact) {
             a0 = wave_sum(a0);
             a1 = wave_sum(a1);
-            if ((threadIdx.x & 63) == 0) {
-                wl[16 + (threadIdx.x >> 6)] = a0;
-                wl[32 + (threadIdx.x >> 6)] = a1;
+            if (lane == 0) {
+                wl[16 + wv] = a0;
+                wl[32 + wv] = a1;
             }
-            __syncthreads();
-            if (threadIdx.x < BPW && (int)threadIdx.x < nvalid) {
+        }
+        __syncthreads();
+        if (wv == 0) {
+            if (A.exact) {
+                // ordered per-block chains, one lane per block and sum
+                if (lane < BPW * nsum) {
+                    const double sv = lane_chain<G::NCELL>(red, lane);
+                    const int blk = lane % BPW;
+                    if (blk < nvalid) part[PS * (w0 + blk) + lane / BPW] = sv;
+                }
+            } else if (lane < BPW && lane < nvalid) {
                 double b0 = 0.0, b1 = 0.0;
-                if (threadIdx.x == 0)
+                if (lane == 0)
                     for (int w = 0; w < NT / 64; w++) {
                         b0 += wl[16 + w];
                         b1 += wl[32 + w];
                     }
-                part[PS * (w0 + threadIdx.x)] = b0;
-                part[PS * (w0 + threadIdx.x) + 1] = b1;
+                part[PS * (w0 + lane)] = b0;
+                part[PS * (w0 + lane) + 1] = b1;
             }
-        }
-        __syncthreads();
-        if (threadIdx.x < BPW && (int)threadIdx.x < nvalid) {
-            double m = 0.0;
-            for (int w = 0; w < NT / 64; w++) m = fmax(m, wl[w]);
-            part[PS * (w0 + threadIdx.x) + 2] = m;
+            if (lane < BPW && lane < nvalid) {
+                double m = 0.0;
+                for (int w = 0; w < NT / 64; w++) m = fmax(m, wl[w]);
+                part[PS * (w0 + lane) + 2] = m;
+            }
         }
         grid.sync();
-        // totals: stage the partials in LDS (parallel loads; the max reduced on the way),
-        // chain the sums in lanes 0 / 1 (every workgroup, same order, same result)
-        double acc = 0.0, tm = 0.0;
-        for (int base = 0; base < nl; base += NT) {
-            const int cnt = nl - base < NT ? nl - base : NT;
-            const int cnt8 = (cnt + 7) & ~7;
-            for (int g = threadIdx.x; g < cnt8; g += NT) {
-                const bool ok = g < cnt;
-                red[g] = ok ? load_published(part + PS * (base + g)) : 0.0;
-                red[NT + g] = ok && nsum > 1 ? load_published(part + PS * (base + g) + 1) : 0.0;
-                if (ok) tm = fmax(tm, load_published(part + PS * (base + g) + 2));
+        // totals (wave 0): the partials staged in LDS by lane in chunks of 64, the max reduced
+        // across the wave, the sums chained in block order in lanes 0 / 1 -- every workgroup
+        // forms the same totals in the same order
+        if (wv == 0) {
+            double acc = 0.0, tm = 0.0;
+            for (int base = 0; base < nl; base += 64) {
+                const int cnt = nl - base < 64 ? nl - base : 64;
+                const int cnt16 = (cnt + 15) & ~15;   // zero padding: acc starts at +0 and stays off -0
+                if (lane < cnt16) {
+                    const bool ok = lane < cnt;
+                    red[lane] = ok ? load_published(part + PS * (base + lane)) : 0.0;
+                    red[64 + lane] = ok && nsum > 1 ? load_published(part + PS * (base + lane) + 1) : 0.0;
+                    if (ok) tm = fmax(tm, load_published(part + PS * (base + lane) + 2));
+                }
+                wave_lds_barrier();
+                if (lane < (nsum > 1 ? 2 : 1)) acc = chain_n(red + lane * 64, cnt16, acc);
+                wave_lds_barrier();
             }
-            __syncthreads();
-            if ((int)threadIdx.x < (nsum > 1 ? 2 : 1)) acc = chain_n(red + threadIdx.x * NT, cnt8, acc);
-            __syncthreads();
-        }
-        tm = wave_max(tm);
-        if ((threadIdx.x & 63) == 0) wl[52 + (threadIdx.x >> 6)] = tm;
-        if (threadIdx.x < 2) wl[48 + threadIdx.x] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double m = 0.0;
-            for (int w = 0; w < NT / 64; w++) m = fmax(m, wl[52 + w]);
-            wl[50] = m;
+            tm = wave_max(tm);
+            if (lane < 2) wl[48 + lane] = acc;
+            if (lane == 0) wl[50] = tm;
         }
         __syncthreads();
         s0 = wl[48];
@@ -249,7 +248,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
     constexpr int PS = B::PS;
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
-    __shared__ double sh[PC ? C::NSHARED_FAST : C::NSHARED];   // operand tiles, then reduction staging
+    __shared__ __attribute__((aligned(16))) double sh[PC ? C::NSHARED_FAST : C::NSHARED];   // operand tiles, then reduction staging
     __shared__ double wl[64];
     __shared__ BlockInfo bi[BPW];
     __shared__ double dvs[BPW];

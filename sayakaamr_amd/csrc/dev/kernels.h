@@ -86,46 +86,83 @@ struct RedBuf {
 };
 
 // Lane l of the calling wave sums the N values staged for chain l, in index order.  Chain
-// l occupies red[l*(N+1) .. l*(N+1)+N) (the one-value pad puts concurrent chains on
-// distinct LDS banks).  A sequential FP64 chain is bound by the dependent-add latency, so
-// the next 8 values are loaded while the current 8 are added.
-template <int N>
+// l occupies red[l*(N+2) .. l*(N+2)+N): the stride keeps every chain 16-byte aligned for
+// 128-bit LDS reads and puts concurrent chains on distinct LDS banks.  A sequential FP64 chain
+// is bound by the dependent-add latency (~10 cycles on MI355X); the chain is unrolled with two
+// windows of 2*W values in flight so that the LDS reads hide behind the adds (measured 11-13
+// cycles per add with W = 8, against 22 for a rolled loop with register rotation).  W = 4
+// keeps the register footprint of kernels that chain only at their end small.
+template <int N, int W = 8>
 __device__ inline double lane_chain(const double* red, int l) {
-    static_assert(N % 16 == 0, "chain length must be a multiple of 16");
-    const double* p = red + l * (N + 1);
+    static_assert(N % (4 * W) == 0, "chain length must be a multiple of 4W");
+    const double2* p = reinterpret_cast<const double2*>(red + l * (N + 2));
     double s = 0.0;
-    double a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5], a6 = p[6], a7 = p[7];
-#pragma unroll 1
-    for (int c = 0; c < N; c += 16) {
-        const double* q = p + c;
-        const double b0 = q[8], b1 = q[9], b2 = q[10], b3 = q[11], b4 = q[12], b5 = q[13], b6 = q[14], b7 = q[15];
-        s += a0; s += a1; s += a2; s += a3; s += a4; s += a5; s += a6; s += a7;
-        if (c + 16 < N) {
-            a0 = q[16]; a1 = q[17]; a2 = q[18]; a3 = q[19]; a4 = q[20]; a5 = q[21]; a6 = q[22]; a7 = q[23];
+    double2 A[W], B[W];
+#pragma unroll
+    for (int u = 0; u < W; u++) A[u] = p[u];
+    // W = 8: full unroll up to 512 values (the 8^3 block); otherwise a rolled outer loop (no
+    // register rotation either way: A and B are reloaded in place)
+#pragma unroll((W == 8 && N <= 512) ? N / (4 * W) : 1)
+    for (int c = 0; c < N / 2; c += 2 * W) {
+#pragma unroll
+        for (int u = 0; u < W; u++) B[u] = p[c + W + u];
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            s += A[u].x;
+            s += A[u].y;
         }
-        s += b0; s += b1; s += b2; s += b3; s += b4; s += b5; s += b6; s += b7;
+        if (c + 2 * W < N / 2) {
+#pragma unroll
+            for (int u = 0; u < W; u++) A[u] = p[c + 2 * W + u];
+        }
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            s += B[u].x;
+            s += B[u].y;
+        }
     }
     return s;
 }
 
-// Sequential sum of n (a multiple of 8) contiguous LDS values onto acc, loads one batch ahead.
+// Sequential sum of n (a multiple of 16) contiguous 16-byte aligned LDS values onto acc, two
+// windows of 8 values in flight (the same scheme as lane_chain, for a run-time length).
 __device__ inline double chain_n(const double* p, int n, double acc) {
     if (n <= 0) return acc;
-    double a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5], a6 = p[6], a7 = p[7];
+    const double2* q = reinterpret_cast<const double2*>(p);
+    double2 A[4], B[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) A[u] = q[u];
 #pragma unroll 1
-    for (int c = 0; c < n; c += 8) {
-        const double* q = p + c;
-        double b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0, b6 = 0, b7 = 0;
-        if (c + 8 < n) {
-            b0 = q[8]; b1 = q[9]; b2 = q[10]; b3 = q[11]; b4 = q[12]; b5 = q[13]; b6 = q[14]; b7 = q[15];
+    for (int c = 0; c < n / 2; c += 8) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) B[u] = q[c + 4 + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            acc += A[u].x;
+            acc += A[u].y;
         }
-        acc += a0; acc += a1; acc += a2; acc += a3; acc += a4; acc += a5; acc += a6; acc += a7;
-        a0 = b0; a1 = b1; a2 = b2; a3 = b3; a4 = b4; a5 = b5; a6 = b6; a7 = b7;
+        if (c + 8 < n / 2) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) A[u] = q[c + 8 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            acc += B[u].x;
+            acc += B[u].y;
+        }
     }
     return acc;
 }
 
-__device__ inline void stage_chain(double* red, int n, int l, int c, double v) { red[l * (n + 1) + c] = v; }
+// LDS written by lanes of this wave becomes visible to the wave's other lanes (no workgroup
+// barrier: the other waves are not taking part).
+__device__ inline void wave_lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline void stage_chain(double* red, int n, int l, int c, double v) { red[l * (n + 2) + c] = v; }
 
 // Sums, in cell order, of nsum (1 or 2) per-cell values held by the threads (cell
 // c = tid + q*NT); the two chains run concurrently in lanes 0 and 1.  `scratch` is LDS of
@@ -144,7 +181,7 @@ __device__ inline void seq_block_sums(const double (&v0)[CPT], const double (&v1
     }
     __syncthreads();
     double r = 0.0;
-    if ((int)threadIdx.x < nsum) r = lane_chain<NCELL>(scratch, threadIdx.x);
+    if ((int)threadIdx.x < nsum) r = lane_chain<NCELL, 4>(scratch, threadIdx.x);
     __syncthreads();
     if (threadIdx.x == 1) scratch[0] = r;
     __syncthreads();
@@ -164,7 +201,7 @@ __device__ inline double load_published(const double* p) {
 template <int NT>
 __device__ inline void sum_published(const double* part, unsigned n, bool exact, int nsum, double& a, double& b,
                                      double& m, double* chunk /* 2*NT doubles of LDS */, double* lds) {
-    static_assert(NT % 8 == 0, "chunk must be a multiple of 8");
+    static_assert(NT % 16 == 0, "chunk must be a multiple of 16");
     a = 0.0;
     b = 0.0;
     m = 0.0;
@@ -172,7 +209,7 @@ __device__ inline void sum_published(const double* part, unsigned n, bool exact,
         double ml = 0.0, acc = 0.0;
         for (unsigned base = 0; base < n; base += NT) {
             const unsigned cnt = n - base < (unsigned)NT ? n - base : (unsigned)NT;
-            const unsigned cnt8 = (cnt + 7u) & ~7u;
+            const unsigned cnt8 = (cnt + 15u) & ~15u;   // zero padding (acc starts at +0)
             const unsigned g = threadIdx.x;
             if (g < cnt8) {
                 const bool ok = g < cnt;
@@ -214,7 +251,7 @@ __device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf
 // previous launch summed (in list order when exact) and the Krylov scalars updated.
 constexpr int RED_NT = 512;
 __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsum, int fin) {
-    __shared__ double chunk[2 * RED_NT];
+    __shared__ __attribute__((aligned(16))) double chunk[2 * RED_NT];
     __shared__ double lds[3 * (RED_NT / 64)];
     double a, b, m;
     sum_published<RED_NT>(rb.partials, (unsigned)n, rb.exact != 0, nsum, a, b, m, chunk, lds);
@@ -235,7 +272,7 @@ __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsu
 }
 
 // Block totals of nsum per-cell sums (q0, q1) and a max, stored as the workgroup's
-// partials.  `scratch` is LDS of at least 2*NCELL+2 doubles.
+// partials.  `scratch` is 16-byte aligned LDS of at least 2*NCELL+4 doubles.
 template <int NT, int NCELL, int CPT>
 __device__ inline void grid_reduce_cells(const double (&q0)[CPT], const double (&q1)[CPT], double mx, int nsum,
                                          const RedBuf& rb, double* scratch, double* lds) {
@@ -484,7 +521,7 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
     constexpr bool REDUCE = MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT;
     constexpr bool SMOOTHING = MODE == ST_SMOOTH || MODE == ST_SMOOTH_DB;
     static_assert(!REDUCE || sizeof(T) == 8, "fused reductions are FP64 only");
-    __shared__ T tile[G::NTILE > 2 * G::NCELL + 2 ? G::NTILE : 2 * G::NCELL + 2];
+    __shared__ __attribute__((aligned(16))) T tile[G::NTILE > 2 * G::NCELL + 4 ? G::NTILE : 2 * G::NCELL + 4];
     __shared__ double lds[3 * ((NT + 63) / 64)];
     __shared__ int biw[BI_WORDS];
     const int b = a.list[blockIdx.x];
@@ -658,7 +695,7 @@ template <int NCELL, int NT>
 __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fin) {
     constexpr int CPT = (NCELL + NT - 1) / NT;
     __shared__ double lds[3 * ((NT + 63) / 64)];
-    __shared__ double scratch[2 * NCELL + 2];
+    __shared__ __attribute__((aligned(16))) double scratch[2 * NCELL + 4];
     const int b = a.list[blockIdx.x];
     double q0[CPT], q1[CPT], mx = 0.0;
     const double* S = a.scal;
