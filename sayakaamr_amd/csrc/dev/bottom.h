@@ -50,7 +50,10 @@ struct BottomArgs {
     double* vpub;          // published v  [blk][NCELL]
     double* rpub;          // published r
     double* partials;      // 3 buffers x 3 values x nlist
-    int* stats;            // [0] calls, [1] total iterations, [2] last iterations, [3] last status
+    int* stats;            // [0] calls, [1] total iterations, [2] last iterations, [3] last status,
+                           // [4] grid-barrier time-outs
+    unsigned long long* flags;     // per workgroup: the last barrier episode it reached
+    unsigned long long episode0;   // episodes of this launch are above every earlier launch's
     double tol_rel, tol_abs;
     int max_iter;
     int exact;             // reference-order sums (see RedBuf::exact)
@@ -61,6 +64,7 @@ struct BottomArgs {
 };
 
 constexpr int PC_MAXBLK = 64;   // level-1 blocks the two-level preconditioner supports
+constexpr unsigned BARRIER_SPIN_MAX = 1u << 18;   // polls before a grid barrier gives up (~0.2 s)
 
 template <int D, int NB>
 struct BotCfg {
@@ -162,12 +166,47 @@ struct Bottom {
         }
     }
 
+    // Grid barrier of the cooperative launch as a flag gather.  The memory operations are those
+    // of grid.sync (workgroup barrier; thread 0: agent-scope release, then the arrival; wait;
+    // agent-scope acquire; workgroup barrier), but each workgroup raises its own flag to the
+    // episode number and wave 0 waits on every flag at once, one lane per workgroup: no
+    // argument loads and no returning atomic on the critical path.  The wait is bounded: a
+    // workgroup that times out counts it in stats[4], and waits no more in this launch
+    // (*broken, in LDS), so every wave ends soon; the host reports the error.
+    __device__ static void grid_barrier(const BottomArgs& A, unsigned long long ep, int nwg, int* broken) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(A.flags + blockIdx.x, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (threadIdx.x < 64) {
+            int late = 0;
+            const int skip = *broken;
+            for (int w = threadIdx.x; w < nwg && !skip; w += 64) {
+                unsigned spins = 0;
+                while (__hip_atomic_load(A.flags + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ep) {
+                    if (++spins > BARRIER_SPIN_MAX) {
+                        late = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (late) {
+                atomicAdd(A.stats + 4, 1);
+                *broken = 1;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    }
+
     // Per-block sums of the staged terms (nsum of them) for the blocks of this workgroup,
     // published with the workgroup's max; a grid barrier; then every workgroup reads all
     // partials and forms the totals (in block order when exact).  a0/a1 are the thread's
     // running sums in fast mode.  Both sides of the barrier run in wave 0 alone after one
     // workgroup barrier, so the other waves only wait once on each side.
-    __device__ static void reduce(cooperative_groups::grid_group& grid, const BottomArgs& A, double a0, double a1,
+    __device__ static void reduce(unsigned long long& ep, int* broken, const BottomArgs& A, double a0, double a1,
                                   int nsum, double mxl, double* part, double* red, double* wl, int nvalid,
                                   double& s0, double& s1, double& mx) {
         const int w0 = blockIdx.x * BPW;
@@ -208,7 +247,7 @@ struct Bottom {
                 part[PS * (w0 + lane) + 2] = m;
             }
         }
-        grid.sync();
+        grid_barrier(A, ++ep, (nl + BPW - 1) / BPW, broken);
         // totals (wave 0): the partials staged in LDS by lane in chunks of 64, the max reduced
         // across the wave, the sums chained in block order in lanes 0 / 1 -- every workgroup
         // forms the same totals in the same order
@@ -246,8 +285,9 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
     using B = Bottom<D, NB, PC>;
     constexpr int BPW = C::BPW, NT = C::NT, CPT = C::CPT, GPT = C::GPT;
     constexpr int PS = B::PS;
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
+    unsigned long long ep = A.episode0;   // barrier episodes of this launch
+    __shared__ int s_broken;              // a grid barrier of this workgroup timed out
+    if (threadIdx.x == 0) s_broken = 0;   // visible after the first __syncthreads below
     __shared__ __attribute__((aligned(16))) double sh[PC ? C::NSHARED_FAST : C::NSHARED];   // operand tiles, then reduction staging
     __shared__ double wl[64];
     __shared__ BlockInfo bi[BPW];
@@ -400,7 +440,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
         block_terms(r);
         B::publish_block_sums(bq, partC, wq, w0, nvalid);
     }
-    B::reduce(grid, A, a0, a1, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
+    B::reduce(ep, &s_broken, A, a0, a1, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
     if constexpr (PC) B::read_block_sums(partC, sr_, nl);
     const double rnorm0 = mx;
     double rho = s0v, rho1 = 0.0, alpha = 0.0, omega = 0.0;
@@ -518,7 +558,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 block_terms(v);
                 B::publish_block_sums(bq, partA, wq, w0, nvalid);
             }
-            B::reduce(grid, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx);
+            B::reduce(ep, &s_broken, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx);
             if constexpr (PC) B::read_block_sums(partA, sv_, nl);
             load_ghosts(vG, A.vpub);
             if (s0v == 0.0) { stat = 1; break; }
@@ -555,7 +595,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 B::stage(sh, ex, 1, blk_of(q), cell_of(q), t[q] * t[q], a1);
             }
             double ts, tt;
-            B::reduce(grid, A, a0, a1, 2, mxl, partB, sh, wl, nvalid, ts, tt, mx);
+            B::reduce(ep, &s_broken, A, a0, a1, 2, mxl, partB, sh, wl, nvalid, ts, tt, mx);
             if (mx < rnorm0 * A.tol_rel || mx < A.tol_abs) { conv = 1; break; }
             if (tt == 0.0) { stat = 1; break; }
             omega = ts / tt;
@@ -577,7 +617,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 block_terms(r);
                 B::publish_block_sums(bq, partC, wq, w0, nvalid);
             }
-            B::reduce(grid, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
+            B::reduce(ep, &s_broken, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
             if constexpr (PC) B::read_block_sums(partC, sr_, nl);
             load_ghosts(rG, A.rpub);
             if (mx < rnorm0 * A.tol_rel + A.tol_abs) { conv = 1; break; }

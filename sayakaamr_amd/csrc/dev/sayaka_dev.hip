@@ -83,6 +83,8 @@ struct sayaka_dev_ctx {
     double* d_bot_rpub = nullptr;
     double* d_bot_part = nullptr;
     int* d_bot_stats = nullptr;
+    unsigned long long* d_bot_flags = nullptr;   // grid-barrier flags of the bottom solve, one per workgroup
+    unsigned long long bot_launches = 0;
     int* h_bot_stats = nullptr;  // pinned
 
     MixedState mixed;
@@ -358,6 +360,8 @@ struct Impl {
         A.rpub = c->d_bot_rpub;
         A.partials = c->d_bot_part;
         A.stats = c->d_bot_stats;
+        A.flags = c->d_bot_flags;
+        A.episode0 = (++c->bot_launches) << 20;   // at most 1 + 3 * 2000 barriers per launch
         A.tol_rel = tol_rel;
         A.tol_abs = tol_abs;
         A.max_iter = max_iter;
@@ -814,7 +818,7 @@ struct Impl {
         c->exact = P->precision == SAYAKA_FP64 ? 1 : 0;
         int precond = P->precond;
         if (precond && L == 1) precond = 0;
-        HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 4 * sizeof(int), c->stream));
+        HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 8 * sizeof(int), c->stream));
         st->status = 2;
         st->iters = 0;
         st->half_step = 0;
@@ -924,8 +928,9 @@ struct Impl {
             a.dst = F[SAYAKA_SOL];
             if ((e = launch_vec<D, NB>(c, T, V_FINAL, a, FIN_NONE))) return e;
         }
-        HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_bot_stats[4]) return fail(-5, "coarsest-level solve: a grid barrier timed out");
         st->bottom_calls = c->h_bot_stats[0];
         st->bottom_iters = c->h_bot_stats[1];
         st->status = conv ? 0 : (stat ? 1 : 2);
@@ -991,8 +996,10 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_jac, (size_t)PC_MAXBLK * c->ncell * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_fsum, (size_t)PC_MAXBLK * 6 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_s0inv, (size_t)PC_MAXBLK * PC_MAXBLK * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 4 * sizeof(int));
-    if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 4 * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 8 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->d_bot_flags, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->d_bot_flags, 0, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 8 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_scal, 0, S_NSCALAR * sizeof(double), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_wcf, 0, (size_t)max_blocks * c->nf * c->nslab * sizeof(double), c->stream);
     for (int f = 0; f < SAYAKA_NFIELD && e == hipSuccess; f++) e = hipMemsetAsync(c->d_field[f], 0, cells * sizeof(double), c->stream);
@@ -1021,7 +1028,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_fill_int) hipFree(c->d_fill_int);
     if (c->d_fill_val) hipFree(c->d_fill_val);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
-                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_binfo, c->d_met, c->d_lists,
+                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_bot_flags, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < SAYAKA_KCLASS_N; k++)
@@ -1497,13 +1504,14 @@ int sayaka_dev_bottom(sayaka_dev_ctx* c, int sol, int rhs, double tol_rel, doubl
                       int* status) {
     if (!c || !field_ok(sol) || !field_ok(rhs) || max_iter < 1) return fail(-1, "bad argument");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 4 * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_bot_stats, 0, 8 * sizeof(int), c->stream));
     int e = dispatch(c, [&](auto I) {
         return decltype(I)::bottom(c, c->d_field[sol], c->d_field[rhs], tol_rel, tol_abs, max_iter);
     });
     if (e) return e;
-    HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->h_bot_stats[4]) return fail(-5, "coarsest-level solve: a grid barrier timed out");
     if (iters) *iters = c->h_bot_stats[2];
     if (status) *status = c->h_bot_stats[3];
     return 0;
