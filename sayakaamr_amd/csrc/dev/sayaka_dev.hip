@@ -7,6 +7,7 @@
 //   tower T(m) = leaves of levels < m and every block of level m, ordered m..1
 //   (MGLevelTower::update, SayakaTower.cpp:28-68), with one sub-list per level.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdio>
 #include <cstring>
@@ -170,19 +171,29 @@ struct ProfScope {
     int k;
     long long cells;
     double bytes;
-    hipEvent_t a = nullptr;
-    ProfScope(sayaka_dev_ctx* c_, int k_, long long cells_, double bytes_)
-        : c(c_), k(k_), cells(cells_), bytes(bytes_) {
+    bool kernel_timed;
+    hipEvent_t a = nullptr, b = nullptr;
+    // kernel_timed: the launch records the two events itself (hipExtLaunchKernel's start / stop
+    // events, the dispatch's own timestamps, which profilers read too); otherwise the scope
+    // records them around whatever it brackets (a marker pair adds a few microseconds)
+    ProfScope(sayaka_dev_ctx* c_, int k_, long long cells_, double bytes_, bool kernel_timed_ = false)
+        : c(c_), k(k_), cells(cells_), bytes(bytes_), kernel_timed(kernel_timed_) {
         if (c->prof_on[k]) {
             a = pool_event(c);
-            if (a) hipEventRecord(a, c->stream);
+            if (a && kernel_timed) b = pool_event(c);
+            if (a && !kernel_timed) hipEventRecord(a, c->stream);
         }
     }
+    hipEvent_t start() const { return kernel_timed ? a : nullptr; }
+    hipEvent_t stop() const { return kernel_timed ? b : nullptr; }
     ~ProfScope() {
         if (!a) return;
-        hipEvent_t b = pool_event(c);
+        if (!kernel_timed) {
+            b = pool_event(c);
+            if (!b) return;
+            hipEventRecord(b, c->stream);
+        }
         if (!b) return;
-        hipEventRecord(b, c->stream);
         c->prof[k].push_back({a, b, cells, bytes});
     }
 };
@@ -250,11 +261,14 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
     dim3 grid(a.nlist), block(G::THREADS);
     RedBuf rb = redbuf(c);
     ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL,
-                 a.nlist * AlgBytes<D, NB, double>::stencil(mode));
+                 a.nlist * AlgBytes<D, NB, double>::stencil(mode), mode == ST_SMOOTH);
     switch (mode) {
         case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID, double>), grid, block, 0, c->stream, a, rb, fin); break;
-        case ST_SMOOTH: hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, a, rb, fin); break;
+        case ST_SMOOTH:
+            hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, ps.start(), ps.stop(),
+                                  0, a, rb, fin);
+            break;
         case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_RESID_INIT: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID_INIT, double>), grid, block, 0, c->stream, a, rb, fin); break;
@@ -564,9 +578,9 @@ struct Impl {
                 a.rhs = f;
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL,
-                             T.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH));
-                hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0, c->stream,
-                                   a, RedBuf{}, FIN_NONE);
+                             T.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
+                hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0,
+                                      c->stream, ps.start(), ps.stop(), 0, a, RedBuf{}, FIN_NONE);
             }
             HIPCHK(hipGetLastError());
             return 0;
@@ -580,9 +594,9 @@ struct Impl {
                 a.rhs = f;
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL,
-                             sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH));
-                hipLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0, c->stream, a,
-                                   RedBuf{}, FIN_NONE);
+                             sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
+                hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0,
+                                      c->stream, ps.start(), ps.stop(), 0, a, RedBuf{}, FIN_NONE);
             }
         }
         HIPCHK(hipGetLastError());
