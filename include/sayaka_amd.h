@@ -155,6 +155,20 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blo
                           int nc, int nlayer, const int* prolong, int nphys, const int* physbc, const int* mathbc,
                           const double* bcval);
 
+/* FillPatch::fillBoundary(scomp, nc, nlayer) on face data of direction fdir (0 x, 1 y, 2 z),
+ * held on the device as [ncomp][blocks_alloc][grown face box]: the face box of a block has
+ * nb+1 faces along fdir and nb across, grown by ngrow on every side (2D not in z), i fastest.
+ * Same-level surroundings copied over the staggered ranges (where two ranges share a face the
+ * surrounding later in the reference's loop order wins; FillPatch::cacheRanges,
+ * SayakaFillPatch.cpp:620-700), coarse ones prolonged one after the other over their non-
+ * overlapping ranges (prolongFaceBlockData_Injection / _CenteredLinear with or without the MC
+ * limiter, SayakaInterpPatch.cpp:58-85, 447-591).  Valid faces are never written; the result
+ * is the reference's for face data whose copies of a face shared by two blocks agree.  Returns
+ * -2 on a tree with physical boundaries: the reference fills no face data next to a wall
+ * (SayakaFillPatch.cpp:318-324).  Needs sayaka_dev_set_surroundings for the current tree. */
+int sayaka_dev_face_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blocks_alloc, int fdir, int ngrow,
+                         int scomp, int nc, int nlayer, const int* prolong);
+
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
  * the SOL field as initial guess; SOL receives the solution on convergence. */
 int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);

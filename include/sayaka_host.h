@@ -21,7 +21,9 @@ typedef struct sy_tree sy_tree;
 typedef struct sy_mac sy_mac;
 
 /* AmrTree::init + initUniformRootLevel + initRefineToMinLevel (SayakaTree.cpp:34-233).
- * bc[6]: neighbour code of each wall (<= -20).  Returns NULL on bad arguments. */
+ * bc[6]: neighbour code of each wall (<= -20), or SY_TREE_PERIODIC for a periodic face
+ * (initUniformRootLevel's is_periodic, SayakaTree.cpp:93-201).  Returns NULL on bad arguments. */
+#define SY_TREE_PERIODIC 1
 sy_tree* sy_tree_create(int ndim, int nb, int ngrow, int minlevel, int maxlevel, int max_blocks,
                         const int* ndiv, const double* lo, const double* hi, const int* bc);
 void sy_tree_destroy(sy_tree* t);

@@ -17,9 +17,22 @@ What it restates (file:line in the reference checkout's src/):
   BCPatch::fillBasicCellBlockBC     SayakaBoundaryPatch.cpp:139-229 (reflection over the
                                      data's full ngrow layers, whatever nlayer is)
   IndexMapping::FineToCrse / CrseToFine  SayakaTreeData.h:34-44
+Face data (FaceFill):
+  FillPatch::cacheRanges            SayakaFillPatch.cpp:660-700 (staggered same-level ranges,
+                                     non-overlapping coarse ranges)
+  InterpPatch::prolongFaceBlockData_Injection / _CenteredLinear (with or without limiter)
+      SayakaInterpPatch.cpp:58-85, 447-591
+  fillBlockBndryCond exits for face data next to a wall (SayakaFillPatch.cpp:318-324): the
+  restatement covers periodic trees only and raises otherwise.
+
+Pinning: both classes reproduce the reference's own fillBoundary output bit for bit
+(tests/golden/fill{2,3}.npz, fface{2,3}.npz, written by the reference compiled from its
+sources, oracle/_ref); the cell prolongations also reproduce linear fields and constants
+(tests/test_fill_oracle.py).
 
 Layout: state[comp] is [block][gz][gy][gx] over the block grown by ngrow (gz = 1 in 2D);
-cell (i, j, k) of the reference's index space sits at [k+g][j+g][i+g].
+cell (i, j, k) of the reference's index space sits at [k+g][j+g][i+g].  Face data of
+direction fd has one more entry along fd (faces 0..nb of the valid box).
 """
 from __future__ import annotations
 
@@ -279,3 +292,125 @@ class StateFill:
                         for sweep, sweepdir, ii, jj, kk in order:
                             if self.surr(b, ii, jj, kk) <= PHYS_BNDRY:
                                 self._bc_one(arr, b, c, sweepdir, ii, jj, kk, bcmap)
+
+
+class FaceFill(StateFill):
+    """FillPatch::fillBoundary on face data of direction fdir (staggered in fdir): the
+    reference's same-level copy over staggered ranges (they overlap on shared faces, the
+    surrounding later in loop order wins, SayakaFillPatch.cpp:125-188, 660-700), coarse
+    prolongation over the non-overlapping ranges with prolongFaceBlockData_Injection /
+    _CenteredLinear (SayakaInterpPatch.cpp:447-591, one surrounding after the other: the middle
+    faces of the second sweep can read faces outside the box), and no physical BC: the reference
+    exits for face data next to a wall (SayakaFillPatch.cpp:318-324), so the tree is periodic.
+
+    state[comp]: [block][gz][gy][gx] over the face box grown by ngrow (one more face in fdir);
+    face (i, j, k) sits at [k+gz][j+g][i+g]."""
+
+    def __init__(self, tree, nb, ndim, ngrow, maxlevel, fdir):
+        super().__init__(tree, nb, ndim, ngrow, maxlevel)
+        self.fd = fdir
+
+    def _stagger(self, box):
+        lo, hi = list(box[0]), list(box[1])
+        hi[self.fd] += 1
+        return lo, hi
+
+    def ranges(self, ii, jj, kk, nlayer):
+        dst, src = super().ranges(ii, jj, kk, nlayer)
+        return self._stagger(dst), self._stagger(src)
+
+    def nonoverlap(self, ii, jj, kk, nlayer):
+        dst, _ = self.ranges(ii, jj, kk, nlayer)
+        lo, hi = dst
+        o = (ii, jj, kk)[self.fd]
+        if o == -1:
+            hi[self.fd] -= 1
+        elif o == 1:
+            lo[self.fd] += 1
+        return lo, hi
+
+    def _prolong_face(self, arr, b, p, box, off, method):
+        lo, hi = box
+        nd, fd = self.nd, self.fd
+        e = [0, 0, 0]
+        e[fd] = 1
+        ftc = lambda v, d: fine_to_crse(v, off[d]) if d < nd else v
+        if method == PROLONG_INJECTION:
+            for k in range(lo[2], hi[2] + 1):
+                for j in range(lo[1], hi[1] + 1):
+                    for i in range(lo[0], hi[0] + 1):
+                        ic, jc, kc = ftc(i, 0), ftc(j, 1), ftc(k, 2)
+                        side = ((i, j, k)[fd] + LARGE) % 2
+                        vl = self._at(arr, p, ic, jc, kc)
+                        if side == 0:
+                            v = vl
+                        else:
+                            v = 0.5 * (vl + self._at(arr, p, ic + e[0], jc + e[1], kc + e[2]))
+                        self._set(arr, b, i, j, k, v)
+            return
+        clo = [ftc(lo[d], d) for d in range(3)]
+        chi = [ftc(hi[d], d) for d in range(3)]
+        rng = [(-1, 1) if (d != fd and d < nd) else (0,) for d in range(3)]
+        for kc in range(clo[2], chi[2] + 1):
+            for jc in range(clo[1], chi[1] + 1):
+                for ic in range(clo[0], chi[0] + 1):
+                    c = self._at(arr, p, ic, jc, kc)
+                    grad = [0.0, 0.0, 0.0]
+                    for d in range(nd):
+                        if d == fd:
+                            continue
+                        u = [0, 0, 0]
+                        u[d] = 1
+                        dn = self._at(arr, p, ic - u[0], jc - u[1], kc - u[2])
+                        up = self._at(arr, p, ic + u[0], jc + u[1], kc + u[2])
+                        grad[d] = limited_slope(dn, c, up) if method == PROLONG_CENTERED_LINEAR_LIMITED \
+                            else 0.5 * (up - dn)
+                    cc = (ic, jc, kc)
+                    for kind in rng[2]:
+                        k = crse_to_fine(kc, (kind + 1) // 2, off[2]) if nd == 3 else kc
+                        if k < lo[2] or k > hi[2]:
+                            continue
+                        krel = 0.25 * float(kind)
+                        for jind in rng[1]:
+                            j = crse_to_fine(jc, (jind + 1) // 2, off[1])
+                            if j < lo[1] or j > hi[1]:
+                                continue
+                            jrel = 0.25 * float(jind)
+                            for iind in rng[0]:
+                                i = crse_to_fine(ic, (iind + 1) // 2, off[0])
+                                if i < lo[0] or i > hi[0]:
+                                    continue
+                                irel = 0.25 * float(iind)
+                                self._set(arr, b, i, j, k, c + irel * grad[0] + jrel * grad[1] + krel * grad[2])
+        # second sweep: faces between two coarse faces, the average of their two fine neighbours
+        for k in range(lo[2], hi[2] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    if ((i, j, k)[fd] + LARGE) % 2 == 1:
+                        vl = self._at(arr, b, i - e[0], j - e[1], k - e[2])
+                        vr = self._at(arr, b, i + e[0], j + e[1], k + e[2])
+                        self._set(arr, b, i, j, k, 0.5 * (vl + vr))
+
+    def fill(self, state, scomp, nc, nlayer, prolong, bcmap=None):
+        if nlayer <= 0:
+            return
+        t = self.t
+        for level in range(1, self.maxlevel + 1):
+            for b in np.nonzero(t["level"] == level)[0]:
+                b = int(b)
+                if any(self.surr(b, *s) <= PHYS_BNDRY for s in self._surr_iter()):
+                    raise ValueError("face data next to a wall (the reference exits)")
+                for c in range(scomp, scomp + nc):
+                    arr = state[c]
+                    for ii, jj, kk in self._surr_iter():
+                        nbk = self.surr(b, ii, jj, kk)
+                        if nbk >= 0:
+                            dst, src = self.ranges(ii, jj, kk, nlayer)
+                            self._view(arr, b, dst)[...] = self._view(arr, nbk, src)
+                    coarse = [s for s in self._surr_iter() if PHYS_BNDRY < self.surr(b, *s) < 0]
+                    if coarse:
+                        p = int(t["parent"][b])
+                        wc = int(t["which_child"][b])
+                        off = [((wc >> d) & 1) * (self.nb // 2) if d < self.nd else 0 for d in range(3)]
+                        for ii, jj, kk in coarse:
+                            self._prolong_face(arr, b, p, self.nonoverlap(ii, jj, kk, nlayer), off, prolong[c])

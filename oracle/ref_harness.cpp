@@ -126,7 +126,8 @@ int ref_tree_create(int nb, int ngrow, int minlevel, int maxlevel, const int* nd
     RealBox pb(Vector3d::VecMake(plo[0], plo[1], plo[2]), Vector3d::VecMake(phi[0], phi[1], phi[2]));
     int bct[MAX_FACE];
     bool per[MAX_FACE];
-    for (int f = 0; f < MAX_FACE; f++) { bct[f] = bc[f]; per[f] = false; }
+    // bc[f] == 1 marks a periodic face (initUniformRootLevel's is_periodic), otherwise the wall code
+    for (int f = 0; f < MAX_FACE; f++) { bct[f] = bc[f] == 1 ? -100 : bc[f]; per[f] = bc[f] == 1; }
     g_tree->initUniformRootLevel(Vector3i::VecMake(ndiv[0], ndiv[1], ndiv[2]), pb, bct, per);
     g_tree->initRefineToMinLevel();
     quiet_end();
@@ -498,6 +499,27 @@ void ref_fill_state(int ncomp, int ngrow, double* data, const int* prolong, int 
         }
     FillPatch fp(*g_tree, *st);
     fp.setBC(bc);
+    for (int c = 0; c < ncomp; c++) fp.setProlongation(c, prolong[c]);
+    quiet_begin();
+    fp.fillBoundary(scomp, nc, nlayer);
+    quiet_end();
+    for (int c = 0; c < ncomp; c++) copy_out_box(*st, c, gb, data + c * per, n);
+    delete st;
+}
+
+// FillPatch::fillBoundary on an ncomp-component face state of direction dir (staggered in dir)
+// with ngrow ghost layers: data [comp][block][grown face box], in and out; prolong per component.
+// No BCPatch: the reference fills face data only on domains without walls (fillBlockBndryCond
+// exits for face data next to a wall), so the tree must be periodic.
+void ref_fill_face_state(int dir, int ncomp, int ngrow, double* data, const int* prolong, int scomp, int nc,
+                         int nlayer) {
+    const int n = g_tree->numBlocks;
+    TreeData* st = TreeData::CreateFaceData(*g_tree, dir, ncomp, ngrow);
+    IndexBox gb = grown_cell_box(ngrow);
+    gb.staggerInDir(dir);
+    const size_t per = (size_t)n * gb.stride();
+    for (int c = 0; c < ncomp; c++) copy_in_box(*st, c, gb, data + c * per, n);
+    FillPatch fp(*g_tree, *st);
     for (int c = 0; c < ncomp; c++) fp.setProlongation(c, prolong[c]);
     quiet_begin();
     fp.fillBoundary(scomp, nc, nlayer);

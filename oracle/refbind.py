@@ -74,6 +74,7 @@ def load(ndim: int):
         "ref_state_get": ([_p], None),
         "ref_tree_init_new_data": ([], None),
         "ref_fill_state": ([_i, _i, _p, _p, _i, _p, _p, _p, _i, _i, _i], None),
+        "ref_fill_face_state": ([_i, _i, _i, _p, _p, _i, _i, _i], None),
         "ref_restrict": ([_i, _i], None),
         "ref_prolong": ([_i, _i], None),
         "ref_dot": ([_i, _i, _i], _d),
@@ -146,6 +147,13 @@ class RefTree:
         bv = np.ascontiguousarray(bcval, dtype=np.float64).reshape(ncomp, pb.size)
         self.lib.ref_fill_state(ncomp, ngrow, _ptr(d), _ptr(pr), pb.size, _ptr(pb), _ptr(mb), _ptr(bv),
                                 scomp, nc, nlayer)
+        return d
+
+    # FillPatch::fillBoundary on a face state of direction dir, [comp][block][grown face box]
+    def fill_face_state(self, dir, data, ngrow, prolong, scomp, nc, nlayer):
+        d = np.array(data, dtype=np.float64, order="C", copy=True)
+        pr = np.ascontiguousarray(prolong, dtype=np.int32)
+        self.lib.ref_fill_face_state(dir, d.shape[0], ngrow, _ptr(d), _ptr(pr), scomp, nc, nlayer)
         return d
 
     def export(self):

@@ -748,6 +748,18 @@ struct Impl {
         return 0;
     }
 
+    // the same fill on face data of one direction (statefill.h, k_face_fill)
+    static int face_fill(sayaka_dev_ctx* c, FaceFillArgs a, size_t lds_bytes) {
+        for (int l = 1; l <= c->maxlevel; l++) {
+            const ListRef lst = c->lv_all[l];
+            if (lst.n <= 0) continue;
+            a.list = dlist(c, lst);
+            hipLaunchKernelGGL((k_face_fill<D, NB>), dim3(lst.n), dim3(FILL_NT), lds_bytes, c->stream, a);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // seeded face velocity of every block into d_vel (projection.h); d_lo = [nblk][3] corners
     static int gen_velocity(sayaka_dev_ctx* c, const double* d_lo, const double* dlo, const double* root,
                             unsigned long long seed, int zero_walls) {
@@ -1451,6 +1463,52 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int block
     a.nphys = nphys;
     for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
     int e = dispatch(c, [&](auto I) { return decltype(I)::state_fill(c, a, lds); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_face_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks_alloc, int fdir, int ngrow,
+                         int scomp, int nc, int nlayer, const int* prolong) {
+    if (!c || !state || ncomp < 1 || ngrow < 1 || scomp < 0 || nc < 0 || scomp + nc > ncomp || !prolong)
+        return fail(-1, "bad argument");
+    if (fdir < 0 || fdir >= c->ndim) return fail(-1, "face direction beyond the tree's dimension");
+    if (nlayer <= 0 || nc == 0) return 0;   // the reference returns at once for nlayer <= 0
+    if (nlayer > ngrow) return fail(-1, "nlayer > ngrow");
+    if (nc > FILL_MAXCOMP) return fail(-1, "more than 16 components in one fill");
+    if (c->surr_n != c->nblk || c->nblk == 0) return fail(-1, "surrounding tables not set for this topology");
+    if (blocks_alloc < c->nblk) return fail(-1, "blocks_alloc below the tree's block count");
+    const int nz = c->ndim == 3 ? c->nb : 1, gz = c->ndim == 3 ? ngrow : 0;
+    const long long e0 = c->nb + 2 * ngrow + (fdir == 0), e1 = c->nb + 2 * ngrow + (fdir == 1);
+    const long long ntile = e0 * e1 * (nz + 2 * gz + (fdir == 2));
+    const size_t lds = (size_t)ntile * sizeof(double);
+    if (lds > 64 * 1024) return fail(-1, "grown face box above 64 KiB (ngrow too large for this block size)");
+    for (int q = 0; q < nc; q++) {
+        if (prolong[q] < 0 || prolong[q] > 2) return fail(-1, "unknown prolongation method");
+        // tangential slopes read one coarse face beyond the mapped one; the mid-face average of
+        // an odd outermost layer reads one face beyond the grown box
+        if (prolong[q] > 0 && ngrow < 2) return fail(-1, "centred-linear prolongation needs ngrow >= 2");
+        if (prolong[q] > 0 && (nlayer & 1) && nlayer == ngrow)
+            return fail(-1, "centred-linear face prolongation of an odd nlayer needs nlayer < ngrow");
+    }
+    // the reference fills no face data next to a wall (SayakaFillPatch.cpp:318-324)
+    for (int b = 0; b < c->nblk; b++)
+        for (int q = 0; q < 27; q++)
+            if (c->h_surr[(size_t)b * 27 + q] <= -20)
+                return fail(-2, "face data fill on a tree with physical boundaries (periodic trees only)");
+    HIPCHK(hipSetDevice(c->device));
+    FaceFillArgs a{};
+    a.binfo = c->d_binfo;
+    a.surr = c->d_surr;
+    a.state = state;
+    a.comp_stride = (long long)blocks_alloc * ntile;
+    a.fd = fdir;
+    a.ngrow = ngrow;
+    a.nlayer = nlayer;
+    a.scomp = scomp;
+    a.nc = nc;
+    for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
+    int e = dispatch(c, [&](auto I) { return decltype(I)::face_fill(c, a, lds); });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;

@@ -198,4 +198,169 @@ __global__ void __launch_bounds__(FILL_NT) k_state_fill(FillArgs a) {
     }
 }
 
+// ----------------------------------------------------------------------- face data
+// The same fill on face data of direction fd (staggered in fd: one more face than cells along
+// it), [comp][block][grown face box].  The reference fills face data only where no wall is near
+// (fillBlockBndryCond exits for face data next to a wall, SayakaFillPatch.cpp:318-324): the host
+// refuses trees with walls.
+//   1. same-level surroundings over the staggered ranges (SayakaFillPatch.cpp:660-675).  These
+//      overlap on the faces shared by two surroundings and include the block's own boundary
+//      faces (fd index 0 and nb), which the reference overwrites with the neighbour's copy; the
+//      surrounding later in the reference's loop order wins, i.e. the one with the larger fd
+//      offset.  The reference fills the blocks of a level one after another, so where the two
+//      copies of a shared face differ its result depends on its block order; for face data whose
+//      shared faces agree (a synchronised face field) every order gives this result.
+//   2. coarse surroundings one after another in the reference's order, over the ranges that
+//      leave the block's own faces alone (dstNonOverlapRanges, SayakaFillPatch.cpp:677-700):
+//      prolongFaceBlockData_Injection / _CenteredLinear (SayakaInterpPatch.cpp:447-591).  The
+//      centred-linear form sets the fine faces on coarse faces first (tangential slopes), then
+//      each face between two coarse faces to the mean of its fd neighbours, which can lie outside
+//      the range; hence one surrounding at a time, with a barrier between the sweeps.
+// Only ghost faces are written back.
+struct FaceFillArgs {
+    const int* list;
+    const BlockInfo* binfo;
+    const int* surr;        // [blk][27]
+    double* state;          // component 0, block 0
+    long long comp_stride;
+    int fd, ngrow, nlayer, scomp, nc;
+    int prolong[FILL_MAXCOMP];
+};
+
+template <int D, int NB>
+__global__ void __launch_bounds__(FILL_NT) k_face_fill(FaceFillArgs a) {
+    extern __shared__ double tile[];
+    __shared__ int s[27];
+    const int b = a.list[blockIdx.x];
+    const int g = a.ngrow, gz = D == 3 ? g : 0, fd = a.fd;
+    constexpr int NZ = D == 3 ? NB : 1;
+    const int N3[3] = {NB, NB, NZ};
+    const int V[3] = {NB + (fd == 0), NB + (fd == 1), NZ + (fd == 2)};   // valid face extents
+    const int E0 = V[0] + 2 * g, E1 = V[1] + 2 * g, E2 = V[2] + 2 * gz;
+    const int ntile = E0 * E1 * E2;
+    if (threadIdx.x < 27) s[threadIdx.x] = a.surr[b * 27 + threadIdx.x];
+    const int p = a.binfo[b].parent, wc = a.binfo[b].which_child;
+    const int off[3] = {side_of(wc, 0) * (NB / 2), side_of(wc, 1) * (NB / 2), D == 3 ? side_of(wc, 2) * (NB / 2) : 0};
+    const int nl = a.nlayer;
+    auto tix = [&](int i, int j, int k) { return ((k + gz) * E1 + (j + g)) * E0 + (i + g); };
+    __syncthreads();
+    for (int cc = 0; cc < a.nc; cc++) {
+        double* base = a.state + (a.scomp + cc) * a.comp_stride;
+        double* blk = base + (long long)b * ntile;
+        for (int e = threadIdx.x; e < ntile; e += FILL_NT) tile[e] = blk[e];
+        __syncthreads();
+        // 1. same-level copy
+        for (int e = threadIdx.x; e < ntile; e += FILL_NT) {
+            const int ijk[3] = {e % E0 - g, (e / E0) % E1 - g, e / (E0 * E1) - gz};
+            int o[3], o2 = -2;   // o2: the fd offset of a second candidate (lower precedence)
+            bool in_layer = true;
+            for (int d = 0; d < 3; d++) {
+                const int x = ijk[d], n = N3[d];
+                if (d >= D) {
+                    o[d] = 0;
+                } else if (d != fd) {
+                    o[d] = x < 0 ? -1 : (x >= n ? 1 : 0);
+                    if (x < -nl || x > n - 1 + nl) in_layer = false;
+                } else {
+                    if (x < 0) o[d] = -1;
+                    else if (x == 0) { o[d] = 0; o2 = -1; }
+                    else if (x < n) o[d] = 0;
+                    else if (x == n) { o[d] = 1; o2 = 0; }
+                    else o[d] = 1;
+                    if (x < -nl || x > n + nl) in_layer = false;
+                }
+            }
+            if (!in_layer) continue;
+            for (int t = 0; t < 2; t++) {
+                int q[3] = {o[0], o[1], o[2]};
+                if (t == 1) {
+                    if (o2 == -2) break;
+                    q[fd] = o2;
+                }
+                if (q[0] == 0 && q[1] == 0 && q[2] == 0) continue;
+                const int sv = s[fill_surr_flat(q[0], q[1], q[2])];
+                if (sv >= 0) {
+                    tile[e] = base[(long long)sv * ntile + tix(ijk[0] - q[0] * NB, ijk[1] - q[1] * NB, ijk[2] - q[2] * NZ)];
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        // 2. coarse surroundings, one after another
+        const int method = a.prolong[cc];
+        if (p >= 0) {
+            const double* par = base + (long long)p * ntile;
+            for (int kk = (D == 3 ? -1 : 0); kk <= (D == 3 ? 1 : 0); kk++)
+                for (int jj = -1; jj <= 1; jj++)
+                    for (int ii = -1; ii <= 1; ii++) {
+                        if (ii == 0 && jj == 0 && kk == 0) continue;
+                        const int sv = s[fill_surr_flat(ii, jj, kk)];
+                        if (!(sv < 0 && sv > PHYS_BNDRY)) continue;   // uniform over the workgroup
+                        const int oo[3] = {ii, jj, kk};
+                        int lo[3], hi[3];
+                        for (int d = 0; d < 3; d++) {
+                            const int n = N3[d];
+                            if (d >= D) { lo[d] = 0; hi[d] = 0; continue; }
+                            lo[d] = oo[d] < 0 ? -nl : (oo[d] > 0 ? n : 0);
+                            hi[d] = oo[d] < 0 ? -1 : (oo[d] > 0 ? n + nl - 1 : n - 1);
+                            if (d == fd) {
+                                hi[d] += 1;                      // staggered
+                                if (oo[d] == -1) hi[d] -= 1;     // non-overlapping
+                                if (oo[d] == 1) lo[d] += 1;
+                            }
+                        }
+                        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+                        const int nbox = nx * ny * nz;
+                        for (int e = threadIdx.x; e < nbox; e += FILL_NT) {
+                            const int f[3] = {lo[0] + e % nx, lo[1] + (e / nx) % ny, lo[2] + e / (nx * ny)};
+                            int c3[3];
+                            for (int d = 0; d < 3; d++) c3[d] = d < D ? fine_to_crse(f[d], off[d]) : f[d];
+                            const int side = (f[fd] + 64) & 1;
+                            const double c = par[tix(c3[0], c3[1], c3[2])];
+                            if (method == PR_INJECTION) {
+                                double v = c;
+                                if (side) {
+                                    int u[3] = {c3[0], c3[1], c3[2]};
+                                    u[fd]++;
+                                    v = 0.5 * (c + par[tix(u[0], u[1], u[2])]);
+                                }
+                                tile[tix(f[0], f[1], f[2])] = v;
+                            } else if (side == 0) {
+                                double gr[3] = {0.0, 0.0, 0.0}, rel[3] = {0.0, 0.0, 0.0};
+                                for (int d = 0; d < D; d++) {
+                                    if (d == fd) continue;
+                                    int up[3] = {c3[0], c3[1], c3[2]}, dn[3] = {c3[0], c3[1], c3[2]};
+                                    up[d]++;
+                                    dn[d]--;
+                                    const double vu = par[tix(up[0], up[1], up[2])], vd = par[tix(dn[0], dn[1], dn[2])];
+                                    gr[d] = method == PR_CENTERED_LINEAR ? 0.5 * (vu - vd) : mc_slope(vd, c, vu);
+                                    rel[d] = ((c3[d] - off[d]) * 2 == f[d]) ? -0.25 : 0.25;
+                                }
+                                tile[tix(f[0], f[1], f[2])] = c + rel[0] * gr[0] + rel[1] * gr[1] + rel[2] * gr[2];
+                            }
+                        }
+                        __syncthreads();
+                        if (method != PR_INJECTION) {
+                            for (int e = threadIdx.x; e < nbox; e += FILL_NT) {
+                                const int f[3] = {lo[0] + e % nx, lo[1] + (e / nx) % ny, lo[2] + e / (nx * ny)};
+                                if (((f[fd] + 64) & 1) == 0) continue;
+                                int m[3] = {f[0], f[1], f[2]}, q[3] = {f[0], f[1], f[2]};
+                                m[fd]--;
+                                q[fd]++;
+                                tile[tix(f[0], f[1], f[2])] = 0.5 * (tile[tix(m[0], m[1], m[2])] + tile[tix(q[0], q[1], q[2])]);
+                            }
+                            __syncthreads();
+                        }
+                    }
+        }
+        // ghost faces back to HBM
+        for (int e = threadIdx.x; e < ntile; e += FILL_NT) {
+            const int i = e % E0 - g, j = (e / E0) % E1 - g, k = e / (E0 * E1) - gz;
+            const bool inside = i >= 0 && i < V[0] && j >= 0 && j < V[1] && k >= 0 && k < V[2];
+            if (!inside) blk[e] = tile[e];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace sydev

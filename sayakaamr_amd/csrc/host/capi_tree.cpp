@@ -30,7 +30,10 @@ sy_tree* sy_tree_create(int ndim, int nb, int ngrow, int minlevel, int maxlevel,
     pb.vhi = Vector3d::VecMake(hi[0], hi[1], hi[2]);
     bool per[MAX_FACE] = {false, false, false, false, false, false};
     int bct[MAX_FACE];
-    for (int f = 0; f < MAX_FACE; f++) bct[f] = bc[f];
+    for (int f = 0; f < MAX_FACE; f++) {
+        per[f] = bc[f] == SY_TREE_PERIODIC;
+        bct[f] = per[f] ? -100 : bc[f];
+    }
     const int nd[3] = {ndiv[0], ndiv[1], ndim == 3 ? ndiv[2] : 1};
     t->initUniformRootLevel(nd, pb, bct, per);
     t->initRefineToMinLevel();

@@ -64,6 +64,7 @@ _SIG = {
     "sayaka_dev_free": ([vp, vp], i32),
     "sayaka_dev_memcpy": ([vp, vp, vp, C.c_size_t, i32], i32),
     "sayaka_dev_state_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp], i32),
+    "sayaka_dev_face_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, i32, vp], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -119,13 +120,17 @@ def _parr(arrs):
 
 
 class DeviceState:
-    """A cell state with ghost layers in device memory, [ncomp][blocks][grown box] doubles (the
-    device counterpart of a grown TreeData, SayakaTreeData.h:86-306)."""
+    """A state with ghost layers in device memory, [ncomp][blocks][grown box] doubles (the
+    device counterpart of a grown TreeData, SayakaTreeData.h:86-306): cell-centred, or face-
+    centred in direction fdir (one more face than cells along fdir)."""
 
-    def __init__(self, solver, ncomp, ngrow, blocks):
-        self.s, self.ncomp, self.ngrow, self.blocks = solver, ncomp, ngrow, blocks
+    def __init__(self, solver, ncomp, ngrow, blocks, fdir=None):
+        self.s, self.ncomp, self.ngrow, self.blocks, self.fdir = solver, ncomp, ngrow, blocks, fdir
         nb, nd = solver.nb, solver.ndim
-        self.shape = (ncomp, blocks) + ((nb + 2 * ngrow,) * 3 if nd == 3 else (1, nb + 2 * ngrow, nb + 2 * ngrow))
+        ext = [nb + 2 * ngrow + (fdir == d) for d in range(3)]
+        if nd == 2:
+            ext[2] = 1 + (fdir == 2)
+        self.shape = (ncomp, blocks, ext[2], ext[1], ext[0])
         self.nbytes = int(np.prod(self.shape)) * 8
         p = vp()
         _chk(solver.lib.sayaka_dev_alloc(solver.h, self.nbytes, C.byref(p)), "alloc")
@@ -142,9 +147,13 @@ class DeviceState:
              "memcpy")
         return out
 
-    def fill(self, nlayer, prolong, bcmap, scomp=0, nc=None):
-        """FillPatch::fillBoundary(scomp, nc, nlayer) on this state."""
-        self.s.state_fill(self.ptr, self.ncomp, self.blocks, self.ngrow, nlayer, prolong, bcmap, scomp, nc)
+    def fill(self, nlayer, prolong, bcmap=None, scomp=0, nc=None):
+        """FillPatch::fillBoundary(scomp, nc, nlayer) on this state (face data: no BC map, the
+        tree must be periodic)."""
+        if self.fdir is None:
+            self.s.state_fill(self.ptr, self.ncomp, self.blocks, self.ngrow, nlayer, prolong, bcmap, scomp, nc)
+        else:
+            self.s.face_fill(self.ptr, self.ncomp, self.blocks, self.fdir, self.ngrow, nlayer, prolong, scomp, nc)
 
     def close(self):
         if self.ptr and getattr(self.s, "h", None):
@@ -284,6 +293,14 @@ class MacSolverGPU:
         _chk(self.lib.sayaka_dev_state_fill(self.h, C.c_void_p(int(state_ptr)), ncomp, blocks_alloc, ngrow, scomp,
                                             nc, nlayer, N.ptr(pr), nphys, N.ptr(pc), N.ptr(mb), N.ptr(bv)),
              "state_fill")
+
+    def face_fill(self, state_ptr, ncomp, blocks_alloc, fdir, ngrow, nlayer, prolong, scomp=0, nc=None):
+        """Ghost fill of device face data of direction fdir at state_ptr ([ncomp][blocks_alloc]
+        [grown face box] doubles); prolong[c] as for state_fill.  Periodic trees only."""
+        nc = ncomp - scomp if nc is None else nc
+        pr = np.ascontiguousarray(prolong[:nc], dtype=np.int32)
+        _chk(self.lib.sayaka_dev_face_fill(self.h, C.c_void_p(int(state_ptr)), ncomp, blocks_alloc, fdir, ngrow,
+                                           scomp, nc, nlayer, N.ptr(pr)), "face_fill")
 
     def flux(self, m, d, phi):
         """MacSolver::mg_flux_level: face flux of direction d on tower m from field phi,

@@ -267,6 +267,31 @@ def state_fill(name):
     print(f"{name}: {n} blocks, {ncomp} components")
 
 
+def face_fill(name):
+    """FillPatch::fillBoundary of the reference on face data of every direction of the case's
+    periodic tree (tests/golden_cases.py FACE_CASES): the tree tables and, per direction and
+    nlayer, the ghost faces of every component after the fill."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import golden_cases as GC
+    spec = GC.face_spec(name)
+    nd, nb = spec.ndim, spec.nb
+    rt = R.RefTree(nd)
+    tree = W.build_tree(rt, spec, GC.face_criterion(name))
+    n = tree["level"].shape[0]
+    out = tree_arrays("tree_", tree)
+    for fd in range(nd):
+        prolong = GC.face_prolong(name, fd)
+        data = GC.face_input(name, tree, fd)
+        mask = GC.face_ghost_mask(nd, nb, GC.FILL_NGROW, fd)
+        for nlayer in GC.FACE_CASES[name][5]:
+            res = rt.fill_face_state(fd, data.reshape(len(prolong), n, -1), GC.FILL_NGROW, prolong, 0,
+                                     len(prolong), nlayer).reshape(data.shape)
+            assert np.array_equal(res[:, :, ~mask], data[:, :, ~mask])   # shared faces agree: unchanged
+            out[f"ghosts_d{fd}_nlayer{nlayer}"] = res[:, :, mask]
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: {n} blocks")
+
+
 def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     """MacSolver::mg_flux_level on the tower of every MG level and every direction, for the
     seeded input of case_ops (the same tree, coefficients and op_x) and for the case's own
@@ -305,6 +330,8 @@ if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "fill":
         state_fill("fill2")
         state_fill("fill3")
+        face_fill("fface2")
+        face_fill("fface3")
         sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "remap":
         regrid_remap(3)

@@ -97,3 +97,89 @@ def ghost_mask(nd, nb, g):
     m = np.ones((nz + 2 * gz, nb + 2 * g, nb + 2 * g), bool)
     m[gz:gz + nz, g:g + nb, g:g + nb] = False
     return m
+
+
+# ------------------------------------------------------------------- face-state fill cases
+# FillPatch::fillBoundary on face data (staggered in one direction) of a periodic domain (the
+# reference fills face data only where no wall is near).  Valid faces hold a hash of their
+# global position on their level, so the two copies of a face shared by same-level blocks
+# agree, as they do in a synchronised face field.
+FACE_CASES = {
+    # name -> (ndim, nb, maxlevel, criterion args, prolong per component, nlayer values)
+    "fface2": (2, 8, 4, ("ring", (0.5, 0.5), 0.2, 0.04), (0, 1, 2), (2, 1)),
+    "fface3": (3, 4, 3, ("shell", (0.3, 0.3, 0.3), 0.1), None, (2,)),
+}
+
+
+def face_prolong(name, fdir):
+    """Prolongation per component: the case's tuple, or in 3D one component per direction with
+    the methods rotated (x centred linear, y limited, z injection)."""
+    p = FACE_CASES[name][4]
+    return p if p is not None else ((1, 2, 0)[fdir],)
+
+
+def face_spec(name):
+    import dataclasses
+    nd, nb, maxlevel, crit, _, _ = FACE_CASES[name]
+    return dataclasses.replace(W.small_spec(nd, maxlevel, nb=nb), bc=(1,) * 6, ngrow=FILL_NGROW)
+
+
+def face_criterion(name):
+    kind, centre, r0, *rest = FACE_CASES[name][3]
+    if kind == "ring":
+        return W.ring_criterion(centre=centre, r0=r0, half=rest[0])
+    return W.shell_criterion(centre=centre, r0=r0)
+
+
+def face_shape(nd, nb, g, fdir):
+    nz, gz = (nb, g) if nd == 3 else (1, 0)
+    shp = [nz + 2 * gz, nb + 2 * g, nb + 2 * g]
+    shp[2 - fdir] += 1
+    return tuple(shp)
+
+
+def face_input(name, tree, fdir):
+    """[comp][block][gz][gy][gx] over the grown face box of direction fdir: valid faces U(-1,1)
+    from a hash of (component, level, global face position with periodic wrap, direction),
+    ghost faces a distinct sentinel each."""
+    nd, nb = FACE_CASES[name][:2]
+    g = FILL_NGROW
+    ncomp = len(face_prolong(name, fdir))
+    n = tree["level"].shape[0]
+    shp = face_shape(nd, nb, g, fdir)
+    out = -3000.0 - np.arange(ncomp * n * int(np.prod(shp)), dtype=np.float64).reshape((ncomp, n) + shp) * 1e-3
+    spec = face_spec(name)
+    gz = g if nd == 3 else 0
+    for b in range(n):
+        lvl = int(tree["level"][b])
+        dx = tree["length"][b, :nd] / nb
+        base = np.rint((tree["lo"][b, :nd] - np.array(spec.lo[:nd])) / dx).astype(np.int64)
+        ncells = np.array(spec.ndiv[:nd], dtype=np.int64) * (1 << (lvl - 1)) * nb
+        ext = [nb + (1 if d == fdir else 0) for d in range(nd)]
+        idx = np.meshgrid(*[np.arange(e) for e in ext[::-1]], indexing="ij")   # [z][y][x] order
+        gpos = [idx[nd - 1 - d] + base[d] for d in range(nd)]
+        gpos[fdir] = gpos[fdir] % ncells[fdir]
+        key = np.zeros(gpos[0].shape, dtype=np.uint64)
+        for d in range(nd):
+            key = key * np.uint64(1 << 20) + gpos[d].astype(np.uint64)
+        key = key * np.uint64(64) + np.uint64(lvl * 4 + fdir)
+        for c in range(ncomp):
+            h = W._splitmix64(key + np.uint64(0x9E37 * (c + 1)))
+            u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+            if nd == 3:
+                out[c, b, gz:gz + ext[2], g:g + ext[1], g:g + ext[0]] = u
+            else:
+                out[c, b, 0, g:g + ext[1], g:g + ext[0]] = u
+    return out
+
+
+def face_ghost_mask(nd, nb, g, fdir):
+    shp = face_shape(nd, nb, g, fdir)
+    m = np.ones(shp, bool)
+    gz = g if nd == 3 else 0
+    ext = [nb + (1 if d == fdir else 0) for d in range(3)]
+    if nd == 3:
+        m[gz:gz + ext[2], g:g + ext[1], g:g + ext[0]] = False
+    else:
+        m[0, g:g + ext[1], g:g + ext[0]] = False
+    return m
