@@ -64,7 +64,7 @@ def build_workload():
     t = AmrTree(3)
     tree = W.build_tree(t, spec, W.shell_criterion())
     inp = W.c1_inputs(tree, spec, seed=0)
-    return spec, tree, inp
+    return spec, tree, inp, t
 
 
 def make_c1_device(D, W, spec, tree, inp):
@@ -256,6 +256,46 @@ def learned_leg(D, n_test):
         dev.close()
 
 
+def output_leg(dev, D, W, spec, tree, tb, sol):
+    """After the solve, the user-side path of the reference's drivers: the solution into a
+    cell state with 2 ghost layers, FillPatch::fillBoundary (centred-linear prolongation,
+    Neumann walls), the valid cells packed in the Chombo file's order on the device, and the
+    HDF5 file written (WriteDataHdf5).  The pack kernel moves 16 B per cell (read + write)."""
+    import tempfile
+
+    from sayakaamr_amd import output
+
+    n = tree["level"].shape[0]
+    nb, g = spec.nb, 2
+    bcmap = [{W.WALL: (W.NEUMANN, 0.0)}]
+    st = D.DeviceState(dev, 1, g, n)
+    try:
+        host = np.zeros(st.shape)
+        host[0, :, g:g + nb, g:g + nb, g:g + nb] = sol.reshape(n, nb, nb, nb)
+        st.upload(host)
+        dev.set_surroundings(tree)
+        st.fill(g, [1], bcmap)   # warm
+        t0 = time.perf_counter()
+        st.fill(g, [1], bcmap)
+        fill_ms = (time.perf_counter() - t0) * 1e3
+        st.pack_levels()         # warm
+        t0 = time.perf_counter()
+        packed = st.pack_levels()
+        pack_ms = (time.perf_counter() - t0) * 1e3
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "c1.h5")
+            t0 = time.perf_counter()
+            output.write_chombo(tb, packed, 1, path, names=["pressure"])
+            write_ms = (time.perf_counter() - t0) * 1e3
+            size = os.path.getsize(path)
+    finally:
+        st.close()
+    return dict(blocks=n, cells=int(packed.size), ghost_fill_ms=round(fill_ms, 3),
+                pack_and_copy_ms=round(pack_ms, 3), pack_alg_bytes=int(packed.size) * 16,
+                write_ms=round(write_ms, 3), file_bytes=size,
+                write_gbs=round(size / (write_ms * 1e-3) / 1e9, 3))
+
+
 def large_leg(D, W):
     """16^3-block snapshot (139 M leaf cells): FP64 operator and smoother GB/s with the
     fields far beyond the caches, and one mixed solve."""
@@ -333,7 +373,7 @@ def pmc_workload():
     from sayakaamr_amd import device as D
     from sayakaamr_amd import workloads as W
 
-    spec, tree, inp = build_workload()
+    spec, tree, inp, tb = build_workload()
     dev = make_c1_device(D, W, spec, tree, inp)
     L = int(tree["level"].max())
     n = tree["level"].shape[0]
@@ -361,7 +401,7 @@ def summarize_pmc(dirs):
 
     from sayakaamr_amd import workloads as W
 
-    spec, tree, _ = build_workload()
+    spec, tree, _, _ = build_workload()
     L = int(tree["level"].max())
     n = tree["level"].shape[0]
     lv, leaf = tree["level"], tree["node_type"] == 1
@@ -432,7 +472,7 @@ def main():
     from sayakaamr_amd import workloads as W
 
     t0 = time.perf_counter()
-    spec, tree, inp = build_workload()
+    spec, tree, inp, tb = build_workload()
     n = tree["level"].shape[0]
     leaf = tree["node_type"] == 1
     nleaf_cells = int(leaf.sum()) * spec.nb ** 3
@@ -472,6 +512,8 @@ def main():
 
     mixed_rows, ms_mixed = mixed_leg(dev, D, W, spec, tree, inp, args.seeds, args.steps)
     rmx, classes_mixed = class_profile(dev, D, D.MIXED)
+    out_leg = output_leg(dev, D, W, spec, tree, tb, sol64)
+    log(f"output: {out_leg}")
     dev.close()
 
     learned = None if args.no_learned else learned_leg(D, args.learned)
@@ -536,6 +578,7 @@ def main():
         "learned": learned,
         "hbm_regime": large,
         "moving": moving,
+        "output": out_leg,
         "roofline": {"bound": "hbm", "kernel": roof_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                      "launches": prof["launches"], "avg_launch_us": round(avg_launch_us, 3),

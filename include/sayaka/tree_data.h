@@ -24,6 +24,8 @@ class TreeData {
     static TreeData MakeCellData(const AmrTree& tree, int ncomp, int ngrow);
     static TreeData MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow);
 
+    // the tree the data was made for (TreeData::tree, SayakaTreeData.h:149); it must outlive the data
+    const AmrTree& tree() const { return *tree_; }
     int numComp() const { return ncomp_; }
     int numGrow() const { return ngrow_; }
     int faceDir() const { return dir_; }   // -1 for cell data
@@ -55,6 +57,7 @@ class TreeData {
         return (std::size_t)blk * blocksize_ + (std::size_t)comp * compsize_ +
                (std::size_t)((i + ngrow_) + lx_ * ((j + ngrow_) + ly_ * (k + kg_)));
     }
+    const AmrTree* tree_ = nullptr;
     int ndim_ = 3, nb_ = 8, ngrow_ = 0, ncomp_ = 0, dir_ = -1, nslot_ = 0;
     int lx_ = 0, ly_ = 0, lz_ = 0, kg_ = 0;
     std::size_t compsize_ = 0, blocksize_ = 0;

@@ -169,6 +169,14 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blo
 int sayaka_dev_face_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blocks_alloc, int fdir, int ngrow,
                          int scomp, int nc, int nlayer, const int* prolong);
 
+/* The valid cells of a device cell state ([ncomp][blocks_alloc][grown box], as for
+ * sayaka_dev_state_fill) in the order of WriteDataHdf5's data:datatype=0 datasets
+ * (SayakaWriterHdf5.cpp:420-512): for every tree level 1..finest, every block of the level in
+ * ascending id, every component, the block's valid cells (i fastest).  out: device buffer of
+ * ncomp * nblk * nb^d doubles; one copy to the host then feeds sy_write_chombo. */
+int sayaka_dev_pack_levels(sayaka_dev_ctx* ctx, const double* state, int ncomp, int blocks_alloc, int ngrow,
+                           double* out);
+
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680): BiCGSTAB on the finest tower from
  * the SOL field as initial guess; SOL receives the solution on convergence. */
 int sayaka_dev_solve(sayaka_dev_ctx* ctx, const sayaka_solve_params* p, sayaka_solve_stats* st);

@@ -87,6 +87,17 @@ int sy_mac_set_params(sy_mac* m, double tol_rel, double tol_abs, int max_iter, i
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680) */
 int sy_mac_cg_solve(sy_mac* m, int* iters, int* status, double* rnorm0, double* rnorm);
 
+/* ---- Chombo-format HDF5 output (WriteDataHdf5, SayakaWriterHdf5.cpp:351-530) -----------
+ * Cell data already in the file's order, packed = for every tree level 1..finest, every block
+ * of the level in ascending id, every component, the block's valid cells (i fastest) -- what
+ * sayaka_dev_pack_levels produces.  WriterMeta (SayakaWriter.h:13-48): time, dt, step
+ * (written as "iteration"), names[ncomp] (NULL, or NULL entries: "comp<i>").  Written by the
+ * build's own HDF5 writer; no HDF5 library needed.  Returns 0, -1 on a bad argument or an I/O
+ * error (sy_writer_last_error). */
+const char* sy_writer_last_error(void);
+int sy_write_chombo(const sy_tree* t, int ncomp, const double* packed, const char* filename, double time,
+                    double dt, int step, const char* const* names);
+
 #ifdef __cplusplus
 }
 #endif

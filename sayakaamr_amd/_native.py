@@ -56,6 +56,8 @@ _HOST_SIG = {
     "sy_mac_flux": ([vp, i32, i32, vp, vp], i32),
     "sy_mac_set_params": ([vp, f64, f64, i32, i32, i32, i32, i32], i32),
     "sy_mac_cg_solve": ([vp, vp, vp, vp, vp], i32),
+    "sy_writer_last_error": ([], C.c_char_p),
+    "sy_write_chombo": ([vp, i32, vp, C.c_char_p, f64, f64, i32, vp], i32),
 }
 
 

@@ -760,6 +760,21 @@ struct Impl {
         return 0;
     }
 
+    // valid cells of a grown state in the output file's order (statefill.h, k_pack_levels)
+    static int pack_levels(sayaka_dev_ctx* c, const double* state, long long comp_stride, int ncomp, int ngrow,
+                           double* out) {
+        long long off = 0;
+        for (int l = 1; l <= c->maxlevel; l++) {
+            const ListRef lst = c->lv_all[l];
+            if (lst.n <= 0) continue;
+            hipLaunchKernelGGL((k_pack_levels<D, NB>), dim3(lst.n * ncomp), dim3(256), 0, c->stream, dlist(c, lst),
+                               ncomp, state, comp_stride, ngrow, out + off);
+            off += (long long)lst.n * ncomp * G::NCELL;
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // seeded face velocity of every block into d_vel (projection.h); d_lo = [nblk][3] corners
     static int gen_velocity(sayaka_dev_ctx* c, const double* d_lo, const double* dlo, const double* root,
                             unsigned long long seed, int zero_walls) {
@@ -1509,6 +1524,22 @@ int sayaka_dev_face_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks
     a.nc = nc;
     for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
     int e = dispatch(c, [&](auto I) { return decltype(I)::face_fill(c, a, lds); });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_pack_levels(sayaka_dev_ctx* c, const double* state, int ncomp, int blocks_alloc, int ngrow,
+                           double* out) {
+    if (!c || !state || !out || ncomp < 1 || ngrow < 0) return fail(-1, "bad argument");
+    if (c->nblk == 0) return fail(-1, "no topology");
+    if (blocks_alloc < c->nblk) return fail(-1, "blocks_alloc below the tree's block count");
+    const long long gx = c->nb + 2 * ngrow;
+    const long long ntile = gx * gx * (c->ndim == 3 ? c->nb + 2 * ngrow : 1);
+    HIPCHK(hipSetDevice(c->device));
+    int e = dispatch(c, [&](auto I) {
+        return decltype(I)::pack_levels(c, state, (long long)blocks_alloc * ntile, ncomp, ngrow, out);
+    });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;

@@ -363,4 +363,25 @@ __global__ void __launch_bounds__(FILL_NT) k_face_fill(FaceFillArgs a) {
     }
 }
 
+// ----------------------------------------------------------------------- output packing
+// Valid cells of a grown cell state in the order of a Chombo output file's data:datatype=0
+// datasets (WriteDataHdf5, SayakaWriterHdf5.cpp:480-512): per block of the level (ascending
+// id), per component, the block's valid cells, i fastest.  One launch per level; workgroup
+// (q, c) = block q of the level, component c.  A pure copy: 16 B of HBM traffic per cell.
+template <int D, int NB>
+__global__ void __launch_bounds__(256) k_pack_levels(const int* __restrict__ list, int ncomp,
+                                                     const double* __restrict__ state, long long comp_stride,
+                                                     int ngrow, double* __restrict__ out) {
+    constexpr int NCELL = D == 3 ? NB * NB * NB : NB * NB;
+    const int q = blockIdx.x / ncomp, c = blockIdx.x % ncomp;
+    const int gx = NB + 2 * ngrow, gz = D == 3 ? ngrow : 0;
+    const long long ntile = (long long)gx * gx * (D == 3 ? NB + 2 * ngrow : 1);
+    const double* src = state + c * comp_stride + (long long)list[q] * ntile;
+    double* dst = out + (long long)blockIdx.x * NCELL;
+    for (int e = threadIdx.x; e < NCELL; e += 256) {
+        const int i = e % NB, j = (e / NB) % NB, k = e / (NB * NB);
+        dst[e] = src[((k + gz) * gx + (j + ngrow)) * gx + (i + ngrow)];
+    }
+}
+
 }  // namespace sydev

@@ -16,6 +16,7 @@ TreeData TreeData::MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngr
     if (ncomp < 1 || ngrow < 0) throw std::invalid_argument("TreeData: ncomp >= 1 and ngrow >= 0 required");
     if (dir >= tree.ndim) throw std::invalid_argument("TreeData: face direction out of range");
     TreeData t;
+    t.tree_ = &tree;
     t.ndim_ = tree.ndim;
     t.nb_ = tree.nb;
     t.ngrow_ = ngrow;

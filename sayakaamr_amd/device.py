@@ -65,6 +65,7 @@ _SIG = {
     "sayaka_dev_memcpy": ([vp, vp, vp, C.c_size_t, i32], i32),
     "sayaka_dev_state_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp], i32),
     "sayaka_dev_face_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, i32, vp], i32),
+    "sayaka_dev_pack_levels": ([vp, vp, i32, i32, i32, vp], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -154,6 +155,30 @@ class DeviceState:
             self.s.state_fill(self.ptr, self.ncomp, self.blocks, self.ngrow, nlayer, prolong, bcmap, scomp, nc)
         else:
             self.s.face_fill(self.ptr, self.ncomp, self.blocks, self.fdir, self.ngrow, nlayer, prolong, scomp, nc)
+
+    def pack_levels(self):
+        """Valid cells in the Chombo file's order (per level, block, component), on the host:
+        packed on the device, then one copy."""
+        if self.fdir is not None:
+            raise ValueError("Chombo output takes cell data only (the reference exits otherwise)")
+        nb, nd = self.s.nb, self.s.ndim
+        n = self.s.n * self.ncomp * nb ** nd
+        p = vp()
+        _chk(self.s.lib.sayaka_dev_alloc(self.s.h, n * 8, C.byref(p)), "alloc")
+        try:
+            _chk(self.s.lib.sayaka_dev_pack_levels(self.s.h, C.c_void_p(self.ptr), self.ncomp, self.blocks,
+                                                   self.ngrow, p), "pack_levels")
+            out = np.empty(n)
+            _chk(self.s.lib.sayaka_dev_memcpy(self.s.h, out.ctypes.data_as(vp), p, n * 8, 0), "memcpy")
+        finally:
+            self.s.lib.sayaka_dev_free(self.s.h, p)
+        return out
+
+    def write_chombo(self, tree, filename, time=0.0, dt=1.0, step=0, names=None):
+        """WriteDataHdf5 (SayakaWriterHdf5.cpp:351-530) of this state on tree (the host
+        AmrTree it was filled on)."""
+        from . import output
+        output.write_chombo(tree, self.pack_levels(), self.ncomp, filename, time, dt, step, names)
 
     def close(self):
         if self.ptr and getattr(self.s, "h", None):
