@@ -69,6 +69,7 @@ struct sayaka_dev_ctx {
     ListRef tw[MAXLV + 1];                    // whole tower, sub-levels m..1
     ListRef tw_sub[MAXLV + 1][MAXLV + 1];     // [m][l]
     ListRef all_blocks;
+    ListRef by_level;                          // every block, levels 1.. in turn, ascending id
 
     double* d_field[SAYAKA_NFIELD] = {};
     double* d_vf = nullptr;
@@ -760,17 +761,14 @@ struct Impl {
         return 0;
     }
 
-    // valid cells of a grown state in the output file's order (statefill.h, k_pack_levels)
+    // valid cells of a grown state in the output file's order (statefill.h, k_pack_levels):
+    // one launch over the blocks in level order
     static int pack_levels(sayaka_dev_ctx* c, const double* state, long long comp_stride, int ncomp, int ngrow,
                            double* out) {
-        long long off = 0;
-        for (int l = 1; l <= c->maxlevel; l++) {
-            const ListRef lst = c->lv_all[l];
-            if (lst.n <= 0) continue;
-            hipLaunchKernelGGL((k_pack_levels<D, NB>), dim3(lst.n * ncomp), dim3(256), 0, c->stream, dlist(c, lst),
-                               ncomp, state, comp_stride, ngrow, out + off);
-            off += (long long)lst.n * ncomp * G::NCELL;
-        }
+        const ListRef lst = c->by_level;
+        if (lst.n <= 0) return 0;
+        hipLaunchKernelGGL((k_pack_levels<D, NB>), dim3(lst.n * ncomp), dim3(256), 0, c->stream, dlist(c, lst), ncomp,
+                           state, comp_stride, ngrow, out);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -1173,6 +1171,11 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
         add(c->lv_nonleaf[l], [&](int b) { return level[b] == l && node_type[b] != 1; });
     }
     add(c->all_blocks, [&](int) { return true; });
+    c->by_level.off = (int)L.size();
+    for (int l = 1; l <= maxlevel; l++)
+        for (int b = 0; b < nblk; b++)
+            if (level[b] == l) L.push_back(b);
+    c->by_level.n = (int)L.size() - c->by_level.off;
     for (int m = 1; m <= maxlevel; m++) {
         for (int l = 0; l <= MAXLV; l++) c->tw_sub[m][l] = ListRef{};
         c->tw[m].off = (int)L.size();

@@ -366,8 +366,9 @@ __global__ void __launch_bounds__(FILL_NT) k_face_fill(FaceFillArgs a) {
 // ----------------------------------------------------------------------- output packing
 // Valid cells of a grown cell state in the order of a Chombo output file's data:datatype=0
 // datasets (WriteDataHdf5, SayakaWriterHdf5.cpp:480-512): per block of the level (ascending
-// id), per component, the block's valid cells, i fastest.  One launch per level; workgroup
-// (q, c) = block q of the level, component c.  A pure copy: 16 B of HBM traffic per cell.
+// id), per component, the block's valid cells, i fastest.  One launch over the blocks listed
+// level by level; workgroup (q, c) = listed block q, component c.  A pure copy: 16 B of HBM
+// traffic per cell.
 template <int D, int NB>
 __global__ void __launch_bounds__(256) k_pack_levels(const int* __restrict__ list, int ncomp,
                                                      const double* __restrict__ state, long long comp_stride,

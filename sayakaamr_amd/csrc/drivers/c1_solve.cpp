@@ -6,20 +6,25 @@
 // projection step's source) with zero mean, then the plain FP64 solve and the
 // mixed-precision solve on the GPU.  (With the binary fractions of setIBFracByLS the
 // reference's MG-BiCGSTAB stagnates near 1e-7 relative residual on this problem; the
-// fractional ones reach 1e-10.)
+// fractional ones reach 1e-10.)  With an output path, the FP64 solution is written as a
+// Chombo HDF5 file (WriteDataHdf5, as the reference's demos write theirs), and beside it
+// <path>.blocks (per block: level as a double, lo[3], hi[3], blockLength[3]) and <path>.phi
+// (per block its valid cells, TreeData::packValid order) for checking the file.
 //
-//   sayaka_c1_solve [maxlevel=4] [seed=0]
+//   sayaka_c1_solve [maxlevel=4] [seed=0] [output.h5]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "amr_tree.h"
 #include "bc_register.h"
 #include "mac_solver.h"
 #include "tree_data.h"
+#include "writer.h"
 
 using namespace sayaka;
 
@@ -81,6 +86,36 @@ static void refine_level(AmrTree& t, int level) {
         std::fprintf(stderr, "regrid: block pool overflow\n");
         std::exit(1);
     }
+}
+
+static int write_output(const AmrTree& tree, const TreeData& phi, const char* path) {
+    WriterMeta meta;
+    meta.name = {"phi"};
+    std::string err;
+    if (WriteDataHdf5(phi, path, meta, &err) != 0) {
+        std::fprintf(stderr, "WriteDataHdf5: %s\n", err.c_str());
+        return -1;
+    }
+    std::vector<double> blocks, cells((size_t)tree.numBlocks * tree.cellsPerBlock());
+    for (int b = 0; b < tree.numBlocks; b++) {
+        blocks.push_back(tree[b].refineLevel);
+        for (int d = 0; d < 3; d++) blocks.push_back(tree[b].boundBox.vlo(d));
+        for (int d = 0; d < 3; d++) blocks.push_back(tree[b].boundBox.vhi(d));
+        for (int d = 0; d < 3; d++) blocks.push_back(tree[b].blockLength(d));
+    }
+    phi.packValid(0, tree.numBlocks, cells.data());
+    const std::string p(path);
+    for (const auto& [suffix, v] : {std::make_pair(std::string(".blocks"), &blocks), std::make_pair(std::string(".phi"), &cells)}) {
+        FILE* f = std::fopen((p + suffix).c_str(), "wb");
+        if (!f || std::fwrite(v->data(), sizeof(double), v->size(), f) != v->size()) {
+            std::fprintf(stderr, "cannot write %s%s\n", path, suffix.c_str());
+            if (f) std::fclose(f);
+            return -1;
+        }
+        std::fclose(f);
+    }
+    std::printf("wrote %s\n", path);
+    return 0;
 }
 
 int main(int argc, char** argv) {
@@ -222,6 +257,7 @@ int main(int argc, char** argv) {
         status[pass] = mac.lastStats().status;
         if (ret != 0) std::printf("cg_solve returned %d\n", ret);
         mac.getSol(phi, 0);
+        if (pass == 0 && argc > 3 && write_output(tree, phi, argv[3]) != 0) return 3;
         auto& s = sols[pass];
         for (int b = 0; b < tree.numBlocks; b++)
             if (tree[b].isLeaf())

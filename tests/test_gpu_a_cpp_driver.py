@@ -3,23 +3,31 @@ call surface, Test7.h:411-418, 612-653), run as its own process on the GPU.
 
 This file sorts first among the GPU tests on purpose: the bottom solve is a cooperative
 launch, and the HIP runtime refuses a cooperative launch in a second process while another
-process on the card holds one (the pytest process would, once any solve had run in it)."""
+process on the card holds one (the pytest process would, once any solve had run in it).
+
+The driver also writes its FP64 solution through WriteDataHdf5(TreeData) with the raw block
+data and geometry beside it; the file must equal, object by object, the reference's writer
+restated over the image's HDF5 library on the same tree and data (skipped without that
+library)."""
 from __future__ import annotations
 
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
+from oracle import chombo_oracle as CO
 from sayakaamr_amd import _native as N
 
 pytestmark = pytest.mark.gpu
 
 
-def test_cpp_driver():
+def test_cpp_driver(tmp_path):
     exe = os.path.join(N.LIBDIR, "sayaka_c1_solve")
     print("[gpu] C++ driver", exe, flush=True)
-    r = subprocess.run([exe, "4", "0"], capture_output=True, text=True, timeout=100)
+    h5 = str(tmp_path / "c1.h5")
+    r = subprocess.run([exe, "4", "0", h5], capture_output=True, text=True, timeout=100)
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stderr
     kv = dict(tok.split("=") for line in r.stdout.splitlines() for tok in line.split() if "=" in tok)
@@ -27,3 +35,20 @@ def test_cpp_driver():
     assert kv["fp64_status"] == "0" and kv["mixed_status"] == "0"
     assert int(kv["fp64_iters"]) < 50
     assert float(kv["rel_l2_mixed_vs_fp64"]) < 1e-3
+    check_output(h5)
+
+
+def check_output(h5):
+    blk = np.fromfile(h5 + ".blocks").reshape(-1, 10)
+    n = blk.shape[0]
+    tree = dict(level=blk[:, 0].astype(np.int32), lo=blk[:, 1:4], hi=blk[:, 4:7], length=blk[:, 7:10])
+    phi = np.fromfile(h5 + ".phi").reshape(1, n, 8, 8, 8)
+    assert np.isfinite(phi).all()
+    if CO.HDF5 is None:
+        pytest.skip("no HDF5 library in this image to read the file back")
+    ref = h5 + ".ref.h5"
+    CO.write_reference(ref, tree, 8, 3, phi, 0, names=["phi"])
+    p, r = CO.HDF5.dump(h5), CO.HDF5.dump(ref)
+    assert sorted(p) == sorted(r)
+    for k in r:
+        assert p[k] == r[k], k
