@@ -169,6 +169,24 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blo
 int sayaka_dev_face_fill(sayaka_dev_ctx* ctx, double* state, int ncomp, int blocks_alloc, int fdir, int ngrow,
                          int scomp, int nc, int nlayer, const int* prolong);
 
+/* AmrTree::regrid's packing of a registered state and initNewBlockDataAfterRegrid
+ * (SayakaRegrid.cpp:530-591, 987-1105) on the device, after sayaka_dev_set_topology and
+ * sayaka_dev_set_surroundings for the NEW tree.  old_state: the state on the old tree
+ * ([ncomp][old_blocks_alloc][grown box], nold blocks); state: the new state, a distinct buffer
+ * ([ncomp][blocks_alloc][grown box]); fdir -1 for cell data, else the face direction (grown
+ * face box, as for sayaka_dev_face_fill); origin[nblk]: each new block's id before the regrid,
+ * -1 for a block the regrid created (sy_tree_regrid_origin).  Surviving blocks move whole; new
+ * blocks start at 0; then, level by level from the coarsest holding a new block, the parents of
+ * the level's new blocks are filled in-level over all ngrow layers with the physical BC
+ * (fillBlockBoundary(.., onlySameLevel = 1); BC register as for sayaka_dev_state_fill), each
+ * new block is prolonged over its valid box from its parent (prolongBlock; prolong[c] per
+ * component) and, for face data, takes the shared faces of its existing same-level face
+ * neighbours (correctFaceBlockData, SayakaFillPatch.cpp:485-513).  Face data: periodic trees
+ * only (-2 otherwise, where the reference exits). */
+int sayaka_dev_state_remap(sayaka_dev_ctx* ctx, const double* old_state, int old_blocks_alloc, int nold,
+                           double* state, int ncomp, int blocks_alloc, int ngrow, int fdir, const int* origin,
+                           const int* prolong, int nphys, const int* physbc, const int* mathbc, const double* bcval);
+
 /* The valid cells of a device cell state ([ncomp][blocks_alloc][grown box], as for
  * sayaka_dev_state_fill) in the order of WriteDataHdf5's data:datatype=0 datasets
  * (SayakaWriterHdf5.cpp:420-512): for every tree level 1..finest, every block of the level in

@@ -265,33 +265,47 @@ class StateFill:
         if nlayer <= 0:
             return
         t = self.t
-        order = self.bc_order()
         for level in range(1, self.maxlevel + 1):
             blocks = np.nonzero(t["level"] == level)[0]
             for b in blocks:
-                b = int(b)
-                for c in range(scomp, scomp + nc):
-                    arr = state[c]
-                    # same level (SayakaFillPatch.cpp:125-188)
-                    for ii, jj, kk in self._surr_iter():
-                        nbk = self.surr(b, ii, jj, kk)
-                        if nbk >= 0:
-                            dst, src = self.ranges(ii, jj, kk, nlayer)
-                            self._view(arr, b, dst)[...] = self._view(arr, nbk, src)
-                    # coarse level (SayakaFillPatch.cpp:225-297)
-                    coarse = [s for s in self._surr_iter() if PHYS_BNDRY < self.surr(b, *s) < 0]
-                    if coarse:
-                        p = int(t["parent"][b])
-                        wc = int(t["which_child"][b])
-                        off = [((wc >> d) & 1) * (self.nb // 2) if d < self.nd else 0 for d in range(3)]
-                        for ii, jj, kk in coarse:
-                            dst, _ = self.ranges(ii, jj, kk, nlayer)
-                            self._prolong_box(arr, b, p, dst, off, prolong[c])
-                    # physical boundary (SayakaFillPatch.cpp:301-368)
-                    if any(self.surr(b, *s) <= PHYS_BNDRY for s in self._surr_iter()):
-                        for sweep, sweepdir, ii, jj, kk in order:
-                            if self.surr(b, ii, jj, kk) <= PHYS_BNDRY:
-                                self._bc_one(arr, b, c, sweepdir, ii, jj, kk, bcmap)
+                self.fill_block(state, int(b), scomp, nc, nlayer, prolong, bcmap)
+
+    def fill_block(self, state, b, scomp, nc, nlayer, prolong, bcmap, same_level_only=False):
+        """fillBlockBoundary (SayakaFillPatch.cpp:54-89) with onlySameLevel as given, physical BC
+        applied and corners filled."""
+        t = self.t
+        for c in range(scomp, scomp + nc):
+            arr = state[c]
+            # same level (SayakaFillPatch.cpp:125-188)
+            for ii, jj, kk in self._surr_iter():
+                nbk = self.surr(b, ii, jj, kk)
+                if nbk >= 0:
+                    dst, src = self.ranges(ii, jj, kk, nlayer)
+                    self._view(arr, b, dst)[...] = self._view(arr, nbk, src)
+            # coarse level (SayakaFillPatch.cpp:225-297)
+            coarse = [s for s in self._surr_iter() if PHYS_BNDRY < self.surr(b, *s) < 0]
+            if coarse and not same_level_only:
+                p = int(t["parent"][b])
+                off = self._offset(b)
+                for ii, jj, kk in coarse:
+                    dst, _ = self.ranges(ii, jj, kk, nlayer)
+                    self._prolong_box(arr, b, p, dst, off, prolong[c])
+            # physical boundary (SayakaFillPatch.cpp:301-368)
+            if any(self.surr(b, *s) <= PHYS_BNDRY for s in self._surr_iter()):
+                for sweep, sweepdir, ii, jj, kk in self.bc_order():
+                    if self.surr(b, ii, jj, kk) <= PHYS_BNDRY:
+                        self._bc_one(arr, b, c, sweepdir, ii, jj, kk, bcmap)
+
+    def _offset(self, b):
+        """The block's offset in its parent's cells (ChildIndexOffset)."""
+        wc = int(self.t["which_child"][b])
+        return [((wc >> d) & 1) * (self.nb // 2) if d < self.nd else 0 for d in range(3)]
+
+    def prolong_block(self, state, b, prolong):
+        """FillPatch::prolongBlock (SayakaFillPatch.cpp:464-483): the valid box from the parent."""
+        p = int(self.t["parent"][b])
+        for c, method in enumerate(prolong):
+            self._prolong_box(state[c], b, p, self._valid(), self._offset(b), method)
 
 
 class FaceFill(StateFill):
@@ -397,20 +411,77 @@ class FaceFill(StateFill):
         t = self.t
         for level in range(1, self.maxlevel + 1):
             for b in np.nonzero(t["level"] == level)[0]:
-                b = int(b)
-                if any(self.surr(b, *s) <= PHYS_BNDRY for s in self._surr_iter()):
-                    raise ValueError("face data next to a wall (the reference exits)")
-                for c in range(scomp, scomp + nc):
-                    arr = state[c]
-                    for ii, jj, kk in self._surr_iter():
-                        nbk = self.surr(b, ii, jj, kk)
-                        if nbk >= 0:
-                            dst, src = self.ranges(ii, jj, kk, nlayer)
-                            self._view(arr, b, dst)[...] = self._view(arr, nbk, src)
-                    coarse = [s for s in self._surr_iter() if PHYS_BNDRY < self.surr(b, *s) < 0]
-                    if coarse:
-                        p = int(t["parent"][b])
-                        wc = int(t["which_child"][b])
-                        off = [((wc >> d) & 1) * (self.nb // 2) if d < self.nd else 0 for d in range(3)]
-                        for ii, jj, kk in coarse:
-                            self._prolong_face(arr, b, p, self.nonoverlap(ii, jj, kk, nlayer), off, prolong[c])
+                self.fill_block(state, int(b), scomp, nc, nlayer, prolong)
+
+    def fill_block(self, state, b, scomp, nc, nlayer, prolong, bcmap=None, same_level_only=False):
+        if any(self.surr(b, *s) <= PHYS_BNDRY for s in self._surr_iter()):
+            raise ValueError("face data next to a wall (the reference exits)")
+        for c in range(scomp, scomp + nc):
+            arr = state[c]
+            for ii, jj, kk in self._surr_iter():
+                nbk = self.surr(b, ii, jj, kk)
+                if nbk >= 0:
+                    dst, src = self.ranges(ii, jj, kk, nlayer)
+                    self._view(arr, b, dst)[...] = self._view(arr, nbk, src)
+            coarse = [s for s in self._surr_iter() if PHYS_BNDRY < self.surr(b, *s) < 0]
+            if coarse and not same_level_only:
+                p = int(self.t["parent"][b])
+                for ii, jj, kk in coarse:
+                    self._prolong_face(arr, b, p, self.nonoverlap(ii, jj, kk, nlayer), self._offset(b), prolong[c])
+
+    def prolong_block(self, state, b, prolong):
+        p = int(self.t["parent"][b])
+        for c, method in enumerate(prolong):
+            self._prolong_face(state[c], b, p, self._stagger(self._valid()), self._offset(b), method)
+
+    def correct_faces(self, state, b, is_new):
+        """The faces of new block b shared with existing same-level face neighbours take the
+        neighbour's copy (correctFaceBlockData, SayakaFillPatch.cpp:485-513, as called by
+        SayakaRegrid.cpp:1078-1100)."""
+        fd = self.fd
+        for side in (0, 1):
+            j = int(self.t["neighbor"][b, 2 * fd + side])
+            if j < 0 or is_new[j]:
+                continue
+            lo, hi = self._stagger(self._valid())
+            dlo, dhi, slo, shi = list(lo), list(hi), list(lo), list(hi)
+            dlo[fd] = dhi[fd] = lo[fd] if side == 0 else hi[fd]
+            slo[fd] = shi[fd] = hi[fd] if side == 0 else lo[fd]
+            for arr in state:
+                self._view(arr, b, (dlo, dhi))[...] = self._view(arr, j, (slo, shi))
+
+
+def remap_state(tree, origin, pre, nb, ndim, ngrow, maxlevel, prolong, bcmap=None, fdir=None):
+    """A registered state after AmrTree::regrid and initNewBlockDataAfterRegrid.
+
+    pre: [comp][old block][grown box] (cell, or face data of direction fdir); origin: per block
+    of the new tree its id before the regrid, -1 for a block the regrid created; tree: the new
+    tree.  Packing (SayakaRegrid.cpp:530-591) moves each surviving block's whole grown box; a new
+    block starts at 0.  Then, level by level from the coarsest level holding a new block
+    (SayakaRegrid.cpp:1013-1103): every non-leaf block of the level below with a new child,
+    in ascending id, is filled in-level over all ngrow layers with the physical BC
+    (fillBlockBoundary(.., onlySameLevel = 1)); every new block of the level, in ascending id,
+    is prolonged from its parent over its valid box (prolongBlock) and, for face data, takes
+    the shared faces of its existing same-level face neighbours (correctFaceBlockData)."""
+    n = origin.shape[0]
+    ncomp = pre.shape[0]
+    post = np.zeros((ncomp, n) + pre.shape[2:])
+    keep = origin >= 0
+    post[:, keep] = pre[:, origin[keep]]
+    new = ~keep
+    if not new.any():
+        return post
+    ff = (StateFill(tree, nb, ndim, ngrow, maxlevel) if fdir is None
+          else FaceFill(tree, nb, ndim, ngrow, maxlevel, fdir))
+    st = [post[c] for c in range(ncomp)]
+    lv = tree["level"]
+    for level in range(int(lv[new].min()), int(lv[new].max()) + 1):
+        for p in np.nonzero(lv == level - 1)[0]:
+            if tree["node_type"][p] == 1 or not new[tree["child"][p][: 1 << ndim]].any():
+                continue
+            ff.fill_block(st, int(p), 0, ncomp, ngrow, prolong, bcmap, same_level_only=True)
+        for b in np.nonzero((lv == level) & new)[0]:
+            ff.prolong_block(st, int(b), prolong)
+            if fdir is not None:
+                ff.correct_faces(st, int(b), new)
+    return post

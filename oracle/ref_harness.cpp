@@ -449,6 +449,63 @@ void ref_state_set(const double* valid) {
 void ref_state_get(double* valid) {
     copy_out_box(*g_tree->treeStateData.back().new_data, 0, valid_box(), valid, g_tree->numBlocks);
 }
+// A general registered state, as the demos register theirs (cell states, Test5.h:590-600;
+// face velocity states per direction, Test4.h:455-470): ncomp components with ngrow ghost
+// layers, cell data (dir < 0) or face data staggered in dir, a FillPatch with one prolongation
+// per component and, for cell data, a BCPatch mapping the nphys physical codes to
+// (mathbc, bcval)[comp][h] on every surrounding position.  Every slot of the pool starts at 0.
+static IndexBox state_box(int dir, int ngrow) {
+    IndexBox gb = grown_cell_box(ngrow);
+    if (dir >= 0) gb.staggerInDir(dir);
+    return gb;
+}
+static int g_state_dir = -1, g_state_ngrow = 0;
+void ref_state_create_general(int dir, int ncomp, int ngrow, const int* prolong, int nphys, const int* physbc,
+                              const int* mathbc, const double* bcval) {
+    TreeData* st = dir < 0 ? TreeData::CreateCellData(*g_tree, ncomp, ngrow)
+                           : TreeData::CreateFaceData(*g_tree, dir, ncomp, ngrow);
+    FillPatch* fp = new FillPatch(*g_tree, *st);
+    for (int c = 0; c < ncomp; c++) fp->setProlongation(c, prolong[c]);
+    TreeStateData tsd;
+    if (dir < 0) {
+        BCPatch* bc = new BCPatch();
+        bc->defineByData(*st);
+        for (int c = 0; c < ncomp; c++)
+            for (int q = 0; q < SurroundingIndex::NumSurr; q++) {
+                SurroundingIndex sidx(q);
+                for (int t = 0; t < nphys; t++) {
+                    bc->boundaryRegister(c).setBCMap(sidx.ipos(), sidx.jpos(), sidx.kpos(), physbc[t],
+                                                     mathbc[c * nphys + t]);
+                    bc->boundaryRegister(c).setBCVal(sidx.ipos(), sidx.jpos(), sidx.kpos(), physbc[t],
+                                                     bcval[c * nphys + t]);
+                }
+            }
+        fp->setBC(*bc);
+        tsd.bc_patch = bc;
+    }
+    tsd.new_data = st;
+    tsd.fill_new_patch = fp;
+    g_tree->treeStateData.push_back(tsd);
+    g_state_dir = dir;
+    g_state_ngrow = ngrow;
+    const IndexBox gb = state_box(dir, ngrow);
+    std::vector<double> zero((size_t)g_tree->maxBlockNum * gb.stride(), 0.0);
+    for (int c = 0; c < ncomp; c++) copy_in_box(*st, c, gb, zero.data(), g_tree->maxBlockNum);
+}
+// The whole grown box of every component of the tree's current blocks: [comp][block][box].
+void ref_state_set_grown(const double* data) {
+    TreeData& st = *g_tree->treeStateData.back().new_data;
+    const IndexBox gb = state_box(g_state_dir, g_state_ngrow);
+    const size_t per = (size_t)g_tree->numBlocks * gb.stride();
+    for (int c = 0; c < st.numComp(); c++) copy_in_box(st, c, gb, data + c * per, g_tree->numBlocks);
+}
+void ref_state_get_grown(double* data) {
+    const TreeData& st = *g_tree->treeStateData.back().new_data;
+    const IndexBox gb = state_box(g_state_dir, g_state_ngrow);
+    const size_t per = (size_t)g_tree->numBlocks * gb.stride();
+    for (int c = 0; c < st.numComp(); c++) copy_out_box(st, c, gb, data + c * per, g_tree->numBlocks);
+}
+
 void ref_tree_init_new_data() {
     quiet_begin();
     g_tree->initNewBlockDataAfterRegrid();

@@ -73,6 +73,9 @@ def load(ndim: int):
         "ref_state_set": ([_p], None),
         "ref_state_get": ([_p], None),
         "ref_tree_init_new_data": ([], None),
+        "ref_state_create_general": ([_i, _i, _i, _p, _i, _p, _p, _p], None),
+        "ref_state_set_grown": ([_p], None),
+        "ref_state_get_grown": ([_p], None),
         "ref_fill_state": ([_i, _i, _p, _p, _i, _p, _p, _p, _i, _i, _i], None),
         "ref_fill_face_state": ([_i, _i, _i, _p, _p, _i, _i, _i], None),
         "ref_restrict": ([_i, _i], None),
@@ -136,6 +139,34 @@ class RefTree:
 
     def init_new_data(self):
         self.lib.ref_tree_init_new_data()
+
+    # a general registered state: cell (dir < 0) or face data, per-component prolongation,
+    # BC register (cell data) as for fill_state; accessed as [comp][block][grown box]
+    def state_create_general(self, dir, ncomp, ngrow, prolong, physbc=(), mathbc=(), bcval=()):
+        pr = np.ascontiguousarray(prolong, dtype=np.int32)
+        pb = np.ascontiguousarray(physbc if len(physbc) else [0], dtype=np.int32)
+        nphys = len(physbc)
+        mb = np.ascontiguousarray(mathbc if nphys else [0], dtype=np.int32)
+        bv = np.ascontiguousarray(bcval if nphys else [0.0], dtype=np.float64)
+        self._gstate = (dir, ncomp, ngrow)
+        self.lib.ref_state_create_general(dir, ncomp, ngrow, _ptr(pr), nphys, _ptr(pb), _ptr(mb), _ptr(bv))
+
+    def _gshape(self, nb):
+        dir, ncomp, ngrow = self._gstate
+        ext = [nb + 2 * ngrow + (dir == d) for d in range(3)]
+        if self.ndim == 2:
+            ext[2] = 1
+        return (ncomp, self.lib.ref_tree_numblocks(), ext[2], ext[1], ext[0])
+
+    def state_set_grown(self, data, nb):
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        assert d.shape == self._gshape(nb), (d.shape, self._gshape(nb))
+        self.lib.ref_state_set_grown(_ptr(d))
+
+    def state_get_grown(self, nb):
+        out = np.zeros(self._gshape(nb))
+        self.lib.ref_state_get_grown(_ptr(out))
+        return out
 
     # FillPatch::fillBoundary on a [comp][block][grown] state (a fresh copy is returned)
     def fill_state(self, data, ngrow, prolong, physbc, mathbc, bcval, scomp, nc, nlayer):

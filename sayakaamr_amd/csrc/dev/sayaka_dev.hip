@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -144,6 +145,8 @@ struct sayaka_dev_ctx {
     int* d_fill_int = nullptr;     // physical codes, then [nc][27][nphys] math BC types
     double* d_fill_val = nullptr;  // [nc][27][nphys] BC values
     size_t fill_icap = 0, fill_vcap = 0;
+    int* d_remap = nullptr;        // state remap: origin, then the per-level block lists
+    size_t remap_cap = 0;
 };
 
 namespace {
@@ -761,6 +764,37 @@ struct Impl {
         return 0;
     }
 
+    // registered-state remap (statefill.h): move, then per level the parents' in-level fill
+    // and the new blocks' prolongation.  lists: per level (1..maxlevel) [parents | new blocks]
+    // at offsets po/no with counts pn/nn, all inside d_remap.
+    static int state_remap(sayaka_dev_ctx* c, const double* old_state, long long old_cstride, double* state,
+                           long long cstride, long long ntile, int ncomp, int ngrow, int fdir,
+                           const std::vector<int>& po, const std::vector<int>& pn, const std::vector<int>& no,
+                           const std::vector<int>& nn, FillArgs fa, FaceFillArgs ffa, ProlongNewArgs pa,
+                           size_t lds_fill, size_t lds_face) {
+        hipLaunchKernelGGL(k_state_move, dim3(c->nblk, ncomp), dim3(256), 0, c->stream, c->d_remap, old_state,
+                           old_cstride, state, cstride, ntile);
+        for (int l = 1; l <= c->maxlevel; l++) {
+            if (nn[l] == 0) continue;
+            if (pn[l] > 0) {
+                if (fdir < 0) {
+                    fa.list = c->d_remap + po[l];
+                    hipLaunchKernelGGL((k_state_fill<D, NB>), dim3(pn[l]), dim3(FILL_NT), lds_fill, c->stream, fa);
+                } else {
+                    ffa.list = c->d_remap + po[l];
+                    hipLaunchKernelGGL((k_face_fill<D, NB>), dim3(pn[l]), dim3(FILL_NT), lds_fill, c->stream, ffa);
+                }
+            }
+            pa.list = c->d_remap + no[l];
+            if (fdir < 0)
+                hipLaunchKernelGGL((k_prolong_new<D, NB>), dim3(nn[l]), dim3(FILL_NT), 0, c->stream, pa);
+            else
+                hipLaunchKernelGGL((k_prolong_new_face<D, NB>), dim3(nn[l]), dim3(FILL_NT), lds_face, c->stream, pa);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // valid cells of a grown state in the output file's order (statefill.h, k_pack_levels):
     // one launch over the blocks in level order
     static int pack_levels(sayaka_dev_ctx* c, const double* state, long long comp_stride, int ncomp, int ngrow,
@@ -1065,6 +1099,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_surr) hipFree(c->d_surr);
     if (c->d_bcref) hipFree(c->d_bcref);
     if (c->d_fill_int) hipFree(c->d_fill_int);
+    if (c->d_remap) hipFree(c->d_remap);
     if (c->d_fill_val) hipFree(c->d_fill_val);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
                     c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_bot_flags, c->d_binfo, c->d_met, c->d_lists,
@@ -1403,30 +1438,11 @@ int sayaka_dev_set_surroundings(sayaka_dev_ctx* c, int nblk, const int* surr27, 
     return 0;
 }
 
-int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks_alloc, int ngrow, int scomp, int nc,
-                          int nlayer, const int* prolong, int nphys, const int* physbc, const int* mathbc,
-                          const double* bcval) {
-    if (!c || !state || ncomp < 1 || ngrow < 1 || scomp < 0 || nc < 0 || scomp + nc > ncomp)
-        return fail(-1, "bad argument");
-    if (nlayer <= 0 || nc == 0) return 0;   // the reference returns at once for nlayer <= 0
-    if (nlayer > ngrow) return fail(-1, "nlayer > ngrow");
-    if (nc > FILL_MAXCOMP) return fail(-1, "more than 16 components in one fill");
-    if (c->surr_n != c->nblk || c->nblk == 0) return fail(-1, "surrounding tables not set for this topology");
-    if (blocks_alloc < c->nblk) return fail(-1, "blocks_alloc below the tree's block count");
-    if (nphys < 0 || (nphys > 0 && (!physbc || !mathbc || !bcval)) || !prolong) return fail(-1, "bad argument");
-    const int nz = c->ndim == 3 ? c->nb : 1, gz = c->ndim == 3 ? ngrow : 0;
-    const long long gx = c->nb + 2 * ngrow;
-    const long long ntile = gx * gx * (nz + 2 * gz);
-    const size_t lds = (size_t)ntile * sizeof(double);
-    if (lds > 64 * 1024) return fail(-1, "grown block above 64 KiB (ngrow too large for this block size)");
-    for (int q = 0; q < nc; q++) {
-        if (prolong[q] < 0 || prolong[q] > 2) return fail(-1, "unknown prolongation method");
-        // centred slopes read one coarse cell beyond the mapped one (the reference exits,
-        // SayakaInterpPatch.cpp:233-236)
-        if (prolong[q] > 0 && ngrow < 2) return fail(-1, "centred-linear prolongation needs ngrow >= 2");
-    }
-    // every (surrounding, physical code) the tree holds needs a BC for every component
-    // (the reference exits on an unknown mapping, SayakaBoundaryPatch.h:101-110)
+// BC register of a cell-state fill (FillArgs.physcode / bctype / bcval): every (surrounding,
+// physical code) the tree holds needs a BC for every filled component (the reference exits on
+// an unknown mapping, SayakaBoundaryPatch.h:101-110); the tables go to the device.
+static int setup_fill_bc(sayaka_dev_ctx* c, int nc, int scomp, int nphys, const int* physbc, const int* mathbc,
+                         const double* bcval) {
     std::vector<char> seen((size_t)27 * (nphys + 1), 0);
     for (int b = 0; b < c->nblk; b++)
         for (int q = 0; q < 27; q++) {
@@ -1465,6 +1481,32 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int block
         HIPCHK(hipMemcpyAsync(c->d_fill_val, bcval, nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));   // the host vector goes out of scope
     }
+    return 0;
+}
+
+int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks_alloc, int ngrow, int scomp, int nc,
+                          int nlayer, const int* prolong, int nphys, const int* physbc, const int* mathbc,
+                          const double* bcval) {
+    if (!c || !state || ncomp < 1 || ngrow < 1 || scomp < 0 || nc < 0 || scomp + nc > ncomp)
+        return fail(-1, "bad argument");
+    if (nlayer <= 0 || nc == 0) return 0;   // the reference returns at once for nlayer <= 0
+    if (nlayer > ngrow) return fail(-1, "nlayer > ngrow");
+    if (nc > FILL_MAXCOMP) return fail(-1, "more than 16 components in one fill");
+    if (c->surr_n != c->nblk || c->nblk == 0) return fail(-1, "surrounding tables not set for this topology");
+    if (blocks_alloc < c->nblk) return fail(-1, "blocks_alloc below the tree's block count");
+    if (nphys < 0 || (nphys > 0 && (!physbc || !mathbc || !bcval)) || !prolong) return fail(-1, "bad argument");
+    const int nz = c->ndim == 3 ? c->nb : 1, gz = c->ndim == 3 ? ngrow : 0;
+    const long long gx = c->nb + 2 * ngrow;
+    const long long ntile = gx * gx * (nz + 2 * gz);
+    const size_t lds = (size_t)ntile * sizeof(double);
+    if (lds > 64 * 1024) return fail(-1, "grown block above 64 KiB (ngrow too large for this block size)");
+    for (int q = 0; q < nc; q++) {
+        if (prolong[q] < 0 || prolong[q] > 2) return fail(-1, "unknown prolongation method");
+        // centred slopes read one coarse cell beyond the mapped one (the reference exits,
+        // SayakaInterpPatch.cpp:233-236)
+        if (prolong[q] > 0 && ngrow < 2) return fail(-1, "centred-linear prolongation needs ngrow >= 2");
+    }
+    if (int e = setup_fill_bc(c, nc, scomp, nphys, physbc, mathbc, bcval)) return e;
     FillArgs a{};
     a.binfo = c->d_binfo;
     a.surr = c->d_surr;
@@ -1529,6 +1571,111 @@ int sayaka_dev_face_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks
     int e = dispatch(c, [&](auto I) { return decltype(I)::face_fill(c, a, lds); });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_state_remap(sayaka_dev_ctx* c, const double* old_state, int old_blocks_alloc, int nold, double* state,
+                           int ncomp, int blocks_alloc, int ngrow, int fdir, const int* origin, const int* prolong,
+                           int nphys, const int* physbc, const int* mathbc, const double* bcval) {
+    if (!c || !old_state || !state || !origin || !prolong || ncomp < 1 || ngrow < 1 || nold < 1)
+        return fail(-1, "bad argument");
+    if (old_state == state) return fail(-1, "the old and the new state must be distinct buffers");
+    if (ncomp > FILL_MAXCOMP) return fail(-1, "more than 16 components");
+    if (fdir < -1 || fdir >= c->ndim) return fail(-1, "face direction beyond the tree's dimension");
+    if (c->surr_n != c->nblk || c->nblk == 0) return fail(-1, "surrounding tables not set for this topology");
+    if (blocks_alloc < c->nblk || old_blocks_alloc < nold) return fail(-1, "block allocation below the block count");
+    const int n = c->nblk, nb = c->nb, nd = c->ndim;
+    const int nz = nd == 3 ? nb : 1, gz = nd == 3 ? ngrow : 0;
+    const long long ntile = (long long)(nb + 2 * ngrow + (fdir == 0)) * (nb + 2 * ngrow + (fdir == 1)) *
+                            (nz + 2 * gz + (fdir == 2));
+    const size_t lds_fill = (size_t)ntile * sizeof(double);
+    if (lds_fill > 64 * 1024) return fail(-1, "grown block above 64 KiB (ngrow too large for this block size)");
+    const size_t lds_face = (size_t)(nb + (fdir == 0)) * (nb + (fdir == 1)) * (nz + (fdir == 2)) * sizeof(double);
+    for (int q = 0; q < ncomp; q++)
+        if (prolong[q] < 0 || prolong[q] > 2) return fail(-1, "unknown prolongation method");
+    // per level: the parents of the level's new blocks (ascending) and the new blocks
+    std::vector<std::vector<int>> par(c->maxlevel + 1), nw(c->maxlevel + 1);
+    std::vector<char> mark(n, 0);
+    for (int b = 0; b < n; b++) {
+        if (origin[b] >= nold || origin[b] < -1) return fail(-1, "origin out of range");
+        if (origin[b] >= 0) continue;
+        const BlockInfo& bi = c->h_binfo[b];
+        if (bi.parent < 0 || bi.level < 2) return fail(-1, "a new block without a parent");
+        nw[bi.level].push_back(b);
+        if (!mark[bi.parent]) {
+            mark[bi.parent] = 1;
+            par[bi.level].push_back(bi.parent);
+        }
+    }
+    for (auto& v : par) std::sort(v.begin(), v.end());
+    if (fdir >= 0) {
+        // the reference fills no face data next to a wall (SayakaFillPatch.cpp:318-324)
+        for (int b = 0; b < n; b++)
+            for (int q = 0; q < 27; q++)
+                if (c->h_surr[(size_t)b * 27 + q] <= -20)
+                    return fail(-2, "face data remap on a tree with physical boundaries (periodic trees only)");
+    } else if (int e = setup_fill_bc(c, ncomp, 0, nphys, physbc, mathbc, bcval)) {
+        return e;
+    }
+    std::vector<int> L(origin, origin + n);
+    std::vector<int> po(c->maxlevel + 1, 0), pn(c->maxlevel + 1, 0), no(c->maxlevel + 1, 0), nn(c->maxlevel + 1, 0);
+    for (int l = 1; l <= c->maxlevel; l++) {
+        po[l] = (int)L.size();
+        pn[l] = (int)par[l].size();
+        L.insert(L.end(), par[l].begin(), par[l].end());
+        no[l] = (int)L.size();
+        nn[l] = (int)nw[l].size();
+        L.insert(L.end(), nw[l].begin(), nw[l].end());
+    }
+    HIPCHK(hipSetDevice(c->device));
+    if (L.size() > c->remap_cap) {
+        if (c->d_remap) HIPCHK(hipFree(c->d_remap));
+        HIPCHK(hipMalloc(&c->d_remap, L.size() * sizeof(int)));
+        c->remap_cap = L.size();
+    }
+    HIPCHK(hipMemcpyAsync(c->d_remap, L.data(), L.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const long long cstride = (long long)blocks_alloc * ntile;
+    FillArgs fa{};
+    fa.binfo = c->d_binfo;
+    fa.surr = c->d_surr;
+    fa.bcref = c->d_bcref;
+    fa.physcode = c->d_fill_int;
+    fa.bctype = c->d_fill_int + nphys;
+    fa.bcval = c->d_fill_val;
+    fa.state = state;
+    fa.comp_stride = cstride;
+    fa.ngrow = ngrow;
+    fa.nlayer = ngrow;
+    fa.scomp = 0;
+    fa.nc = ncomp;
+    fa.nphys = nphys;
+    fa.same_level_only = 1;
+    FaceFillArgs ffa{};
+    ffa.binfo = c->d_binfo;
+    ffa.surr = c->d_surr;
+    ffa.state = state;
+    ffa.comp_stride = cstride;
+    ffa.fd = fdir;
+    ffa.ngrow = ngrow;
+    ffa.nlayer = ngrow;
+    ffa.scomp = 0;
+    ffa.nc = ncomp;
+    ffa.same_level_only = 1;
+    ProlongNewArgs pa{};
+    pa.origin = c->d_remap;
+    pa.binfo = c->d_binfo;
+    pa.state = state;
+    pa.comp_stride = cstride;
+    pa.ngrow = ngrow;
+    pa.nc = ncomp;
+    pa.fd = fdir;
+    for (int q = 0; q < ncomp; q++) fa.prolong[q] = ffa.prolong[q] = pa.prolong[q] = prolong[q];
+    int e = dispatch(c, [&](auto I) {
+        return decltype(I)::state_remap(c, old_state, (long long)old_blocks_alloc * ntile, state, cstride, ntile,
+                                        ncomp, ngrow, fdir, po, pn, no, nn, fa, ffa, pa, lds_fill, lds_face);
+    });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));   // L goes out of scope
     return 0;
 }
 

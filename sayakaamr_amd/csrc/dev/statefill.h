@@ -43,6 +43,7 @@ struct FillArgs {
     double* state;          // component 0, block 0
     long long comp_stride;  // doubles per component
     int ngrow, nlayer, scomp, nc, nphys;
+    int same_level_only;    // fillBlockBoundary(.., onlySameLevel = 1): no coarse surroundings
     int prolong[FILL_MAXCOMP];
 };
 
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__(FILL_NT) k_state_fill(FillArgs a) {
             const int sv = s[fill_surr_flat(ii, jj, kk)];
             if (sv >= 0) {
                 tile[e] = base[(long long)sv * ntile + tix(i - ii * NB, j - jj * NB, k - kk * NZ)];
-            } else if (sv > PHYS_BNDRY && p >= 0) {
+            } else if (sv > PHYS_BNDRY && p >= 0 && !a.same_level_only) {
                 const double* par = base + (long long)p * ntile;
                 const int ic = fine_to_crse(i, off[0]), jc = fine_to_crse(j, off[1]);
                 const int kc = D == 3 ? fine_to_crse(k, off[2]) : k;
@@ -224,6 +225,7 @@ struct FaceFillArgs {
     double* state;          // component 0, block 0
     long long comp_stride;
     int fd, ngrow, nlayer, scomp, nc;
+    int same_level_only;    // no coarse surroundings (fillBlockBoundary's onlySameLevel)
     int prolong[FILL_MAXCOMP];
 };
 
@@ -288,7 +290,7 @@ __global__ void __launch_bounds__(FILL_NT) k_face_fill(FaceFillArgs a) {
         __syncthreads();
         // 2. coarse surroundings, one after another
         const int method = a.prolong[cc];
-        if (p >= 0) {
+        if (p >= 0 && !a.same_level_only) {
             const double* par = base + (long long)p * ntile;
             for (int kk = (D == 3 ? -1 : 0); kk <= (D == 3 ? 1 : 0); kk++)
                 for (int jj = -1; jj <= 1; jj++)
@@ -382,6 +384,176 @@ __global__ void __launch_bounds__(256) k_pack_levels(const int* __restrict__ lis
     for (int e = threadIdx.x; e < NCELL; e += 256) {
         const int i = e % NB, j = (e / NB) % NB, k = e / (NB * NB);
         dst[e] = src[((k + gz) * gx + (j + ngrow)) * gx + (i + ngrow)];
+    }
+}
+
+// ----------------------------------------------------------------- registered-state remap
+// A state registered with the tree through AmrTree::regrid and initNewBlockDataAfterRegrid
+// (SayakaRegrid.cpp:530-591, 987-1105).  The host runs, on the new tree:
+//   k_state_move      every block's whole grown box from its old position (origin), 0 for a
+//                     new block (the packing moves whole blocks; new slots hold zeros);
+//   then per level L holding new blocks, coarsest first:
+//   k_state_fill / k_face_fill with same_level_only over the parents of level L's new blocks
+//                     (fillBlockBoundary(.., onlySameLevel = 1): same level, physical BC);
+//   k_prolong_new / k_prolong_new_face over level L's new blocks: the valid box from the
+//                     parent (prolongBlock, SayakaFillPatch.cpp:464-483), and for face data
+//                     the shared faces of existing same-level face neighbours
+//                     (correctFaceBlockData, SayakaFillPatch.cpp:485-513).
+__global__ void __launch_bounds__(256) k_state_move(const int* __restrict__ origin, const double* __restrict__ old,
+                                                    long long old_cstride, double* __restrict__ st,
+                                                    long long cstride, long long ntile) {
+    const int b = blockIdx.x, c = blockIdx.y;
+    const int o = origin[b];
+    double* dst = st + c * cstride + (long long)b * ntile;
+    if (o >= 0) {
+        const double* src = old + c * old_cstride + (long long)o * ntile;
+        for (long long e = threadIdx.x; e < ntile; e += 256) dst[e] = src[e];
+    } else {
+        for (long long e = threadIdx.x; e < ntile; e += 256) dst[e] = 0.0;
+    }
+}
+
+struct ProlongNewArgs {
+    const int* list;        // new blocks of one level
+    const int* origin;      // per block: old id, -1 new
+    const BlockInfo* binfo;
+    double* state;
+    long long comp_stride;
+    int ngrow, nc, fd;
+    int prolong[FILL_MAXCOMP];
+};
+
+// cell data: every valid cell of the new block from its parent's grown box, the arithmetic of
+// the ghost prolongation in k_state_fill (prolongCellBlockData_*, SayakaInterpPatch.cpp:157-391)
+template <int D, int NB>
+__global__ void __launch_bounds__(FILL_NT) k_prolong_new(ProlongNewArgs a) {
+    constexpr int NZ = D == 3 ? NB : 1;
+    constexpr int NCELL = NB * NB * NZ;
+    const int b = a.list[blockIdx.x];
+    const int g = a.ngrow, gz = D == 3 ? g : 0;
+    const int GX = NB + 2 * g;
+    const long long ntile = (long long)GX * GX * (NZ + 2 * gz);
+    const int p = a.binfo[b].parent, wc = a.binfo[b].which_child;
+    const int off[3] = {side_of(wc, 0) * (NB / 2), side_of(wc, 1) * (NB / 2), D == 3 ? side_of(wc, 2) * (NB / 2) : 0};
+    auto tix = [&](int i, int j, int k) { return ((k + gz) * GX + (j + g)) * GX + (i + g); };
+    for (int cc = 0; cc < a.nc; cc++) {
+        double* base = a.state + cc * a.comp_stride;
+        const double* par = base + (long long)p * ntile;
+        double* blk = base + (long long)b * ntile;
+        const int method = a.prolong[cc];
+        for (int e = threadIdx.x; e < NCELL; e += FILL_NT) {
+            const int i = e % NB, j = (e / NB) % NB, k = e / (NB * NB);
+            const int ic = fine_to_crse(i, off[0]), jc = fine_to_crse(j, off[1]);
+            const int kc = D == 3 ? fine_to_crse(k, off[2]) : k;
+            const double c = par[tix(ic, jc, kc)];
+            double v = c;
+            if (method != PR_INJECTION) {
+                double gr[3] = {0.0, 0.0, 0.0};
+                const int ci[3] = {ic, jc, kc};
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    int up[3] = {ci[0], ci[1], ci[2]}, dn[3] = {ci[0], ci[1], ci[2]};
+                    up[d]++;
+                    dn[d]--;
+                    const double vu = par[tix(up[0], up[1], up[2])];
+                    const double vd = par[tix(dn[0], dn[1], dn[2])];
+                    gr[d] = method == PR_CENTERED_LINEAR ? 0.5 * (vu - vd) : mc_slope(vd, c, vu);
+                }
+                const double irel = ((ic - off[0]) * 2 == i) ? -0.25 : 0.25;
+                const double jrel = ((jc - off[1]) * 2 == j) ? -0.25 : 0.25;
+                const double krel = D == 3 ? (((kc - off[2]) * 2 == k) ? -0.25 : 0.25) : 0.0;
+                v = c + irel * gr[0] + jrel * gr[1] + krel * gr[2];
+            }
+            blk[tix(i, j, k)] = v;
+        }
+    }
+}
+
+// face data of direction fd: the valid face box of the new block staged in LDS, prolonged
+// (prolongFaceBlockData_*, SayakaInterpPatch.cpp:447-591; the centred-linear second sweep
+// after a barrier), the faces shared with existing same-level face neighbours replaced by the
+// neighbour's copy, then written back
+template <int D, int NB>
+__global__ void __launch_bounds__(FILL_NT) k_prolong_new_face(ProlongNewArgs a) {
+    extern __shared__ double vt[];
+    constexpr int NZ = D == 3 ? NB : 1;
+    const int b = a.list[blockIdx.x];
+    const int g = a.ngrow, gz = D == 3 ? g : 0, fd = a.fd;
+    const int V[3] = {NB + (fd == 0), NB + (fd == 1), NZ + (fd == 2)};
+    const int E0 = V[0] + 2 * g, E1 = V[1] + 2 * g, E2 = V[2] + 2 * gz;
+    const long long ntile = (long long)E0 * E1 * E2;
+    const int nv = V[0] * V[1] * V[2];
+    const int p = a.binfo[b].parent, wc = a.binfo[b].which_child;
+    const int off[3] = {side_of(wc, 0) * (NB / 2), side_of(wc, 1) * (NB / 2), D == 3 ? side_of(wc, 2) * (NB / 2) : 0};
+    auto tix = [&](int i, int j, int k) { return ((k + gz) * E1 + (j + g)) * E0 + (i + g); };
+    auto vix = [&](int i, int j, int k) { return (k * V[1] + j) * V[0] + i; };
+    for (int cc = 0; cc < a.nc; cc++) {
+        double* base = a.state + cc * a.comp_stride;
+        const double* par = base + (long long)p * ntile;
+        double* blk = base + (long long)b * ntile;
+        const int method = a.prolong[cc];
+        for (int e = threadIdx.x; e < nv; e += FILL_NT) {
+            const int f[3] = {e % V[0], (e / V[0]) % V[1], e / (V[0] * V[1])};
+            int c3[3];
+            for (int d = 0; d < 3; d++) c3[d] = d < D ? fine_to_crse(f[d], off[d]) : f[d];
+            const int side = (f[fd] + 64) & 1;
+            const double c = par[tix(c3[0], c3[1], c3[2])];
+            if (method == PR_INJECTION) {
+                double v = c;
+                if (side) {
+                    int u[3] = {c3[0], c3[1], c3[2]};
+                    u[fd]++;
+                    v = 0.5 * (c + par[tix(u[0], u[1], u[2])]);
+                }
+                vt[e] = v;
+            } else if (side == 0) {
+                double gr[3] = {0.0, 0.0, 0.0}, rel[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < D; d++) {
+                    if (d == fd) continue;
+                    int up[3] = {c3[0], c3[1], c3[2]}, dn[3] = {c3[0], c3[1], c3[2]};
+                    up[d]++;
+                    dn[d]--;
+                    const double vu = par[tix(up[0], up[1], up[2])], vd = par[tix(dn[0], dn[1], dn[2])];
+                    gr[d] = method == PR_CENTERED_LINEAR ? 0.5 * (vu - vd) : mc_slope(vd, c, vu);
+                    rel[d] = ((c3[d] - off[d]) * 2 == f[d]) ? -0.25 : 0.25;
+                }
+                vt[e] = c + rel[0] * gr[0] + rel[1] * gr[1] + rel[2] * gr[2];
+            }
+        }
+        __syncthreads();
+        if (method != PR_INJECTION) {
+            for (int e = threadIdx.x; e < nv; e += FILL_NT) {
+                const int f[3] = {e % V[0], (e / V[0]) % V[1], e / (V[0] * V[1])};
+                if (((f[fd] + 64) & 1) == 0) continue;
+                int m[3] = {f[0], f[1], f[2]}, q[3] = {f[0], f[1], f[2]};
+                m[fd]--;
+                q[fd]++;
+                vt[e] = 0.5 * (vt[vix(m[0], m[1], m[2])] + vt[vix(q[0], q[1], q[2])]);
+            }
+            __syncthreads();
+        }
+        for (int side = 0; side < 2; side++) {
+            const int j = a.binfo[b].nbr[2 * fd + side];
+            if (j < 0 || a.origin[j] < 0) continue;   // uniform over the workgroup
+            const double* nb_ = base + (long long)j * ntile;
+            const int dl = side == 0 ? 0 : V[fd] - 1, sl = side == 0 ? V[fd] - 1 : 0;
+            const int t0 = fd == 0 ? 1 : 0, t1 = fd == 2 ? 1 : 2;   // the two tangential directions
+            const int n0 = V[t0], n1 = V[t1];
+            for (int e = threadIdx.x; e < n0 * n1; e += FILL_NT) {
+                int dsti[3], srci[3];
+                dsti[fd] = dl;
+                srci[fd] = sl;
+                dsti[t0] = srci[t0] = e % n0;
+                dsti[t1] = srci[t1] = e / n0;
+                vt[vix(dsti[0], dsti[1], dsti[2])] = nb_[tix(srci[0], srci[1], srci[2])];
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < nv; e += FILL_NT) {
+            const int f[3] = {e % V[0], (e / V[0]) % V[1], e / (V[0] * V[1])};
+            blk[tix(f[0], f[1], f[2])] = vt[e];
+        }
+        __syncthreads();
     }
 }
 

@@ -66,6 +66,7 @@ _SIG = {
     "sayaka_dev_state_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp], i32),
     "sayaka_dev_face_fill": ([vp, vp, i32, i32, i32, i32, i32, i32, i32, vp], i32),
     "sayaka_dev_pack_levels": ([vp, vp, i32, i32, i32, vp], i32),
+    "sayaka_dev_state_remap": ([vp, vp, i32, i32, vp, i32, i32, i32, i32, vp, vp, i32, vp, vp, vp], i32),
     "sayaka_dev_set_mixed_smoother": ([vp, i32], i32),
     "sayaka_cnn_setup": ([vp, i32], i32),
     "sayaka_cnn_set_params": ([vp, vp], i32),
@@ -155,6 +156,25 @@ class DeviceState:
             self.s.state_fill(self.ptr, self.ncomp, self.blocks, self.ngrow, nlayer, prolong, bcmap, scomp, nc)
         else:
             self.s.face_fill(self.ptr, self.ncomp, self.blocks, self.fdir, self.ngrow, nlayer, prolong, scomp, nc)
+
+    def remap_from(self, old, origin, prolong, bcmap=None):
+        """This state (sized for the tree after a regrid) from `old` (the same state on the tree
+        before it): AmrTree::regrid's packing and initNewBlockDataAfterRegrid
+        (SayakaRegrid.cpp:530-591, 987-1105).  The solver must hold the new tree's topology and
+        surroundings; origin = AmrTree.regrid_origin(); prolong[c] per component; bcmap as for
+        fill (cell data)."""
+        if old.fdir != self.fdir or old.ncomp != self.ncomp or old.ngrow != self.ngrow:
+            raise ValueError("old and new state differ in location, components or ghost layers")
+        org = np.ascontiguousarray(origin, dtype=np.int32)
+        pr = np.ascontiguousarray(prolong[:self.ncomp], dtype=np.int32)
+        if self.fdir is None:
+            nphys, pc, mb, bv = self.s._bc_tables(bcmap or [{}] * self.ncomp, self.ncomp)
+        else:
+            nphys, pc, mb, bv = 0, np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1)
+        _chk(self.s.lib.sayaka_dev_state_remap(self.s.h, C.c_void_p(old.ptr), old.blocks, old.blocks,
+                                               C.c_void_p(self.ptr), self.ncomp, self.blocks, self.ngrow,
+                                               -1 if self.fdir is None else self.fdir, N.ptr(org), N.ptr(pr),
+                                               nphys, N.ptr(pc), N.ptr(mb), N.ptr(bv)), "state_remap")
 
     def pack_levels(self):
         """Valid cells in the Chombo file's order (per level, block, component), on the host:
@@ -298,13 +318,9 @@ class MacSolverGPU:
         r = np.ascontiguousarray(tree["surr_bcref"], dtype=np.int32)
         _chk(self.lib.sayaka_dev_set_surroundings(self.h, self.n, N.ptr(s), N.ptr(r)), "set_surroundings")
 
-    def state_fill(self, state_ptr, ncomp, blocks_alloc, ngrow, nlayer, prolong, bcmap, scomp=0, nc=None):
-        """Ghost fill of the device state at state_ptr ([ncomp][blocks_alloc][grown box] doubles,
-        e.g. DeviceState.ptr).
-        prolong[c]: method of component scomp+c (0 injection, 1 centred linear, 2 limited);
-        bcmap[c]: {physical code: (math BC, value)} of component scomp+c, the same on every
-        surrounding position (BCRegister filled position by position the way the drivers do)."""
-        nc = ncomp - scomp if nc is None else nc
+    @staticmethod
+    def _bc_tables(bcmap, nc):
+        """BC register arrays of the C ABI from per-component {physical code: (math BC, value)}."""
         codes = sorted({int(k) for m in bcmap[:nc] for k in m})
         nphys = len(codes)
         mb = np.full((nc, 27, max(nphys, 1)), -1, np.int32)
@@ -313,8 +329,18 @@ class MacSolverGPU:
             for h, code in enumerate(codes):
                 if code in bcmap[c]:
                     mb[c, :, h], bv[c, :, h] = bcmap[c][code]
-        pr = np.ascontiguousarray(prolong[:nc], dtype=np.int32)
         pc = np.array(codes if codes else [0], dtype=np.int32)
+        return nphys, pc, mb, bv
+
+    def state_fill(self, state_ptr, ncomp, blocks_alloc, ngrow, nlayer, prolong, bcmap, scomp=0, nc=None):
+        """Ghost fill of the device state at state_ptr ([ncomp][blocks_alloc][grown box] doubles,
+        e.g. DeviceState.ptr).
+        prolong[c]: method of component scomp+c (0 injection, 1 centred linear, 2 limited);
+        bcmap[c]: {physical code: (math BC, value)} of component scomp+c, the same on every
+        surrounding position (BCRegister filled position by position the way the drivers do)."""
+        nc = ncomp - scomp if nc is None else nc
+        nphys, pc, mb, bv = self._bc_tables(bcmap, nc)
+        pr = np.ascontiguousarray(prolong[:nc], dtype=np.int32)
         _chk(self.lib.sayaka_dev_state_fill(self.h, C.c_void_p(int(state_ptr)), ncomp, blocks_alloc, ngrow, scomp,
                                             nc, nlayer, N.ptr(pr), nphys, N.ptr(pc), N.ptr(mb), N.ptr(bv)),
              "state_fill")

@@ -326,7 +326,69 @@ def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     print(f"{name}_flux: {n} blocks, wrote {len(out)} arrays")
 
 
+def state_remap(name):
+    """Regrids of the case's moving refinement region with a registered state (cell, or one
+    face state per direction): per regrid the flags and, after regrid +
+    initNewBlockDataAfterRegrid, the whole grown state of the blocks the remap writes (new
+    blocks and parents of new blocks).  Every other block must hold its moved data; that is
+    asserted here against the reference's output.  The product tree runs in lockstep (same
+    tables) and gives each new block's origin."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import golden_cases as GC
+    from sayakaamr_amd.tree import AmrTree
+    spec = GC.remap_spec(name)
+    nd, nb = spec.ndim, spec.nb
+    nsteps = GC.REMAP_CASES[name][5]
+    dirs = GC.REMAP_CASES[name][3]
+    out = {}
+    for fd in (dirs if dirs is not None else (None,)):
+        rt = R.RefTree(nd)
+        W.build_tree(rt, spec, GC.remap_criterion(name, 0))
+        tb = AmrTree(nd, max_blocks=4000)
+        W.build_tree(tb, spec, GC.remap_criterion(name, 0))
+        prolong = GC.remap_prolong(name, fd)
+        if fd is None:
+            maps = GC.remap_bcmaps(name)
+            codes = list(GC.FILL_WALLS)
+            mb = [maps[c][k][0] for c in range(len(prolong)) for k in codes]
+            bv = [maps[c][k][1] for c in range(len(prolong)) for k in codes]
+            rt.state_create_general(-1, len(prolong), GC.FILL_NGROW, prolong, codes, mb, bv)
+        else:
+            rt.state_create_general(fd, len(prolong), GC.FILL_NGROW, prolong)
+        tag = "c" if fd is None else f"d{fd}"
+        for step in range(nsteps):
+            t = rt.export()
+            tp = tb.export()
+            for k in W.TREE_FIELDS:
+                assert np.array_equal(t[k], tp[k]), k
+            pre = GC.remap_input(name, t, step, fd)
+            rt.state_set_grown(pre, nb)
+            refine, coarsen = GC.remap_flags(t, spec, GC.remap_criterion(name, step + 1))
+            for x in (rt, tb):
+                x.set_refine_flags(refine)
+                x.set_coarsen_flags(coarsen)
+                x.regrid()
+            rt.init_new_data()
+            after = rt.export()
+            origin = tb.regrid_origin()
+            post = rt.state_get_grown(nb)
+            touched = GC.remap_touched(after, origin)
+            keep = ~touched
+            assert np.array_equal(post[:, keep], pre[:, origin[keep]]), "untouched blocks hold moved data"
+            out[f"{tag}_step{step}_refine"] = refine
+            out[f"{tag}_step{step}_coarsen"] = coarsen
+            out[f"{tag}_step{step}_touched"] = np.nonzero(touched)[0].astype(np.int32)
+            out[f"{tag}_step{step}_post"] = post[:, touched]
+            print(f"{name} {tag} step {step}: {after['level'].shape[0]} blocks, {int((origin < 0).sum())} new, "
+                  f"{int(touched.sum())} touched")
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "sremap":
+        for name in ("sremap2", "sremap3", "fremap2", "fremap3"):
+            state_remap(name)
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "fill":
         state_fill("fill2")
         state_fill("fill3")

@@ -143,12 +143,19 @@ def face_input(name, tree, fdir):
     from a hash of (component, level, global face position with periodic wrap, direction),
     ghost faces a distinct sentinel each."""
     nd, nb = FACE_CASES[name][:2]
+    return consistent_faces(tree, face_spec(name), fdir, len(face_prolong(name, fdir)), 0)
+
+
+def consistent_faces(tree, spec, fdir, ncomp, salt):
+    """Face data of direction fdir on tree: valid faces U(-1,1) from a hash of (salt,
+    component, level, global face position with periodic wrap, direction), ghost faces a
+    distinct sentinel each."""
+    nd, nb = spec.ndim, spec.nb
     g = FILL_NGROW
-    ncomp = len(face_prolong(name, fdir))
     n = tree["level"].shape[0]
     shp = face_shape(nd, nb, g, fdir)
     out = -3000.0 - np.arange(ncomp * n * int(np.prod(shp)), dtype=np.float64).reshape((ncomp, n) + shp) * 1e-3
-    spec = face_spec(name)
+    out -= 100.0 * salt
     gz = g if nd == 3 else 0
     for b in range(n):
         lvl = int(tree["level"][b])
@@ -164,7 +171,7 @@ def face_input(name, tree, fdir):
             key = key * np.uint64(1 << 20) + gpos[d].astype(np.uint64)
         key = key * np.uint64(64) + np.uint64(lvl * 4 + fdir)
         for c in range(ncomp):
-            h = W._splitmix64(key + np.uint64(0x9E37 * (c + 1)))
+            h = W._splitmix64(key + np.uint64(0x9E37 * (c + 1)) + np.uint64(salt) * np.uint64(0x10000001))
             u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
             if nd == 3:
                 out[c, b, gz:gz + ext[2], g:g + ext[1], g:g + ext[0]] = u
@@ -183,3 +190,87 @@ def face_ghost_mask(nd, nb, g, fdir):
     else:
         m[0, g:g + ext[1], g:g + ext[0]] = False
     return m
+
+
+# ------------------------------------------------------ registered-state regrid remap cases
+# A state registered with the tree (TreeStateData) through a sequence of regrids of a moving
+# ring / shell: regrid packing, then initNewBlockDataAfterRegrid (in-level fill of the parents
+# of new blocks, prolongation of the new blocks, face correction from existing neighbours).
+# Cell states on a tree whose walls each have their own code, face states on a periodic tree.
+REMAP_CASES = {
+    # name -> (ndim, nb, maxlevel, face directions (None: cell data), prolong per comp, regrids)
+    "sremap2": (2, 8, 4, None, (0, 1, 2), 2),
+    "sremap3": (3, 4, 3, None, (1, 2), 2),
+    "fremap2": (2, 8, 4, (0, 1), (0, 1), 2),
+    "fremap3": (3, 4, 3, (0, 1, 2), None, 2),
+}
+
+
+def remap_spec(name):
+    import dataclasses
+    nd, nb, maxlevel, dirs, _, _ = REMAP_CASES[name]
+    bc = FILL_WALLS if dirs is None else (1,) * 6
+    return dataclasses.replace(W.small_spec(nd, maxlevel, nb=nb), bc=bc, ngrow=FILL_NGROW)
+
+
+def remap_criterion(name, step):
+    """The refinement region of regrid `step` (step 0: the initial tree)."""
+    x = 0.35 + 0.05 * step
+    if REMAP_CASES[name][0] == 2:
+        return W.ring_criterion(centre=(x, 0.5), r0=0.2, half=0.04)
+    return W.shell_criterion(centre=(x, 0.5, 0.5), r0=0.18)
+
+
+def remap_flags(tree, spec, crit):
+    """Refine the leaves the criterion hits, coarsen the other leaves above minAmrLevel."""
+    leaf_or_parent = (tree["node_type"] == 1) | (tree["node_type"] == 2)
+    hit = np.zeros(tree["level"].shape[0], bool)
+    cand = np.nonzero(leaf_or_parent)[0]
+    hit[cand] = crit(tree, spec.nb, spec.ndim, cand)
+    refine = (hit & (tree["node_type"] == 1)).astype(np.int32)
+    coarsen = ((~hit) & (tree["node_type"] == 1) & (tree["level"] > spec.minlevel)).astype(np.int32)
+    return refine, coarsen
+
+
+def remap_prolong(name, fdir=None):
+    p = REMAP_CASES[name][4]
+    return p if p is not None else ((1, 2, 0)[fdir],)
+
+
+def remap_bcmaps(name):
+    """Per component {wall code: (math BC, value)} (cell cases), from the fill cases' mixes."""
+    src = FILL_COMPS["fill2"]
+    ncomp = len(remap_prolong(name))
+    return [{code: (src["mathbc"][c][q], src["bcval"][c][q]) for q, code in enumerate(FILL_WALLS)}
+            for c in range(ncomp)]
+
+
+def remap_input(name, tree, step, fdir=None):
+    """The state before regrid `step`: cell data U(-1,1) in valid cells and a distinct
+    sentinel in every ghost; face data as consistent_faces with the step as salt."""
+    spec = remap_spec(name)
+    nd, nb = spec.ndim, spec.nb
+    if fdir is not None:
+        return consistent_faces(tree, spec, fdir, len(remap_prolong(name, fdir)), step + 1)
+    g = FILL_NGROW
+    ncomp = len(remap_prolong(name))
+    n = tree["level"].shape[0]
+    nz, gz = (nb, g) if nd == 3 else (1, 0)
+    shp = (ncomp, n, nz + 2 * gz, nb + 2 * g, nb + 2 * g)
+    idx = np.arange(np.prod(shp), dtype=np.uint64).reshape(shp)
+    h = W._splitmix64(idx + np.uint64(0x7E3A0000) + np.uint64(step) * np.uint64(0x100000000))
+    u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+    out = -2000.0 - 100.0 * step - idx.astype(np.float64) * 1e-3
+    sl = (slice(None), slice(None), slice(gz, gz + nz), slice(g, g + nb), slice(g, g + nb))
+    out[sl] = u[sl]
+    return out
+
+
+def remap_touched(tree, origin):
+    """Blocks initNewBlockDataAfterRegrid writes: the new blocks and the parents of new
+    blocks (their ghosts are refilled in-level); every other block holds its moved data."""
+    new = origin < 0
+    par = np.zeros_like(new)
+    p = tree["parent"][new]
+    par[p[p >= 0]] = True
+    return new | par
