@@ -256,11 +256,14 @@ def learned_leg(D, n_test):
         dev.close()
 
 
-def output_leg(dev, D, W, spec, tree, tb, sol):
-    """After the solve, the user-side path of the reference's drivers: the solution into a
-    cell state with 2 ghost layers, FillPatch::fillBoundary (centred-linear prolongation,
-    Neumann walls), the valid cells packed in the Chombo file's order on the device, and the
-    HDF5 file written (WriteDataHdf5).  The pack kernel moves 16 B per cell (read + write)."""
+def output_leg(D, W, spec, tree, tb, sol):
+    """After the solve, the user-side path of the reference's drivers on the benchmark tree: the
+    solution into a cell state with 2 ghost layers, FillPatch::fillBoundary (centred-linear
+    prolongation, Neumann walls), the valid cells packed in the Chombo file's order on the
+    device and the HDF5 file written (WriteDataHdf5); then one regrid pass (the sphere moved by
+    0.03 in x) with the state carried over (packing + initNewBlockDataAfterRegrid on the
+    device).  The pack kernel moves 16 B per cell (read + write).  Own device context, sized
+    for the regridded tree."""
     import tempfile
 
     from sayakaamr_amd import output
@@ -268,12 +271,15 @@ def output_leg(dev, D, W, spec, tree, tb, sol):
     n = tree["level"].shape[0]
     nb, g = spec.nb, 2
     bcmap = [{W.WALL: (W.NEUMANN, 0.0)}]
+    dev = D.MacSolverGPU(3, nb, 2 * n, device=0)
     st = D.DeviceState(dev, 1, g, n)
+    st2 = None
     try:
+        dev.prepare(tree, spec)
+        dev.set_surroundings(tree)
         host = np.zeros(st.shape)
         host[0, :, g:g + nb, g:g + nb, g:g + nb] = sol.reshape(n, nb, nb, nb)
         st.upload(host)
-        dev.set_surroundings(tree)
         st.fill(g, [1], bcmap)   # warm
         t0 = time.perf_counter()
         st.fill(g, [1], bcmap)
@@ -288,12 +294,29 @@ def output_leg(dev, D, W, spec, tree, tb, sol):
             output.write_chombo(tb, packed, 1, path, names=["pressure"])
             write_ms = (time.perf_counter() - t0) * 1e3
             size = os.path.getsize(path)
+        # one regrid pass for the sphere moved by 0.03 in x (the moving cycle's flag rule)
+        tb.flag_shell(np.array([0.53, 0.5, 0.5]), 0.2, W.SHELL_FACTOR)
+        tb.regrid()
+        after = tb.export()
+        origin = tb.regrid_origin()
+        n2 = after["level"].shape[0]
+        dev.prepare(after, spec)
+        dev.set_surroundings(after)
+        st2 = D.DeviceState(dev, 1, g, n2)
+        st2.remap_from(st, origin, [1], bcmap)   # warm
+        t0 = time.perf_counter()
+        st2.remap_from(st, origin, [1], bcmap)
+        remap_ms = (time.perf_counter() - t0) * 1e3
     finally:
         st.close()
+        if st2 is not None:
+            st2.close()
+        dev.close()
     return dict(blocks=n, cells=int(packed.size), ghost_fill_ms=round(fill_ms, 3),
                 pack_and_copy_ms=round(pack_ms, 3), pack_alg_bytes=int(packed.size) * 16,
                 write_ms=round(write_ms, 3), file_bytes=size,
-                write_gbs=round(size / (write_ms * 1e-3) / 1e9, 3))
+                write_gbs=round(size / (write_ms * 1e-3) / 1e9, 3),
+                regrid_blocks=n2, regrid_new_blocks=int((origin < 0).sum()), state_remap_ms=round(remap_ms, 3))
 
 
 def large_leg(D, W):
@@ -512,9 +535,9 @@ def main():
 
     mixed_rows, ms_mixed = mixed_leg(dev, D, W, spec, tree, inp, args.seeds, args.steps)
     rmx, classes_mixed = class_profile(dev, D, D.MIXED)
-    out_leg = output_leg(dev, D, W, spec, tree, tb, sol64)
-    log(f"output: {out_leg}")
     dev.close()
+    out_leg = output_leg(D, W, spec, tree, tb, sol64)
+    log(f"output: {out_leg}")
 
     learned = None if args.no_learned else learned_leg(D, args.learned)
     if learned:
