@@ -1,0 +1,66 @@
+// fill_patch.h -- ghost fill of a caller's TreeData on the GPU (the reference's user fills).
+//
+// Mirrors InterpPatch::ProlongMethod (SayakaInterpPatch.h:78-83), BCPatch
+// (SayakaBoundaryPatch.h:177-260: one BCRegister per component) and FillPatch
+// (SayakaFillPatch.h:60-200: setBC, setProlongation, fillBoundary), so that a driver's
+//     FillPatch fp(tree, data); fp.setBC(bc); fp.setProlongation(c, m); fp.fillBoundary(0, n, 2);
+// compiles unchanged.  fillBoundary runs on the device (sayaka_dev_state_fill for cell data,
+// sayaka_dev_face_fill for face data): the tree's topology and surroundings and the data's
+// grown boxes go up, the filled ghosts come back.  The result is the reference's
+// FillPatch::fillBoundary (SayakaFillPatch.cpp:17-368) bit for bit.  Errors throw
+// std::runtime_error where the reference exits (an unmapped wall code, face data next to a wall).
+#pragma once
+
+#include <vector>
+
+#include "bc_register.h"
+#include "tree_data.h"
+
+struct sayaka_dev_ctx;
+
+namespace sayaka {
+inline namespace amd {
+
+struct InterpPatch {
+    enum ProlongMethod {
+        PROLONG_IGNORE = -1,
+        PROLONG_INJECTION = 0,
+        PROLONG_CENTERED_LINEAR,
+        PROLONG_CENTERED_LINEAR_LIMITED,
+    };
+};
+
+struct BCPatch {
+    std::vector<BCRegister> regs;
+    void defineByData(const TreeData& data) { regs.assign(data.numComp(), BCRegister{}); }
+    BCRegister& boundaryRegister(int comp) { return regs.at(comp); }
+    const BCRegister& boundaryRegister(int comp) const { return regs.at(comp); }
+};
+
+class FillPatch {
+   public:
+    // The tree and the data must outlive the FillPatch.  A device context of the given GPU is
+    // made on the first fill and kept.
+    FillPatch(const AmrTree& tree, TreeData& data, int device = 0);
+    ~FillPatch();
+    FillPatch(const FillPatch&) = delete;
+    FillPatch& operator=(const FillPatch&) = delete;
+
+    void setBC(const BCPatch& bc) { bc_ = &bc; }
+    // default: injection for every component (SayakaInterpPatch.h:109)
+    void setProlongation(int comp, int prolong);
+    // FillPatch::fillBoundary (SayakaFillPatch.cpp:17-30): levels 1..finest, every block
+    void fillBoundary(int scomp, int ncomp, int nlayer);
+
+   private:
+    void prepareDevice();
+    const AmrTree* tree_;
+    TreeData* data_;
+    const BCPatch* bc_ = nullptr;
+    std::vector<int> prolong_;
+    int device_;
+    sayaka_dev_ctx* ctx_ = nullptr;
+};
+
+}  // namespace amd
+}  // namespace sayaka

@@ -51,6 +51,8 @@ class TreeData {
     // Component comp of blocks [0, nblk), grown region included, to a packed array
     // [blk][grown box] (the layout sayaka_dev_fix_bc reads).
     void packGrown(int comp, int nblk, double* out) const;
+    void unpackGrown(int comp, int nblk, const double* in);
+    std::size_t grownSize() const { return compsize_; }   // values per block and component
 
    private:
     std::size_t offset(int blk, int i, int j, int k, int comp) const {

@@ -24,7 +24,7 @@ ARCH = "gfx950"
 DEV_SOURCES = ["dev/sayaka_dev.hip"]
 DEV_HEADERS = ["dev/dev_common.h", "dev/kernels.h", "dev/bottom.h", "dev/coef.h", "dev/mixed.h"]
 HOST_SOURCES = ["host/amr_tree.cpp", "host/capi_tree.cpp", "host/tree_data.cpp", "host/mac_solver.cpp",
-                "host/capi_mac.cpp", "host/h5lite.cpp", "host/writer_hdf5.cpp"]
+                "host/capi_mac.cpp", "host/h5lite.cpp", "host/writer_hdf5.cpp", "host/fill_patch.cpp"]
 DRIVERS = {"sayaka_c1_solve": "drivers/c1_solve.cpp"}
 
 

@@ -9,7 +9,10 @@
 // fractional ones reach 1e-10.)  With an output path, the FP64 solution is written as a
 // Chombo HDF5 file (WriteDataHdf5, as the reference's demos write theirs), and beside it
 // <path>.blocks (per block: level as a double, lo[3], hi[3], blockLength[3]) and <path>.phi
-// (per block its valid cells, TreeData::packValid order) for checking the file.
+// (per block its valid cells, TreeData::packValid order) for checking the file; then the
+// solution is copied into a state with 2 ghost layers and filled with FillPatch (centred-linear
+// prolongation, Neumann walls on every surrounding, as the demos fill their states,
+// Test5.h:194-214), written as <path>.ghost (per block its grown box).
 //
 //   sayaka_c1_solve [maxlevel=4] [seed=0] [output.h5]
 #include <chrono>
@@ -23,6 +26,7 @@
 #include "amr_tree.h"
 #include "bc_register.h"
 #include "mac_solver.h"
+#include "fill_patch.h"
 #include "tree_data.h"
 #include "writer.h"
 
@@ -86,6 +90,35 @@ static void refine_level(AmrTree& t, int level) {
         std::fprintf(stderr, "regrid: block pool overflow\n");
         std::exit(1);
     }
+}
+
+static int write_filled(const AmrTree& tree, const TreeData& phi, const char* path, int wall) {
+    TreeData st = TreeData::MakeCellData(tree, 1, 2);
+    const int nb = tree.nb;
+    for (int b = 0; b < tree.numBlocks; b++)
+        for (int k = 0; k < nb; k++)
+            for (int j = 0; j < nb; j++)
+                for (int i = 0; i < nb; i++) st(b, i, j, k, 0) = phi(b, i, j, k, 0);
+    BCPatch bc;
+    bc.defineByData(st);
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) bc.boundaryRegister(0).setBCMap(i, j, k, wall, Neumann);
+    FillPatch fp(tree, st);
+    fp.setBC(bc);
+    fp.setProlongation(0, InterpPatch::PROLONG_CENTERED_LINEAR);
+    fp.fillBoundary(0, 1, 2);
+    std::vector<double> grown((size_t)tree.numBlocks * st.grownSize());
+    st.packGrown(0, tree.numBlocks, grown.data());
+    const std::string p = std::string(path) + ".ghost";
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f || std::fwrite(grown.data(), sizeof(double), grown.size(), f) != grown.size()) {
+        std::fprintf(stderr, "cannot write %s\n", p.c_str());
+        if (f) std::fclose(f);
+        return -1;
+    }
+    std::fclose(f);
+    return 0;
 }
 
 static int write_output(const AmrTree& tree, const TreeData& phi, const char* path) {
@@ -257,7 +290,8 @@ int main(int argc, char** argv) {
         status[pass] = mac.lastStats().status;
         if (ret != 0) std::printf("cg_solve returned %d\n", ret);
         mac.getSol(phi, 0);
-        if (pass == 0 && argc > 3 && write_output(tree, phi, argv[3]) != 0) return 3;
+        if (pass == 0 && argc > 3 && (write_output(tree, phi, argv[3]) != 0 || write_filled(tree, phi, argv[3], WALL) != 0))
+            return 3;
         auto& s = sols[pass];
         for (int b = 0; b < tree.numBlocks; b++)
             if (tree[b].isLeaf())

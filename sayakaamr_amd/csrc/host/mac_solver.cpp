@@ -3,7 +3,9 @@
 
 #include <cstdio>
 #include <stdexcept>
+#include <string>
 
+#include "dev_tree.h"
 #include "sayaka_amd.h"
 
 namespace sayaka {
@@ -23,24 +25,45 @@ void MacSolver::check(int rc, const char* what) const {
     if (rc < 0) throw std::runtime_error(std::string(what) + ": " + sayaka_dev_last_error());
 }
 
+void upload_tree_topology(sayaka_dev_ctx* ctx, const AmrTree& t) {
+    const int n = t.numBlocks;
+    std::vector<int> level(n), parent(n), wc(n), child(8 * n), nbr(6 * n), type(n);
+    std::vector<double> len(3 * n);
+    for (int b = 0; b < n; b++) {
+        const AmrTreeNode& nd = t[b];
+        level[b] = nd.refineLevel;
+        parent[b] = nd.parent;
+        wc[b] = nd.whichChild;
+        type[b] = nd.nodeType;
+        for (int q = 0; q < 8; q++) child[8 * b + q] = nd.child[q];
+        for (int f = 0; f < 6; f++) nbr[6 * b + f] = nd.neighbor[f];
+        for (int d = 0; d < 3; d++) len[3 * b + d] = nd.blockLength(d);
+    }
+    if (sayaka_dev_set_topology(ctx, n, t.minAmrLevel, t.maxAmrLevel, level.data(), parent.data(), wc.data(),
+                                child.data(), nbr.data(), type.data(), len.data()) < 0)
+        throw std::runtime_error(std::string("tree topology: ") + sayaka_dev_last_error());
+}
+
+void upload_tree_surroundings(sayaka_dev_ctx* ctx, const AmrTree& t) {
+    const int n = t.numBlocks;
+    std::vector<int> surr((size_t)n * 27), bcref((size_t)n * 27);
+    for (int b = 0; b < n; b++) {
+        const SurroundingBlocks& s = t.surrBlocks[b];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    surr[(size_t)b * 27 + i * 9 + j * 3 + k] = s.blocks[i][j][k];
+                    bcref[(size_t)b * 27 + i * 9 + j * 3 + k] = s.bcrefs[i][j][k];
+                }
+    }
+    if (sayaka_dev_set_surroundings(ctx, n, surr.data(), bcref.data()) < 0)
+        throw std::runtime_error(std::string("tree surroundings: ") + sayaka_dev_last_error());
+}
+
 void MacSolver::prepareSolver() {
     const AmrTree& t = *tree_;
     nblk_ = t.numBlocks;
-    std::vector<int> level(nblk_), parent(nblk_), wc(nblk_), child(8 * nblk_), nbr(6 * nblk_), type(nblk_);
-    std::vector<double> len(3 * nblk_);
-    for (int b = 0; b < nblk_; b++) {
-        const AmrTreeNode& n = t[b];
-        level[b] = n.refineLevel;
-        parent[b] = n.parent;
-        wc[b] = n.whichChild;
-        type[b] = n.nodeType;
-        for (int q = 0; q < 8; q++) child[8 * b + q] = n.child[q];
-        for (int f = 0; f < 6; f++) nbr[6 * b + f] = n.neighbor[f];
-        for (int d = 0; d < 3; d++) len[3 * b + d] = n.blockLength(d);
-    }
-    check(sayaka_dev_set_topology(ctx_, nblk_, t.minAmrLevel, t.maxAmrLevel, level.data(), parent.data(), wc.data(),
-                                  child.data(), nbr.data(), type.data(), len.data()),
-          "MacSolver::prepareSolver");
+    upload_tree_topology(ctx_, t);
     // prepareSolver resets the coefficients to unit values (SayakaMacSolver.cpp:12-26)
     unitIB_ = unitB_ = true;
     alpha_ = 0.0;

@@ -71,5 +71,11 @@ void TreeData::packGrown(int comp, int nblk, double* out) const {
                     compsize_ * sizeof(double));
 }
 
+void TreeData::unpackGrown(int comp, int nblk, const double* in) {
+    for (int b = 0; b < nblk; b++)
+        std::memcpy(data_.data() + (std::size_t)b * blocksize_ + (std::size_t)comp * compsize_,
+                    in + (std::size_t)b * compsize_, compsize_ * sizeof(double));
+}
+
 }  // namespace amd
 }  // namespace sayaka

@@ -8,7 +8,9 @@ process on the card holds one (the pytest process would, once any solve had run 
 The driver also writes its FP64 solution through WriteDataHdf5(TreeData) with the raw block
 data and geometry beside it; the file must equal, object by object, the reference's writer
 restated over the image's HDF5 library on the same tree and data (skipped without that
-library)."""
+library).  It then fills the solution's 2 ghost layers with the host FillPatch (centred linear,
+Neumann walls); the filled state must equal oracle/fill_oracle.py's fill of the same data on
+the same tree, bit for bit."""
 from __future__ import annotations
 
 import os
@@ -18,6 +20,9 @@ import numpy as np
 import pytest
 
 from oracle import chombo_oracle as CO
+from oracle import fill_oracle as F
+from sayakaamr_amd import workloads as W
+from sayakaamr_amd.tree import AmrTree
 from sayakaamr_amd import _native as N
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +40,25 @@ def test_cpp_driver(tmp_path):
     assert kv["fp64_status"] == "0" and kv["mixed_status"] == "0"
     assert int(kv["fp64_iters"]) < 50
     assert float(kv["rel_l2_mixed_vs_fp64"]) < 1e-3
+    check_fill(h5)
     check_output(h5)
+
+
+def check_fill(h5):
+    blk = np.fromfile(h5 + ".blocks").reshape(-1, 10)
+    n = blk.shape[0]
+    # the driver's tree is the benchmark's: same refinement rule, same numbering
+    t = W.build_tree(AmrTree(3, max_blocks=8000), W.C1_SPEC, W.shell_criterion())
+    assert t["level"].shape[0] == n
+    assert np.array_equal(t["level"], blk[:, 0].astype(np.int32))
+    assert np.array_equal(t["lo"], blk[:, 1:4]) and np.array_equal(t["hi"], blk[:, 4:7])
+    phi = np.fromfile(h5 + ".phi").reshape(n, 8, 8, 8)
+    got = np.fromfile(h5 + ".ghost").reshape(n, 12, 12, 12)
+    st = np.zeros((n, 12, 12, 12))
+    st[:, 2:10, 2:10, 2:10] = phi
+    ref = [st]
+    F.StateFill(t, 8, 3, 2, W.C1_SPEC.maxlevel).fill(ref, 0, 1, 2, [1], [{W.WALL: (F.NEUMANN, 0.0)}])
+    assert np.array_equal(got.view(np.int64), ref[0].view(np.int64))
 
 
 def check_output(h5):
