@@ -53,6 +53,23 @@ struct SurroundingBlocks {
     void setUnused();
 };
 
+class TreeData;
+struct BCPatch;
+class FillPatch;
+
+// A state registered with the tree (TreeStateData, SayakaTree.h:528-560): regrid moves its
+// blocks along with the tree's, initNewBlockDataAfterRegrid fills the new ones through its
+// FillPatch.  The pointers are the caller's, as in the reference.
+struct TreeStateData {
+    TreeData* old_data = nullptr;
+    TreeData* new_data = nullptr;
+    BCPatch* bc_patch = nullptr;
+    FillPatch* fill_new_patch = nullptr;
+    bool hasCurrData() const { return new_data != nullptr; }
+    TreeData& currData() { return *new_data; }
+    const TreeData& currData() const { return *new_data; }
+};
+
 struct AmrTree {
     // configuration (set before init())
     int ndim = 3;
@@ -74,6 +91,7 @@ struct AmrTree {
     std::vector<int> regridOrigin;
     std::vector<SurroundingBlocks> surrBlocks;
     std::vector<std::vector<int>> levels;  // levels[l] = ids of level-l blocks, ascending
+    std::vector<TreeStateData> treeStateData;
 
     int numFace() const { return 2 * ndim; }
     int numChild() const { return 1 << ndim; }
@@ -84,8 +102,12 @@ struct AmrTree {
                               const bool is_periodic[MAX_FACE]);
     void initRefineToMinLevel();
 
-    // Returns 0 on success, -1 when the node pool overflows (the reference exits).
+    // Returns 0 on success, -1 when the node pool overflows (the reference exits).  The
+    // registered states' blocks move with the tree's (SayakaRegrid.cpp:530-591).
     int regrid();
+    // AmrTree::initNewBlockDataAfterRegrid (SayakaRegrid.cpp:987-1105) for every registered
+    // state, on the GPU through its FillPatch (FillPatch::initNewBlocks).
+    void initNewBlockDataAfterRegrid();
 
     const AmrTreeNode& operator[](int i) const { return blocks[i]; }
     AmrTreeNode& operator[](int i) { return blocks[i]; }

@@ -51,9 +51,16 @@ class FillPatch {
     void setProlongation(int comp, int prolong);
     // FillPatch::fillBoundary (SayakaFillPatch.cpp:17-30): levels 1..finest, every block
     void fillBoundary(int scomp, int ncomp, int nlayer);
+    // initNewBlockDataAfterRegrid's work on this state (SayakaRegrid.cpp:1013-1103), after the
+    // regrid moved its blocks: the parents of new blocks filled in-level, the new blocks
+    // (isnew[b] != 0) prolonged, face data corrected from existing neighbours; on the GPU
+    // (sayaka_dev_state_remap).  Called by AmrTree::initNewBlockDataAfterRegrid.
+    void initNewBlocks(const std::vector<int>& isnew);
 
    private:
     void prepareDevice();
+    void bcTables(int scomp, int ncomp, std::vector<int>& codes, std::vector<int>& mathbc,
+                  std::vector<double>& bcval) const;
     const AmrTree* tree_;
     TreeData* data_;
     const BCPatch* bc_ = nullptr;

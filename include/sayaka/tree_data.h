@@ -52,6 +52,9 @@ class TreeData {
     // [blk][grown box] (the layout sayaka_dev_fix_bc reads).
     void packGrown(int comp, int nblk, double* out) const;
     void unpackGrown(int comp, int nblk, const double* in);
+    // the regrid's packing: slot b takes slot origin[b] (the whole block), 0 where origin is
+    // -1; slots from nblk on are zeroed (SayakaRegrid.cpp:530-591)
+    void remapBlocks(const std::vector<int>& origin, int nblk);
     std::size_t grownSize() const { return compsize_; }   // values per block and component
 
    private:

@@ -50,6 +50,21 @@ int sy_tree_build_shell(sy_tree* t, const double* centre, double r0, double fact
 /* refine / coarsen flags of the current leaves for the shell |x - centre| = r0 (the rule of
  * sy_tree_build_shell); returns the number of flagged blocks; follow with sy_tree_regrid */
 int sy_tree_flag_shell(sy_tree* t, const double* centre, double r0, double factor);
+/* States registered with the tree (TreeStateData, SayakaTree.h:528-560, as the demos register
+ * theirs, Test5.h:590-610): cell data (dir < 0) or face data of direction dir, ncomp
+ * components, ngrow ghost layers, a FillPatch with prolong[c] per component and, for cell data,
+ * a BCPatch mapping each of the nphys physical codes to (mathbc, bcval)[c * nphys + q] on every
+ * surrounding position.  Owned by the tree handle.  sy_tree_regrid moves their blocks
+ * (SayakaRegrid.cpp:530-591); sy_tree_init_new_data is AmrTree::initNewBlockDataAfterRegrid
+ * (SayakaRegrid.cpp:987-1105), run on the GPU.  Data as [comp][block][grown box] for the
+ * tree's current blocks.  Returns: the state index / 0, or -1 (sy_tree_last_error). */
+const char* sy_tree_last_error(void);
+int sy_tree_state_register(sy_tree* t, int dir, int ncomp, int ngrow, const int* prolong, int nphys,
+                           const int* physbc, const int* mathbc, const double* bcval);
+int sy_tree_state_set_grown(sy_tree* t, int idx, const double* data);
+int sy_tree_state_get_grown(const sy_tree* t, int idx, double* data);
+int sy_tree_init_new_data(sy_tree* t);
+
 /* AmrTree::getTreeLevel (SayakaTree.h:317-318) */
 int sy_tree_level_count(const sy_tree* t, int level);
 void sy_tree_level_blocks(const sy_tree* t, int level, int* out);

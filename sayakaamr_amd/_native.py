@@ -57,6 +57,11 @@ _HOST_SIG = {
     "sy_mac_set_params": ([vp, f64, f64, i32, i32, i32, i32, i32], i32),
     "sy_mac_cg_solve": ([vp, vp, vp, vp, vp], i32),
     "sy_writer_last_error": ([], C.c_char_p),
+    "sy_tree_last_error": ([], C.c_char_p),
+    "sy_tree_state_register": ([vp, i32, i32, i32, vp, i32, vp, vp, vp], i32),
+    "sy_tree_state_set_grown": ([vp, i32, vp], i32),
+    "sy_tree_state_get_grown": ([vp, i32, vp], i32),
+    "sy_tree_init_new_data": ([vp], i32),
     "sy_write_chombo": ([vp, i32, vp, C.c_char_p, f64, f64, i32, vp], i32),
 }
 

@@ -7,6 +7,9 @@
 // loop implements; the loops are written against this file's run-time dimension.
 #include "amr_tree.h"
 
+#include "fill_patch.h"
+#include "tree_data.h"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -169,7 +172,19 @@ int AmrTree::regrid() {
     buildSurroundingBlocks();
     cacheTreeLevels();
     gridChanged = 1;
+    for (TreeStateData& st : treeStateData)
+        if (st.new_data) st.new_data->remapBlocks(regridOrigin, numBlocks);
     return 0;
+}
+
+void AmrTree::initNewBlockDataAfterRegrid() {
+    bool any = false;
+    for (int i = 0; i < numBlocks && !any; i++) any = regridOrigin.size() > (size_t)i && regridOrigin[i] < 0;
+    if (!any) return;   // SayakaRegrid.cpp:1004-1005
+    std::vector<int> isnew(numBlocks);
+    for (int i = 0; i < numBlocks; i++) isnew[i] = regridOrigin[i] < 0;
+    for (TreeStateData& st : treeStateData)
+        if (st.new_data && st.fill_new_patch) st.fill_new_patch->initNewBlocks(isnew);
 }
 
 // A leaf may be coarsened only when no face neighbour has children, no neighbour of such a

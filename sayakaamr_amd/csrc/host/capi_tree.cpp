@@ -4,10 +4,29 @@
 #include <cstring>
 #include <new>
 
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
 #include "amr_tree.h"
+#include "fill_patch.h"
 #include "sayaka_host.h"
+#include "tree_data.h"
 
 using namespace sayaka;
+
+namespace {
+// the registered states of each tree handle, owned here (the reference's drivers own theirs)
+struct OwnedState {
+    TreeData data;
+    BCPatch bc;
+    std::unique_ptr<FillPatch> fill;
+};
+std::map<const AmrTree*, std::vector<std::unique_ptr<OwnedState>>> g_states;
+thread_local std::string g_terr;
+}  // namespace
 
 extern "C" {
 
@@ -40,7 +59,86 @@ sy_tree* sy_tree_create(int ndim, int nb, int ngrow, int minlevel, int maxlevel,
     return reinterpret_cast<sy_tree*>(t);
 }
 
-void sy_tree_destroy(sy_tree* h) { delete reinterpret_cast<AmrTree*>(h); }
+void sy_tree_destroy(sy_tree* h) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    if (t) t->treeStateData.clear();
+    g_states.erase(t);
+    delete t;
+}
+
+const char* sy_tree_last_error(void) { return g_terr.c_str(); }
+
+int sy_tree_state_register(sy_tree* h, int dir, int ncomp, int ngrow, const int* prolong, int nphys,
+                           const int* physbc, const int* mathbc, const double* bcval) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    try {
+        if (!t || !prolong || ncomp < 1 || ngrow < 0 || (nphys > 0 && (!physbc || !mathbc || !bcval)))
+            throw std::invalid_argument("bad argument");
+        auto st = std::make_unique<OwnedState>();
+        st->data = dir < 0 ? TreeData::MakeCellData(*t, ncomp, ngrow) : TreeData::MakeFaceData(*t, dir, ncomp, ngrow);
+        st->bc.defineByData(st->data);
+        for (int c = 0; c < ncomp; c++)
+            for (int i = -1; i <= 1; i++)
+                for (int j = -1; j <= 1; j++)
+                    for (int k = -1; k <= 1; k++)
+                        for (int q = 0; q < nphys; q++) {
+                            st->bc.boundaryRegister(c).setBCMap(i, j, k, physbc[q], mathbc[c * nphys + q]);
+                            st->bc.boundaryRegister(c).setBCVal(i, j, k, physbc[q], bcval[c * nphys + q]);
+                        }
+        st->fill = std::make_unique<FillPatch>(*t, st->data);
+        st->fill->setBC(st->bc);
+        for (int c = 0; c < ncomp; c++) st->fill->setProlongation(c, prolong[c]);
+        TreeStateData tsd;
+        tsd.new_data = &st->data;
+        tsd.bc_patch = &st->bc;
+        tsd.fill_new_patch = st->fill.get();
+        t->treeStateData.push_back(tsd);
+        g_states[t].push_back(std::move(st));
+        return (int)t->treeStateData.size() - 1;
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
+}
+
+static TreeData* state_data(const AmrTree* t, int idx) {
+    if (!t || idx < 0 || idx >= (int)t->treeStateData.size()) return nullptr;
+    return t->treeStateData[idx].new_data;
+}
+
+int sy_tree_state_set_grown(sy_tree* h, int idx, const double* data) {
+    TreeData* d = state_data(reinterpret_cast<AmrTree*>(h), idx);
+    if (!d || !data) {
+        g_terr = "bad state index";
+        return -1;
+    }
+    const int n = reinterpret_cast<AmrTree*>(h)->numBlocks;
+    const size_t per = (size_t)n * d->grownSize();
+    for (int c = 0; c < d->numComp(); c++) d->unpackGrown(c, n, data + c * per);
+    return 0;
+}
+
+int sy_tree_state_get_grown(const sy_tree* h, int idx, double* data) {
+    const TreeData* d = state_data(reinterpret_cast<const AmrTree*>(h), idx);
+    if (!d || !data) {
+        g_terr = "bad state index";
+        return -1;
+    }
+    const int n = reinterpret_cast<const AmrTree*>(h)->numBlocks;
+    const size_t per = (size_t)n * d->grownSize();
+    for (int c = 0; c < d->numComp(); c++) d->packGrown(c, n, data + c * per);
+    return 0;
+}
+
+int sy_tree_init_new_data(sy_tree* h) {
+    try {
+        reinterpret_cast<AmrTree*>(h)->initNewBlockDataAfterRegrid();
+        return 0;
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
+}
 
 int sy_tree_numblocks(const sy_tree* h) { return reinterpret_cast<const AmrTree*>(h)->numBlocks; }
 int sy_tree_finest(const sy_tree* h) { return reinterpret_cast<const AmrTree*>(h)->currentFinestLevel(); }

@@ -77,5 +77,14 @@ void TreeData::unpackGrown(int comp, int nblk, const double* in) {
                     in + (std::size_t)b * compsize_, compsize_ * sizeof(double));
 }
 
+void TreeData::remapBlocks(const std::vector<int>& origin, int nblk) {
+    std::vector<double> out(data_.size(), 0.0);
+    for (int b = 0; b < nblk && b < nslot_; b++)
+        if (origin[b] >= 0)
+            std::memcpy(out.data() + (std::size_t)b * blocksize_, data_.data() + (std::size_t)origin[b] * blocksize_,
+                        blocksize_ * sizeof(double));
+    data_.swap(out);
+}
+
 }  // namespace amd
 }  // namespace sayaka

@@ -98,6 +98,51 @@ class AmrTree:
                                                                 "surr_type", "surr_bcref")])
         return t
 
+    # ---- registered states (TreeStateData; regrid moves them, init_new_data fills new blocks)
+    def _tchk(self, rc):
+        if rc < 0:
+            raise RuntimeError(self.lib.sy_tree_last_error().decode())
+        return rc
+
+    def state_register(self, dir, ncomp, ngrow, prolong, physbc=(), mathbc=(), bcval=()):
+        """Register a state (cell: dir None or -1; face: direction); mathbc / bcval are
+        [comp][physical code] on every surrounding position.  Returns its index."""
+        d = -1 if dir is None else int(dir)
+        pr = np.ascontiguousarray(prolong, dtype=np.int32)
+        nphys = len(physbc)
+        pb = np.ascontiguousarray(physbc if nphys else [0], dtype=np.int32)
+        mb = np.ascontiguousarray(mathbc if nphys else [0], dtype=np.int32)
+        bv = np.ascontiguousarray(bcval if nphys else [0.0], dtype=np.float64)
+        idx = self._tchk(self.lib.sy_tree_state_register(self.h, d, ncomp, ngrow, N.ptr(pr), nphys, N.ptr(pb),
+                                                         N.ptr(mb), N.ptr(bv)))
+        if not hasattr(self, "_states"):
+            self._states = {}
+        self._states[idx] = (d, ncomp, ngrow)
+        return idx
+
+    def _state_shape(self, idx):
+        d, ncomp, ngrow = self._states[idx]
+        nb = self.spec.nb
+        ext = [nb + 2 * ngrow + (d == q) for q in range(3)]
+        if self.ndim == 2:
+            ext[2] = 1
+        return (ncomp, self.numblocks, ext[2], ext[1], ext[0])
+
+    def state_set_grown(self, idx, data):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        if a.shape != self._state_shape(idx):
+            raise ValueError(f"state shape {a.shape}, expected {self._state_shape(idx)}")
+        self._tchk(self.lib.sy_tree_state_set_grown(self.h, idx, N.ptr(a)))
+
+    def state_get_grown(self, idx):
+        out = np.zeros(self._state_shape(idx))
+        self._tchk(self.lib.sy_tree_state_get_grown(self.h, idx, N.ptr(out)))
+        return out
+
+    def init_new_data(self):
+        """AmrTree::initNewBlockDataAfterRegrid for every registered state (on the GPU)."""
+        self._tchk(self.lib.sy_tree_init_new_data(self.h))
+
     def level_blocks(self, level: int) -> np.ndarray:
         n = self.lib.sy_tree_level_count(self.h, level)
         out = np.zeros(n, np.int32)

@@ -129,3 +129,43 @@ def test_device_remap_errors():
                 x.close()
     finally:
         dev.close()
+
+
+@pytest.mark.parametrize("name", ["sremap2", "sremap3", "fremap2"])
+def test_host_registered_state_matches_reference(name, golden):
+    """The same regrids through the host C++ surface a driver uses: a state registered with
+    the AmrTree (TreeStateData with its FillPatch / BCPatch), AmrTree::regrid moving its blocks
+    on the host and AmrTree::initNewBlockDataAfterRegrid filling the new ones on the GPU."""
+    g = golden(name)
+    spec = GC.remap_spec(name)
+    dirs = GC.REMAP_CASES[name][3]
+    for fd in (dirs if dirs is not None else (None,)):
+        tag = "c" if fd is None else f"d{fd}"
+        tb = AmrTree(spec.ndim, max_blocks=4000)
+        W.build_tree(tb, spec, GC.remap_criterion(name, 0))
+        prolong = GC.remap_prolong(name, fd)
+        if fd is None:
+            maps = GC.remap_bcmaps(name)
+            codes = list(GC.FILL_WALLS)
+            idx = tb.state_register(None, len(prolong), GC.FILL_NGROW, prolong, codes,
+                                    [maps[c][k][0] for c in range(len(prolong)) for k in codes],
+                                    [maps[c][k][1] for c in range(len(prolong)) for k in codes])
+        else:
+            idx = tb.state_register(fd, len(prolong), GC.FILL_NGROW, prolong)
+        for step in range(GC.REMAP_CASES[name][5]):
+            before = tb.export()
+            pre = GC.remap_input(name, before, step, fd)
+            tb.state_set_grown(idx, pre)
+            tb.set_refine_flags(g[f"{tag}_step{step}_refine"])
+            tb.set_coarsen_flags(g[f"{tag}_step{step}_coarsen"])
+            tb.regrid()
+            print(f"[gpu] host registered state {name} {tag} step {step}", flush=True)
+            tb.init_new_data()
+            after = tb.export()
+            origin = tb.regrid_origin()
+            post = tb.state_get_grown(idx)
+            touched = GC.remap_touched(after, origin)
+            assert np.array_equal(post[:, touched].view(np.int64), g[f"{tag}_step{step}_post"].view(np.int64)), \
+                (fd, step)
+            keep = ~touched
+            assert np.array_equal(post[:, keep], pre[:, origin[keep]])
