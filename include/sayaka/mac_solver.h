@@ -86,6 +86,12 @@ class MacSolver {
     void setPreSmoothNum(int n) { preSmooth_ = n; }
     void setPostSmoothNum(int n) { postSmooth_ = n; }
     void setCGUseMGPrecond(int on) { precond_ = on; }
+    // mg_solve runs exactly n V-cycles when n > 0 (SayakaMacSolver.h:355-356)
+    void setMGFixedIter(int n) { mgFixedIter_ = n; }
+    bool mgUseFixedIter() const { return mgFixedIter_ > 0; }
+    // the A coefficient array (SayakaMacCoef.cpp:15-18): kept, never read -- the operator
+    // requires alpha == 0, as the reference's does
+    void setUniformCoefA(double a) { coefA_ = a; }
     void setPrecision(Precision p) { precision_ = p; }
 
     // The same calls on packed valid-only arrays ([blk][k][j][i] over the tree's blocks, face
@@ -101,6 +107,15 @@ class MacSolver {
 
     // MacSolver::cg_solve (SayakaMacSolver.cpp:455-680)
     int cg_solve();
+    // MacSolver::mg_solve (SayakaMacSolver.cpp:43-162): V-cycles on the finest tower from the
+    // current solution (no pre-smooth, restriction of the residual, coarsest-level BiCGSTAB,
+    // prolongation + post-smooths, correction) until |r| < |r0| tol_rel + tol_abs, or exactly
+    // setMGFixedIter cycles; every operation in FP64 in the reference's order on the GPU.
+    // Returns 0 as the reference does (lastStats() holds the cycles and the residuals).
+    int mg_solve();
+    // MacSolver::relax_solve (SayakaMacSolver.cpp:377-452): red-black Gauss-Seidel passes on the
+    // finest tower until the same test holds or max_iter passes; returns 0.
+    int relax_solve();
 
     // MacSolver::mg_flux_level (SayakaMacSolver.h:275, SayakaMacOperator.cpp:161-239): the face
     // flux -d(phi)/dx of direction dir on tower mg_level into flux (face data of dir): in-block
@@ -124,7 +139,8 @@ class MacSolver {
     int verbose_ = 0;
     double alpha_ = 0.0, beta_ = 1.0;
     double tolRel_ = 1e-10, tolAbs_ = 0.0;
-    int maxIter_ = 2000, preSmooth_ = 2, postSmooth_ = 2, precond_ = 1;
+    int maxIter_ = 2000, preSmooth_ = 2, postSmooth_ = 2, precond_ = 1, mgFixedIter_ = 0;
+    double coefA_ = 0.0;
     Precision precision_ = FP64;
     bool unitIB_ = true, unitB_ = true;
     std::vector<double> vf_, af_[3], b_[3];   // leaf values, packed valid [blk][...]

@@ -110,6 +110,8 @@ int sayaka_dev_prolong(sayaka_dev_ctx* ctx, int fine_level, int field);         
 int sayaka_dev_dot(sayaka_dev_ctx* ctx, int m, int a, int b, double* dot, double* maxabs_a);
                                                      /* mg_dotprod_level, mg_norm_level */
 int sayaka_dev_precond(sayaka_dev_ctx* ctx, int m, int sol, int rhs);            /* mg_precond_level */
+int sayaka_dev_add(sayaka_dev_ctx* ctx, int m, int dst, int src);                /* mg_correct_level:
+                                                        dst += src on tower m (SayakaMacUtil.cpp:109-133) */
 int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, double tol_abs, int max_iter,
                       int* iters, int* status);      /* mg_solve_bottom */
 

@@ -101,6 +101,11 @@ int sy_mac_set_params(sy_mac* m, double tol_rel, double tol_abs, int max_iter, i
                       int pre_smooth, int post_smooth);
 /* MacSolver::cg_solve (SayakaMacSolver.cpp:455-680) */
 int sy_mac_cg_solve(sy_mac* m, int* iters, int* status, double* rnorm0, double* rnorm);
+/* MacSolver::mg_solve (SayakaMacSolver.cpp:43-162) with setMGFixedIter(fixed_iter) (0: iterate
+ * to the tolerance, up to max_iter V-cycles) and MacSolver::relax_solve (377-452); from the
+ * current solution; status 0 converged, 2 not (the reference returns 0 either way) */
+int sy_mac_mg_solve(sy_mac* m, int fixed_iter, int* iters, int* status, double* rnorm0, double* rnorm);
+int sy_mac_relax_solve(sy_mac* m, int* iters, int* status, double* rnorm0, double* rnorm);
 
 /* ---- Chombo-format HDF5 output (WriteDataHdf5, SayakaWriterHdf5.cpp:351-530) -----------
  * Cell data already in the file's order, packed = for every tree level 1..finest, every block

@@ -354,6 +354,21 @@ void ref_mac_set_params(double tol_rel, double tol_abs, int max_iter, int precon
     g_mac->setCGUseMGPrecond(precond);
 }
 
+// mg_solve / relax_solve (SayakaMacSolver.cpp:43-162, 377-452): the multigrid and the
+// relaxation solve of the finest tower from the SOL buffer as initial guess; fixed_iter > 0
+// is setMGFixedIter (mg_solve runs exactly that many V-cycles).
+void ref_mac_mg_solve(int fixed_iter) {
+    g_mac->setMGFixedIter(fixed_iter);
+    quiet_begin();
+    g_mac->mg_solve();
+    quiet_end();
+}
+void ref_mac_relax_solve() {
+    quiet_begin();
+    g_mac->relax_solve();
+    quiet_end();
+}
+
 // cg_solve (SayakaMacSolver.cpp:455-680).  The reference returns only a breakdown
 // flag; the iteration count and the residual history are recovered from its
 // verbose log lines (SayakaMacSolver.cpp:768, 836, 887), as the survey prescribes.

@@ -73,6 +73,8 @@ def load(ndim: int):
         "ref_state_set": ([_p], None),
         "ref_state_get": ([_p], None),
         "ref_tree_init_new_data": ([], None),
+        "ref_mac_mg_solve": ([_i], None),
+        "ref_mac_relax_solve": ([], None),
         "ref_state_create_general": ([_i, _i, _i, _p, _i, _p, _p, _p], None),
         "ref_state_set_grown": ([_p], None),
         "ref_state_get_grown": ([_p], None),
@@ -283,6 +285,12 @@ class RefMac:
 
     def set_params(self, tol_rel, tol_abs, max_iter, precond=1):
         self.lib.ref_mac_set_params(tol_rel, tol_abs, max_iter, precond)
+
+    def mg_solve(self, fixed_iter=0):
+        self.lib.ref_mac_mg_solve(fixed_iter)
+
+    def relax_solve(self):
+        self.lib.ref_mac_relax_solve()
 
     def cg_solve(self, hist_cap=20000):
         it = np.zeros(1, np.int32)

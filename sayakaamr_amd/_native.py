@@ -56,6 +56,8 @@ _HOST_SIG = {
     "sy_mac_flux": ([vp, i32, i32, vp, vp], i32),
     "sy_mac_set_params": ([vp, f64, f64, i32, i32, i32, i32, i32], i32),
     "sy_mac_cg_solve": ([vp, vp, vp, vp, vp], i32),
+    "sy_mac_mg_solve": ([vp, i32, vp, vp, vp, vp], i32),
+    "sy_mac_relax_solve": ([vp, vp, vp, vp, vp], i32),
     "sy_writer_last_error": ([], C.c_char_p),
     "sy_tree_last_error": ([], C.c_char_p),
     "sy_tree_state_register": ([vp, i32, i32, i32, vp, i32, vp, vp, vp], i32),

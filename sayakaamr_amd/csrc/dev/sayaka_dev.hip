@@ -1789,6 +1789,21 @@ int sayaka_dev_copy(sayaka_dev_ctx* c, int dst, int src) {
     return 0;
 }
 
+int sayaka_dev_add(sayaka_dev_ctx* c, int m, int dst, int src) {
+    if (!c || !field_ok(dst) || !field_ok(src) || m < 1 || m > c->finest) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    int e = dispatch(c, [&](auto I) {
+        using Im = decltype(I);
+        VecArgs a = vec_args(c, c->tw[m]);
+        a.dst = c->d_field[dst];
+        a.x = c->d_field[src];
+        return launch_vec<Im::D_, Im::NB_>(c, c->tw[m], V_FINAL, a, FIN_NONE);
+    });
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int sayaka_dev_fill(sayaka_dev_ctx* c, int field, double value) {
     if (!c || !field_ok(field)) return fail(-1, "bad argument");
     HIPCHK(hipSetDevice(c->device));

@@ -147,4 +147,29 @@ int sy_mac_cg_solve(sy_mac* h, int* iters, int* status, double* rnorm0, double* 
     });
 }
 
+int sy_mac_mg_solve(sy_mac* h, int fixed_iter, int* iters, int* status, double* rnorm0, double* rnorm) {
+    return guarded([&] {
+        M(h)->setMGFixedIter(fixed_iter);
+        const int r = M(h)->mg_solve();
+        const SolveStats& s = M(h)->lastStats();
+        if (iters) *iters = s.iters;
+        if (status) *status = s.status;
+        if (rnorm0) *rnorm0 = s.rnorm0;
+        if (rnorm) *rnorm = s.rnorm;
+        return r;
+    });
+}
+
+int sy_mac_relax_solve(sy_mac* h, int* iters, int* status, double* rnorm0, double* rnorm) {
+    return guarded([&] {
+        const int r = M(h)->relax_solve();
+        const SolveStats& s = M(h)->lastStats();
+        if (iters) *iters = s.iters;
+        if (status) *status = s.status;
+        if (rnorm0) *rnorm0 = s.rnorm0;
+        if (rnorm) *rnorm = s.rnorm;
+        return r;
+    });
+}
+
 }  // extern "C"

@@ -296,5 +296,68 @@ int MacSolver::cg_solve() {
     return st.status == 1 ? 1 : 0;
 }
 
+int MacSolver::mg_solve() {
+    const int fin = tree_->currentFinestLevel();
+    const int R = SAYAKA_RESID, X = SAYAKA_SOL, B = SAYAKA_RHS, CORR = SAYAKA_CORR;
+    double dot = 0.0, rnorm0 = 0.0, rnorm = 0.0;
+    check(sayaka_dev_residual(ctx_, fin, R, X, B), "MacSolver::mg_solve");
+    check(sayaka_dev_dot(ctx_, fin, R, R, &dot, &rnorm0), "MacSolver::mg_solve");
+    const int max_iter = mgUseFixedIter() ? mgFixedIter_ : maxIter_;
+    bool conv = false;
+    int iter = 0;
+    for (iter = 1; iter <= max_iter; iter++) {
+        check(sayaka_dev_fill(ctx_, CORR, 0.0), "MacSolver::mg_solve");
+        for (int l = fin; l >= 2; l--) check(sayaka_dev_restrict(ctx_, l - 1, R), "MacSolver::mg_solve");
+        int bit = 0, bst = 0;
+        // mg_solve_bottom (SayakaMacSolver.cpp:164-242): level-1 BiCGSTAB to 1e-6 / 1e-6, 2000 its
+        check(sayaka_dev_bottom(ctx_, CORR, R, 1.0e-6, 1.0e-6, 2000, &bit, &bst), "MacSolver::mg_solve");
+        if (bst == 1) throw std::runtime_error("MacSolver::mg_solve: MG: Bottom break down");
+        for (int l = 2; l <= fin; l++) {
+            check(sayaka_dev_prolong(ctx_, l, CORR), "MacSolver::mg_solve");
+            for (int q = 0; q < postSmooth_; q++) check(sayaka_dev_smooth(ctx_, l, CORR, R), "MacSolver::mg_solve");
+        }
+        check(sayaka_dev_add(ctx_, fin, X, CORR), "MacSolver::mg_solve");
+        check(sayaka_dev_residual(ctx_, fin, R, X, B), "MacSolver::mg_solve");
+        check(sayaka_dev_dot(ctx_, fin, R, R, &dot, &rnorm), "MacSolver::mg_solve");
+        if (rnorm < rnorm0 * tolRel_ + tolAbs_) conv = true;
+        if (!mgUseFixedIter() && conv) break;
+    }
+    stats_ = SolveStats{};
+    stats_.status = conv ? 0 : 2;
+    stats_.iters = iter > max_iter ? max_iter : iter;
+    stats_.rnorm0 = rnorm0;
+    stats_.rnorm = rnorm;
+    if (verbose_ || (!mgUseFixedIter() && !conv))
+        std::printf("MG: %s, iter=%d, |resid|=%e, |resid0|=%e\n", conv ? "converge" : "failure", stats_.iters, rnorm,
+                    rnorm0);
+    return 0;
+}
+
+int MacSolver::relax_solve() {
+    const int fin = tree_->currentFinestLevel();
+    const int R = SAYAKA_RESID, X = SAYAKA_SOL, B = SAYAKA_RHS;
+    double dot = 0.0, rnorm0 = 0.0, rnorm = 0.0;
+    check(sayaka_dev_fill(ctx_, SAYAKA_CORR, 0.0), "MacSolver::relax_solve");
+    check(sayaka_dev_residual(ctx_, fin, R, X, B), "MacSolver::relax_solve");
+    check(sayaka_dev_dot(ctx_, fin, R, R, &dot, &rnorm0), "MacSolver::relax_solve");
+    bool conv = false;
+    int iter = 0;
+    for (iter = 1; iter <= maxIter_; iter++) {
+        check(sayaka_dev_smooth(ctx_, fin, X, B), "MacSolver::relax_solve");
+        check(sayaka_dev_residual(ctx_, fin, R, X, B), "MacSolver::relax_solve");
+        check(sayaka_dev_dot(ctx_, fin, R, R, &dot, &rnorm), "MacSolver::relax_solve");
+        if (rnorm < rnorm0 * tolRel_ + tolAbs_) conv = true;
+        if (conv) break;
+    }
+    stats_ = SolveStats{};
+    stats_.status = conv ? 0 : 2;
+    stats_.iters = iter > maxIter_ ? maxIter_ : iter;
+    stats_.rnorm0 = rnorm0;
+    stats_.rnorm = rnorm;
+    if (verbose_ || !conv)
+        std::printf("relax_solve: %s, iter=%d, |resid|=%e\n", conv ? "converge" : "failure", stats_.iters, rnorm);
+    return 0;
+}
+
 }  // namespace amd
 }  // namespace sayaka

@@ -127,6 +127,23 @@ class MacSolver:
     def setPrecision(self, p):
         self._params["precision"] = p
 
+    def _iter_solve(self, fn, name, *pre):
+        p = self._params
+        self._chk(self.lib.sy_mac_set_params(self.h, p["tol_rel"], p["tol_abs"], p["max_iter"], p["precond"],
+                                             p["precision"], p["pre"], p["post"]), "setParams")
+        it, st, r0, r = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        ret = self._chk(fn(self.h, *pre, C.byref(it), C.byref(st), C.byref(r0), C.byref(r)), name)
+        self.stats = dict(iters=it.value, status=st.value, rnorm0=r0.value, rnorm=r.value)
+        return ret
+
+    def mg_solve(self, fixed_iter=0):
+        """MacSolver::mg_solve (SayakaMacSolver.cpp:43-162); fixed_iter as setMGFixedIter."""
+        return self._iter_solve(self.lib.sy_mac_mg_solve, "mg_solve", int(fixed_iter))
+
+    def relax_solve(self):
+        """MacSolver::relax_solve (SayakaMacSolver.cpp:377-452)."""
+        return self._iter_solve(self.lib.sy_mac_relax_solve, "relax_solve")
+
     def cg_solve(self):
         p = self._params
         self._chk(self.lib.sy_mac_set_params(self.h, p["tol_rel"], p["tol_abs"], p["max_iter"], p["precond"],

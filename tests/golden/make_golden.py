@@ -326,6 +326,37 @@ def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     print(f"{name}_flux: {n} blocks, wrote {len(out)} arrays")
 
 
+def mg_relax(name, spec, crit, inp_fn, bc):
+    """MacSolver::mg_solve (3 fixed V-cycles, then to 1e-8 from zero) and relax_solve (6
+    passes) of the reference from a zero solution: the solutions of every block."""
+    rt = R.RefTree(spec.ndim)
+    tree = W.build_tree(rt, spec, crit)
+    n, nc = tree["level"].shape[0], spec.nb ** spec.ndim
+    inp = inp_fn(tree, spec)
+    mac = R.RefMac(rt)
+    mac.prepare()
+    if inp.get("vf") is not None:
+        mac.set_ib_frac(inp["vf"], inp["af"])
+    if inp.get("b") is not None:
+        mac.set_coefb(inp["b"])
+    mac.set_bc([W.WALL], [bc], [0.0])
+    mac.set_rhs(inp["rhs"])
+    out = {}
+    mac.set_params(1e-8, 0.0, 40)
+    mac.set_zero_sol()
+    mac.mg_solve(3)
+    out["mg_fixed3"] = mac.get_sol(n, nc)
+    mac.set_zero_sol()
+    mac.mg_solve(0)
+    out["mg_tol"] = mac.get_sol(n, nc)
+    mac.set_params(1e-12, 0.0, 6)
+    mac.set_zero_sol()
+    mac.relax_solve()
+    out["relax6"] = mac.get_sol(n, nc)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: {n} blocks")
+
+
 def state_remap(name):
     """Regrids of the case's moving refinement region with a registered state (cell, or one
     face state per direction): per regrid the flags and, after regrid +
@@ -385,6 +416,11 @@ def state_remap(name):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "mgrelax":
+        mg_relax("c0_mgrelax", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
+        mg_relax("s3_mgrelax", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs,
+                 W.NEUMANN)
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "sremap":
         for name in ("sremap2", "sremap3", "fremap2", "fremap3"):
             state_remap(name)
