@@ -23,6 +23,17 @@ class TreeData {
     // TreeData::MakeCellData / MakeFaceData (SayakaTreeData.h:245-306)
     static TreeData MakeCellData(const AmrTree& tree, int ncomp, int ngrow);
     static TreeData MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow);
+    // heap-allocated forms (SayakaTreeData.h:227-232); the caller deletes them
+    static TreeData* CreateCellData(const AmrTree& tree, int ncomp, int ngrow);
+    static TreeData* CreateFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow);
+    static std::vector<TreeData*> CreateFaceDataPArray(const AmrTree& tree, int ncomp, int ngrow);
+    static void ReleaseDataPArray(std::vector<TreeData*>& parray, bool clear_array = true);
+    // TreeData::Copy / AddEqual (SayakaTreeData.cpp:230-340): components [scomp, scomp+ncomp) of
+    // src into [dcomp, ...) of dst over the valid box grown by ngrow, every block of the tree
+    // (AddEqual: dst += coef * src)
+    static void Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow);
+    static void AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp,
+                         int ngrow);
 
     // the tree the data was made for (TreeData::tree, SayakaTreeData.h:149); it must outlive the data
     const AmrTree& tree() const { return *tree_; }

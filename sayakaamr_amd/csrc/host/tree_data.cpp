@@ -33,6 +33,60 @@ TreeData TreeData::MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngr
     return t;
 }
 
+TreeData* TreeData::CreateCellData(const AmrTree& tree, int ncomp, int ngrow) {
+    return new TreeData(MakeCellData(tree, ncomp, ngrow));
+}
+
+TreeData* TreeData::CreateFaceData(const AmrTree& tree, int dir, int ncomp, int ngrow) {
+    return new TreeData(MakeFaceData(tree, dir, ncomp, ngrow));
+}
+
+std::vector<TreeData*> TreeData::CreateFaceDataPArray(const AmrTree& tree, int ncomp, int ngrow) {
+    std::vector<TreeData*> out;
+    for (int d = 0; d < tree.ndim; d++) out.push_back(CreateFaceData(tree, d, ncomp, ngrow));
+    return out;
+}
+
+void TreeData::ReleaseDataPArray(std::vector<TreeData*>& parray, bool clear_array) {
+    for (TreeData*& p : parray) {
+        delete p;
+        p = nullptr;
+    }
+    if (clear_array) parray.clear();
+}
+
+template <class F>
+static void for_grown_box(const TreeData& d, int ngrow, F&& f) {
+    const int gz = d.extent(2) > 1 ? ngrow : 0;
+    for (int k = -gz; k < d.validExtent(2) + gz; k++)
+        for (int j = -ngrow; j < d.validExtent(1) + ngrow; j++)
+            for (int i = -ngrow; i < d.validExtent(0) + ngrow; i++) f(i, j, k);
+}
+
+static void check_pair(const TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow) {
+    if (dst.faceDir() != src.faceDir() || dcomp < 0 || scomp < 0 || dcomp + ncomp > dst.numComp() ||
+        scomp + ncomp > src.numComp() || ngrow < 0 || ngrow > dst.numGrow() || ngrow > src.numGrow())
+        throw std::invalid_argument("TreeData: incompatible data, components or ghost layers");
+}
+
+void TreeData::Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow) {
+    check_pair(dst, src, dcomp, scomp, ncomp, ngrow);
+    const int n = dst.tree().numBlocks;
+    for (int b = 0; b < n; b++)
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) { dst(b, i, j, k, dcomp + c) = src(b, i, j, k, scomp + c); });
+}
+
+void TreeData::AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp, int ngrow) {
+    check_pair(dst, src, dcomp, scomp, ncomp, ngrow);
+    const int n = dst.tree().numBlocks;
+    for (int b = 0; b < n; b++)
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) {
+                dst(b, i, j, k, dcomp + c) += coef * src(b, i, j, k, scomp + c);
+            });
+}
+
 int TreeData::validExtent(int d) const {
     if (d == 2 && ndim_ == 2) return 1;
     return nb_ + (d == dir_ ? 1 : 0);
