@@ -103,6 +103,9 @@ class MacSolver {
     void setRhsPacked(const double* rhs);
     void setInitSolPacked(const double* sol);
     void getSolPacked(double* sol) const;
+    // fixBC on phi over the grown box ([blk][grown box], ngrow >= 1 layers, i fastest) and the
+    // packed rhs, updated in place
+    void fixBCPacked(const double* phi_grown, int ngrow, double* rhs) const;
     int numBlocks() const { return nblk_; }
 
     // MacSolver::cg_solve (SayakaMacSolver.cpp:455-680)

@@ -86,6 +86,10 @@ int sy_mac_set_ib_ls(sy_mac* m, const double* ls_grown, int ngrow);
 int sy_mac_set_ib_frac(sy_mac* m, const double* vf, const double* const* af);
 /* MacSolver::setCoefB (SayakaMacCoef.cpp:319-367): b[dir][blk][faces] */
 int sy_mac_set_coefb(sy_mac* m, const double* const* b);
+/* MacSolver::fixBC (SayakaMacSolver.h:166, SayakaMacOperator.cpp:1470-1594): phi over the grown
+ * box [blk][grown box] (ngrow >= 1 layers, i fastest; the wall ghosts carry the BC values), rhs
+ * packed [blk][cells], updated in place with the Dirichlet / SimpleFill wall terms */
+int sy_mac_fix_bc(sy_mac* m, const double* phi_grown, int ngrow, double* rhs);
 /* MacSolver::setRhs / setInitSol / setZeroSol / getSol (SayakaMacSolver.h:117-132) */
 int sy_mac_set_rhs(sy_mac* m, const double* rhs);
 int sy_mac_set_init_sol(sy_mac* m, const double* sol);

@@ -51,6 +51,7 @@ _HOST_SIG = {
     "sy_mac_set_coefb": ([vp, vp], i32),
     "sy_mac_set_rhs": ([vp, vp], i32),
     "sy_mac_set_init_sol": ([vp, vp], i32),
+    "sy_mac_fix_bc": ([vp, vp, i32, vp], i32),
     "sy_mac_set_zero_sol": ([vp], i32),
     "sy_mac_get_sol": ([vp, vp], i32),
     "sy_mac_flux": ([vp, i32, i32, vp, vp], i32),

@@ -97,6 +97,14 @@ int sy_mac_set_init_sol(sy_mac* h, const double* sol) {
     });
 }
 
+int sy_mac_fix_bc(sy_mac* h, const double* phi_grown, int ngrow, double* rhs) {
+    return guarded([&] {
+        if (!phi_grown || !rhs) throw std::invalid_argument("sy_mac_fix_bc: null array");
+        M(h)->fixBCPacked(phi_grown, ngrow, rhs);
+        return 0;
+    });
+}
+
 int sy_mac_set_zero_sol(sy_mac* h) {
     return guarded([&] {
         M(h)->setZeroSol();

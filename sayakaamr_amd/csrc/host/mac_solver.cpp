@@ -221,15 +221,22 @@ void MacSolver::setBC(const BCRegister& reg) {
     check(sayaka_dev_set_bc(ctx_, bc.data()), "MacSolver::setBC");
 }
 
+// fixBC / fixBlockBC (SayakaMacOperator.cpp:1470-1594); the reference exits when phi has no
+// ghost layer, this throws
 void MacSolver::fixBC(const TreeData& phi, TreeData& rhs, int phicomp, int rhscomp) const {
     std::vector<double> g((std::size_t)nblk_ * phi.extent(0) * phi.extent(1) * phi.extent(2));
     phi.packGrown(phicomp, nblk_, g.data());
     std::vector<double> r((std::size_t)nblk_ * ncell_);
     rhs.packValid(rhscomp, nblk_, r.data());
-    check(sayaka_dev_upload(ctx_, SAYAKA_TMP, r.data()), "MacSolver::fixBC");
-    check(sayaka_dev_fix_bc(ctx_, g.data(), phi.numGrow(), SAYAKA_TMP), "MacSolver::fixBC");
-    check(sayaka_dev_download(ctx_, SAYAKA_TMP, r.data()), "MacSolver::fixBC");
+    fixBCPacked(g.data(), phi.numGrow(), r.data());
     rhs.unpackValid(rhscomp, nblk_, r.data());
+}
+
+void MacSolver::fixBCPacked(const double* phi_grown, int ngrow, double* rhs) const {
+    if (ngrow < 1) throw std::runtime_error("MacSolver::fixBC: BC value NGROW<1");
+    check(sayaka_dev_upload(ctx_, SAYAKA_TMP, rhs), "MacSolver::fixBC");
+    check(sayaka_dev_fix_bc(ctx_, phi_grown, ngrow, SAYAKA_TMP), "MacSolver::fixBC");
+    check(sayaka_dev_download(ctx_, SAYAKA_TMP, rhs), "MacSolver::fixBC");
 }
 
 void MacSolver::setRhs(const TreeData& rhs, int comp) {

@@ -93,6 +93,14 @@ class MacSolver:
         s = np.ascontiguousarray(sol, dtype=np.float64)
         self._chk(self.lib.sy_mac_set_init_sol(self.h, N.ptr(s)), "setInitSol")
 
+    def fixBC(self, phi_grown, ngrow, rhs):
+        """MacSolver::fixBC: phi over the grown box [blk][grown] (wall ghosts carry the BC
+        values); returns rhs [blk][cell] with the wall terms added."""
+        g = np.ascontiguousarray(phi_grown, dtype=np.float64)
+        r = np.ascontiguousarray(rhs, dtype=np.float64).copy()
+        self._chk(self.lib.sy_mac_fix_bc(self.h, N.ptr(g), int(ngrow), N.ptr(r)), "fixBC")
+        return r
+
     def setZeroSol(self):
         self._chk(self.lib.sy_mac_set_zero_sol(self.h), "setZeroSol")
 

@@ -9,6 +9,7 @@ against on machines where the reference is absent.
     python tests/golden/make_golden.py flux    (face fluxes: tests/golden/{c0,s3}_flux.npz)
     python tests/golden/make_golden.py remap   (regrid data movement: regrid_remap{2,3}.npz)
     python tests/golden/make_golden.py fill    (general state fill: fill{2,3}.npz)
+    python tests/golden/make_golden.py fixbc   (fixBC, valued walls, warm start: {c0,s3}_fixbc.npz)
 """
 from __future__ import annotations
 
@@ -326,6 +327,48 @@ def case_flux(name, spec, crit, inp_fn, bc, seed=7):
     print(f"{name}_flux: {n} blocks, wrote {len(out)} arrays")
 
 
+def case_fixbc(name):
+    """MacSolver::fixBC on the case's phi (wall ghosts carrying the BC values), a solve of the
+    corrected rhs with those walls, and the same solve from a non-zero setInitSol guess."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import golden_cases as GC
+    spec = GC.fixbc_spec(name)
+    nd, nb, ng = spec.ndim, spec.nb, GC.FIXBC_CASES[name][4]
+    rt = R.RefTree(nd)
+    tree = W.build_tree(rt, spec, GC.fixbc_criterion(name))
+    n = tree["level"].shape[0]
+    inp = GC.fixbc_inputs(name, tree, spec)
+    mac = R.RefMac(rt)
+    mac.prepare()
+    if inp.get("vf") is not None:
+        mac.set_ib_frac(inp["vf"], inp["af"])
+    if inp.get("b") is not None:
+        mac.set_coefb(inp["b"])
+    codes, mb, bv = GC.fixbc_walls(name)
+    mac.set_bc(codes, mb, bv)
+    phi = GC.fixbc_phi(name, tree)
+    rhs = mac.fix_bc(phi, ng, inp["rhs"])
+    out = {"in_rhs": inp["rhs"], "fix_rhs": rhs}
+    for k in W.TREE_FIELDS:
+        out[f"tree_{k}"] = tree[k]
+    mac.set_rhs(rhs)
+    mac.set_zero_sol()
+    mac.set_params(1e-10, 0.0, 2000, 1)
+    res = mac.cg_solve()
+    out["solve_iters"] = np.array([res["iters"], int(res["converged"]), res["stat"]])
+    out["solve_hist"] = res["hist"]
+    out["solve_sol"] = mac.get_sol(n, nb ** nd)
+    guess = GC.fixbc_guess(name, tree)
+    mac.set_init_sol(guess)
+    res = mac.cg_solve()
+    out["warm_iters"] = np.array([res["iters"], int(res["converged"]), res["stat"]])
+    out["warm_hist"] = res["hist"]
+    out["warm_sol"] = mac.get_sol(n, nb ** nd)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: {n} blocks, fixBC changed {int((rhs != inp['rhs']).sum())} cells, iters "
+          f"{out['solve_iters'][0]} (zero guess) / {out['warm_iters'][0]} (warm)")
+
+
 def mg_relax(name, spec, crit, inp_fn, bc):
     """MacSolver::mg_solve (3 fixed V-cycles, then to 1e-8 from zero) and relax_solve (6
     passes) of the reference from a zero solution: the solutions of every block."""
@@ -416,6 +459,10 @@ def state_remap(name):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "fixbc":
+        case_fixbc("c0_fixbc")
+        case_fixbc("s3_fixbc")
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "mgrelax":
         mg_relax("c0_mgrelax", W.C0_SPEC, W.ring_criterion(), c0_inputs, W.DIRICHLET)
         mg_relax("s3_mgrelax", W.small_spec(3, 3, nb=4), W.shell_criterion(r0=0.25, factor=0.25), c1_inputs,

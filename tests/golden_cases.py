@@ -274,3 +274,90 @@ def remap_touched(tree, origin):
     p = tree["parent"][new]
     par[p[p >= 0]] = True
     return new | par
+
+
+# ------------------------------------------------------- fixBC, valued walls, warm starts
+# MacSolver::fixBC (SayakaMacOperator.cpp:1470-1594) as Test6 drives it: phi's wall ghosts
+# carry the BC values (Test6.h:270-276, 607-609 use Dirichlet phi = 10), fixBC adds the wall
+# terms to the rhs, and the solve runs with those walls in the operator.  Every wall has its own
+# physical code and its own math BC / value.  Then the same solve from a non-zero initial guess
+# (setInitSol, SayakaMacSolver.cpp:722-735, 906-914).
+FIXBC_CASES = {
+    # name -> (ndim, nb, maxlevel, tree, ngrow, math BC per wall, value per wall)
+    "c0_fixbc": (2, 8, 3, "c0", 1, (1, 2, 1, 0, 0, 0), (10.0, 2.5, -3.0, 0.0, 0.0, 0.0)),
+    "s3_fixbc": (3, 4, 3, "s3", 2, (1, 0, 2, 1, 0, 1), (10.0, 0.0, 1.5, 0.5, 0.0, -2.0)),
+}
+
+
+def fixbc_spec(name):
+    import dataclasses
+    nd, nb, maxlevel, kind, _, _, _ = FIXBC_CASES[name]
+    base = W.C0_SPEC if kind == "c0" else W.small_spec(3, maxlevel, nb=nb)
+    return dataclasses.replace(base, bc=FILL_WALLS)
+
+
+def fixbc_criterion(name):
+    return W.ring_criterion() if FIXBC_CASES[name][3] == "c0" else W.shell_criterion(r0=0.25, factor=0.25)
+
+
+def fixbc_inputs(name, tree, spec):
+    """rhs (and, in 3D, the sphere's fractions and density coefficient) of the case."""
+    if FIXBC_CASES[name][3] == "c0":
+        rhs, _ = W.c0_rhs_exact(tree, spec)
+        return dict(rhs=rhs, vf=None, af=None, b=None)
+    return W.c1_inputs(tree, spec, seed=0)
+
+
+def fixbc_walls(name):
+    """(physical codes, math BC, values) of the case's BC register."""
+    _, _, _, _, _, mb, bv = FIXBC_CASES[name]
+    nd = FIXBC_CASES[name][0]
+    nw = 2 * nd
+    return list(FILL_WALLS[:nw]), list(mb[:nw]), list(bv[:nw])
+
+
+def fixbc_phi(name, tree):
+    """phi over the grown box [block][gz][gy][gx] (i fastest): valid cells and ghosts from a
+    counter hash, then the first ghost layer on every wall face set as a BC fill leaves it for
+    a wall value v (BCPatch::fillBasicCellBlockBC, SayakaBoundaryPatch.cpp:139-229): Dirichlet
+    2v - phi_inside, SimpleFill v, Neumann phi_inside."""
+    nd, nb, _, _, g, mb, bv = FIXBC_CASES[name]
+    n = tree["level"].shape[0]
+    nz, gz = (nb, g) if nd == 3 else (1, 0)
+    shp = (n, nz + 2 * gz, nb + 2 * g, nb + 2 * g)
+    idx = np.arange(np.prod(shp), dtype=np.uint64).reshape(shp)
+    h = W._splitmix64(idx + np.uint64(0xF1BC0000))
+    phi = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+    codes = FILL_WALLS
+    for b in range(n):
+        for f in range(2 * nd):
+            code = int(tree["neighbor"][b, f])
+            if code > -20:
+                continue
+            q = codes.index(code)
+            d, s = f // 2, f % 2
+            ax = 3 - d   # array axis of direction d in [b][z][y][x]
+            lo_g, hi_g = (gz, gz + nz - 1) if d == 2 else (g, g + nb - 1)
+            gi, vi = (lo_g - 1, lo_g) if s == 0 else (hi_g + 1, hi_g)
+            sl_g = [b, slice(gz, gz + nz), slice(g, g + nb), slice(g, g + nb)]
+            sl_v = list(sl_g)
+            sl_g[ax] = gi
+            sl_v[ax] = vi
+            inside = phi[tuple(sl_v)]
+            if mb[q] == W.DIRICHLET:
+                phi[tuple(sl_g)] = 2.0 * bv[q] - inside
+            elif mb[q] == W.SIMPLEFILL:
+                phi[tuple(sl_g)] = bv[q]
+            else:
+                phi[tuple(sl_g)] = inside
+    return phi.reshape(n, -1)
+
+
+def fixbc_guess(name, tree):
+    """A non-zero initial guess [block][cell] (hash, U(-1,1) times 5)."""
+    nd, nb = FIXBC_CASES[name][0], FIXBC_CASES[name][1]
+    n = tree["level"].shape[0]
+    idx = np.arange(n * nb ** nd, dtype=np.uint64)
+    h = W._splitmix64(idx + np.uint64(0x6E550000))
+    u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+    return (5.0 * u).reshape(n, -1)
