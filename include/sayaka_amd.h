@@ -231,7 +231,7 @@ enum sayaka_kernel_class {
     SAYAKA_KCLASS_APPLY = 0,     /* FP64 operator / residual stencils (fused dot products) */
     SAYAKA_KCLASS_SMOOTH,        /* FP64 red-black Gauss-Seidel colour sweeps */
     SAYAKA_KCLASS_SMOOTH32,      /* FP32 red-black sweeps of the mixed V-cycle */
-    SAYAKA_KCLASS_BOTTOM,        /* coarsest-level BiCGSTAB (one cooperative launch) */
+    SAYAKA_KCLASS_BOTTOM,        /* coarsest-level BiCGSTAB (one persistent launch) */
     SAYAKA_KCLASS_VEC,           /* fused vector updates and reductions */
     SAYAKA_KCLASS_TRANSFER,      /* restriction / prolongation */
     SAYAKA_KCLASS_N

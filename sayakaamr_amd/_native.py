@@ -8,8 +8,11 @@ anywhere but on the GPU.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
@@ -92,3 +95,41 @@ def register(lib, sig: dict):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+
+
+# ------------------------------------------------------------------ orderly teardown
+# Objects holding a device context or device memory register here.  At interpreter exit the
+# atexit hook closes the live ones, newest first, while the HIP runtime is still fully up; a
+# finalizer that runs later (during interpreter finalization) makes no HIP call at all.
+_live = {}
+_next_id = 0
+
+
+def track(obj):
+    """Register obj (with a close() method) for closing at exit; returns its key."""
+    global _next_id
+    _next_id += 1
+    _live[_next_id] = weakref.ref(obj)
+    return _next_id
+
+
+def untrack(key):
+    _live.pop(key, None)
+
+
+def finalizing():
+    return sys.is_finalizing()
+
+
+def _close_all():
+    for key in sorted(_live, reverse=True):
+        ref = _live.pop(key, None)
+        obj = ref() if ref is not None else None
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(_close_all)

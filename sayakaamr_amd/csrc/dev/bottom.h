@@ -1,4 +1,4 @@
-// bottom.h -- coarsest-level solve as one cooperative persistent kernel.
+// bottom.h -- coarsest-level solve as one persistent kernel with its own grid barrier.
 //
 // The reference solves the level-1 problem with unpreconditioned BiCGSTAB to 1e-6
 // relative + 1e-6 absolute Linf residual, up to 2000 iterations, twice per outer
@@ -36,8 +36,6 @@
 // problem (32^3 cells, density ratio 1000) it cuts BiCGSTAB from ~50 to ~15 iterations for a
 // 1e-2 residual reduction.
 #pragma once
-
-#include <hip/hip_cooperative_groups.h>
 
 #include "kernels.h"
 
@@ -166,7 +164,7 @@ struct Bottom {
         }
     }
 
-    // Grid barrier of the cooperative launch as a flag gather.  The memory operations are those
+    // Grid barrier of the persistent launch as a flag gather.  The memory operations are those
     // of grid.sync (workgroup barrier; thread 0: agent-scope release, then the arrival; wait;
     // agent-scope acquire; workgroup barrier), but each workgroup raises its own flag to the
     // episode number and wave 0 waits on every flag at once, one lane per workgroup: no

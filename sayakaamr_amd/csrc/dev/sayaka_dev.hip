@@ -88,6 +88,7 @@ struct sayaka_dev_ctx {
     int* d_bot_stats = nullptr;
     unsigned long long* d_bot_flags = nullptr;   // grid-barrier flags of the bottom solve, one per workgroup
     unsigned long long bot_launches = 0;
+    int bot_fit[2] = {0, 0};                     // workgroups of k_bottom<.., PC> resident at once
     int* h_bot_stats = nullptr;  // pinned
 
     MixedState mixed;
@@ -387,13 +388,32 @@ struct Impl {
         A.s0inv = c->d_pc_s0inv;
         A.jac = c->d_pc_jac;
         A.l1pos = c->d_l1pos;
-        void* args[] = {&A};
-        // iterative and latency-bound: no per-launch byte count
-        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
         using BC = BotCfg<D, NB>;
         const int nwg = (lst.n + BC::BPW - 1) / BC::BPW;
         const void* kern = two_level ? (const void*)k_bottom<D, NB, true> : (const void*)k_bottom<D, NB, false>;
-        HIPCHK(hipLaunchCooperativeKernel(kern, dim3(nwg), dim3(BC::NT), args, 0, c->stream));
+        // The kernel's grid barriers need every workgroup resident at once.  An ordinary launch
+        // is enough for that when the grid fits the chip at the kernel's occupancy (checked
+        // here; kernels earlier on the stream have finished before it starts); the barrier's
+        // bounded wait turns a workgroup that is not scheduled (another process filling the
+        // GPU) into error -5 instead of a hang.  A cooperative launch would add nothing but a
+        // device-wide lock that keeps a second process from solving on the same GPU.
+        int& fit = c->bot_fit[two_level ? 1 : 0];
+        if (fit == 0) {
+            int per_cu = 0, ncu = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BC::NT, 0));
+            HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+            fit = per_cu * ncu;
+        }
+        if (nwg > fit)
+            return fail(-2, "coarsest level too large for one resident grid: " + std::to_string(nwg) +
+                                " workgroups, " + std::to_string(fit) + " fit");
+        // iterative and latency-bound: no per-launch byte count
+        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
+        if (two_level)
+            hipLaunchKernelGGL((k_bottom<D, NB, true>), dim3(nwg), dim3(BC::NT), 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((k_bottom<D, NB, false>), dim3(nwg), dim3(BC::NT), 0, c->stream, A);
+        HIPCHK(hipGetLastError());
         return 0;
     }
     // Two-level preconditioner data for the current coefficients and BCs (bottom.h): the

@@ -138,6 +138,7 @@ class DeviceState:
         p = vp()
         _chk(solver.lib.sayaka_dev_alloc(solver.h, self.nbytes, C.byref(p)), "alloc")
         self.ptr = p.value
+        self._key = N.track(self)   # registered after its context: closed before it at exit
 
     def upload(self, host):
         a = np.ascontiguousarray(host, dtype=np.float64).reshape(self.shape)
@@ -205,6 +206,15 @@ class DeviceState:
         if self.ptr and getattr(self.s, "h", None):
             self.s.lib.sayaka_dev_free(self.s.h, C.c_void_p(self.ptr))
         self.ptr = None
+        N.untrack(getattr(self, "_key", None))
+
+    def __del__(self):
+        if N.finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MacSolverGPU:
@@ -220,13 +230,19 @@ class MacSolverGPU:
         self.h = h
         self.n = 0
         self.finest = 0
+        self._key = N.track(self)
 
     def close(self):
         if getattr(self, "h", None):
             self.lib.sayaka_dev_destroy(self.h)
             self.h = None
+        N.untrack(getattr(self, "_key", None))
 
     def __del__(self):
+        # at interpreter finalization the atexit hook has closed every live context already;
+        # nothing may reach the HIP runtime from a finalizer then
+        if N.finalizing():
+            return
         try:
             self.close()
         except Exception:

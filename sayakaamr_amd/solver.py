@@ -41,6 +41,7 @@ class MacSolver:
         self.ncell = self.nb ** self.ndim
         self._params = dict(tol_rel=1e-10, tol_abs=0.0, max_iter=2000, precond=1, precision=FP64, pre=2, post=2)
         self.stats = {}
+        self._key = N.track(self)
 
     def _chk(self, rc, what):
         if rc < 0:
@@ -51,8 +52,11 @@ class MacSolver:
         if getattr(self, "h", None):
             self.lib.sy_mac_destroy(self.h)
             self.h = None
+        N.untrack(getattr(self, "_key", None))
 
     def __del__(self):
+        if N.finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -15,6 +15,7 @@ class AmrTree:
         self.lib = N.host()
         self.h = None
         self.spec = None
+        self._key = N.track(self)   # registered states may hold device memory: closed at exit
 
     def create(self, spec):
         if self.h:
@@ -31,11 +32,17 @@ class AmrTree:
             raise ValueError(f"bad tree spec {spec}")
         return self.numblocks
 
+    def close(self):
+        if self.h:
+            self.lib.sy_tree_destroy(self.h)
+            self.h = None
+        N.untrack(getattr(self, "_key", None))
+
     def __del__(self):
+        if N.finalizing():
+            return
         try:
-            if self.h:
-                self.lib.sy_tree_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 

@@ -1,9 +1,9 @@
 """A C++ driver written against the MacSolver / AmrTree / TreeData headers (the reference's
 call surface, Test7.h:411-418, 612-653), run as its own process on the GPU.
 
-This file sorts first among the GPU tests on purpose: the bottom solve is a cooperative
-launch, and the HIP runtime refuses a cooperative launch in a second process while another
-process on the card holds one (the pytest process would, once any solve had run in it).
+The driver runs as a second process on the GPU beside the pytest process, which has solved
+before it when the file order puts it later: the coarsest-level solve is an ordinary launch
+with its own grid barrier, so two processes can solve on one GPU.
 
 The driver also writes its FP64 solution through WriteDataHdf5(TreeData) with the raw block
 data and geometry beside it; the file must equal, object by object, the reference's writer
