@@ -110,6 +110,16 @@ int sayaka_dev_prolong(sayaka_dev_ctx* ctx, int fine_level, int field);         
 int sayaka_dev_dot(sayaka_dev_ctx* ctx, int m, int a, int b, double* dot, double* maxabs_a);
                                                      /* mg_dotprod_level, mg_norm_level */
 int sayaka_dev_precond(sayaka_dev_ctx* ctx, int m, int sol, int rhs);            /* mg_precond_level */
+/* sayaka_dev_dot without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  a and b
+ * are device arrays [block][cell] in the context's layout (e.g. from sayaka_dev_alloc at set-up);
+ * out2 is a device array of 2 doubles that receives dot(a, b) over tower m (summed in the same
+ * order as sayaka_dev_dot, so the result is bit-identical to it) and max|a|.  Stream-ordered on
+ * the context's stream: it returns without waiting (sayaka_dev_sync, or a copy on that stream,
+ * orders a read of out2).  The per-block partials live in a fixed file-scope device array of
+ * 1.5 MiB per GPU, so towers of at most 65 536 blocks (larger: -1); every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other streams, other contexts) race. */
+int sayaka_dev_dot_noalloc(sayaka_dev_ctx* ctx, int m, const double* a, const double* b, double* out2);
 int sayaka_dev_add(sayaka_dev_ctx* ctx, int m, int dst, int src);                /* mg_correct_level:
                                                         dst += src on tower m (SayakaMacUtil.cpp:109-133) */
 int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, double tol_abs, int max_iter,

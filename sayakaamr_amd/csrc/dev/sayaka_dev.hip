@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,29 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int MAXLV = 16;
+
+// Allocation-free tower dot product (sayaka_dev_dot_noalloc): its per-workgroup partials and
+// scalars live in this file-scope array, one copy per GPU, held until the process ends.  Host
+// code must not pass the array's name to a kernel (that is a host-side stand-in, not a device
+// address): the device address of each GPU's copy is looked up once, at context creation
+// with that GPU current, and kept in scratch_dev[device].
+constexpr int NOALLOC_MAX_BLOCKS = 1 << 16;                       // blocks of one tower
+constexpr size_t SCRATCH_DOUBLES = 3 * (size_t)NOALLOC_MAX_BLOCKS + 64;   // partials, then scalars
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+constexpr int SCRATCH_MAX_DEV = 64;
+std::mutex scratch_mu;
+double* scratch_dev[SCRATCH_MAX_DEV] = {};
+
+// device address of this GPU's scratch array (the calling thread has made `device` current)
+hipError_t scratch_lookup(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (scratch_dev[device]) return hipSuccess;
+    double* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) scratch_dev[device] = ptr;
+    return e;
+}
 
 struct ListRef {
     int off = 0, n = 0;
@@ -153,6 +177,13 @@ struct sayaka_dev_ctx {
 namespace {
 
 void cnn_free(sayaka_dev_ctx::Cnn& n);
+
+__global__ void k_copy_dot_norm(const double* __restrict__ scal, double* __restrict__ out2) {
+    if (threadIdx.x == 0) {
+        out2[0] = scal[S_DOT];
+        out2[1] = scal[S_NORM];
+    }
+}
 
 __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -893,6 +924,24 @@ struct Impl {
         return 0;
     }
 
+    // dot over tower m with the partials and scalars in `scr` (the GPU's scratch array), the
+    // two results copied to out2 on the device; nothing allocated, nothing waited for
+    static int dot_noalloc(sayaka_dev_ctx* c, int m, const double* x, const double* y, double* out2, double* scr) {
+        ListRef T = c->tw[m];
+        RedBuf rb{scr, scr + 3 * (size_t)NOALLOC_MAX_BLOCKS, c->exact};
+        if (T.n > 0) {
+            VecArgs a = vec_args(c, T);
+            a.src = x;
+            a.t = y;
+            hipLaunchKernelGGL((k_vec<G::NCELL, G::THREADS>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a, (int)V_DOT,
+                               rb, (int)FIN_DOTNORM);
+        }
+        hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, rb, T.n, 1, (int)FIN_DOTNORM);
+        hipLaunchKernelGGL(k_copy_dot_norm, dim3(1), dim3(64), 0, c->stream, (const double*)rb.scal, out2);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // MacSolver::cg_solve_level on the finest tower (SayakaMacSolver.cpp:683-920), scalars
     // kept on the device; the host reads them back only at the two convergence tests of
     // each iteration.
@@ -1070,7 +1119,8 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     c->nface = (nb + 1) * nb * nbz;
     c->nslab = nb * nbz;
     c->nf = 2 * ndim;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = scratch_lookup(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     const size_t cells = (size_t)max_blocks * c->ncell;
     const size_t faces = (size_t)max_blocks * ndim * c->nface;
     for (int f = 0; f < SAYAKA_NFIELD && e == hipSuccess; f++) e = hipMalloc(&c->d_field[f], cells * sizeof(double));
@@ -1763,6 +1813,17 @@ int sayaka_dev_flux(sayaka_dev_ctx* c, int m, int dir, int phi, double* flux_out
 int sayaka_dev_dot(sayaka_dev_ctx* c, int m, int a, int b, double* dotv, double* mx) {
     if (!c || !field_ok(a) || !field_ok(b) || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
     return dispatch(c, [&](auto I) { return decltype(I)::dot(c, m, c->d_field[a], c->d_field[b], dotv, mx); });
+}
+
+int sayaka_dev_dot_noalloc(sayaka_dev_ctx* c, int m, const double* a, const double* b, double* out2) {
+    if (!c || !a || !b || !out2 || m < 1 || m > c->maxlevel) return fail(-1, "bad argument");
+    if (c->tw[m].n > NOALLOC_MAX_BLOCKS)
+        return fail(-1, "sayaka_dev_dot_noalloc: tower of " + std::to_string(c->tw[m].n) + " blocks exceeds the " +
+                            std::to_string(NOALLOC_MAX_BLOCKS) + "-block scratch");
+    double* scr = (c->device >= 0 && c->device < SCRATCH_MAX_DEV) ? scratch_dev[c->device] : nullptr;
+    if (!scr) return fail(-3, "sayaka_dev_dot_noalloc: no scratch address for this device");
+    HIPCHK(hipSetDevice(c->device));
+    return dispatch(c, [&](auto I) { return decltype(I)::dot_noalloc(c, m, a, b, out2, scr); });
 }
 
 int sayaka_dev_precond(sayaka_dev_ctx* c, int m, int sol, int rhs) {

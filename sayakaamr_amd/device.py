@@ -49,6 +49,7 @@ _SIG = {
     "sayaka_dev_restrict": ([vp, i32, i32], i32),
     "sayaka_dev_prolong": ([vp, i32, i32], i32),
     "sayaka_dev_dot": ([vp, i32, i32, i32, C.POINTER(f64), C.POINTER(f64)], i32),
+    "sayaka_dev_dot_noalloc": ([vp, i32, vp, vp, vp], i32),
     "sayaka_dev_precond": ([vp, i32, i32, i32], i32),
     "sayaka_dev_add": ([vp, i32, i32, i32], i32),
     "sayaka_dev_bottom": ([vp, i32, i32, f64, f64, i32, C.POINTER(i32), C.POINTER(i32)], i32),
