@@ -36,6 +36,7 @@ Reported (one JSON line on stdout):
 from __future__ import annotations
 
 import argparse
+import csv
 import glob
 import json
 import os
@@ -50,6 +51,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "c1_fp64_smooth_traffic.md")
+# Algorithmic bytes of one coarsest-level BiCGSTAB iteration per level-1 cell (FP64, nb = 8):
+# two operator applications 2 x (3P + 3fP) = 102 B (P = 8 B, f = 9/8); the inner products
+# rhat.v, t.s, t.t, rhat.r 2P + 2P + P + 2P = 56 B; the norms |s|, |r| 2P = 16 B; the updates
+# p = r + beta (p - omega v) 4P, s = r - alpha v 3P, x += alpha p and x += omega s 2 x 3P,
+# r = s - omega t 3P = 128 B.  The kernel keeps every vector in registers and LDS, so this is what
+# the same iteration moves as separate HBM-bound kernels.
+BOTTOM_BYTES_PER_CELL_ITER = 302.0
 
 
 def log(msg):
@@ -97,9 +105,10 @@ def rel_l2(a, b, leaf):
     return float(np.linalg.norm(a[leaf] - b[leaf]) / np.linalg.norm(b[leaf]))
 
 
-def time_solves(dev, D, precision, steps, prof_class=None):
-    if prof_class is not None:
-        dev.profile(prof_class, True)
+def time_solves(dev, D, precision, steps, prof_classes=()):
+    """steps solves from a zero guess; the launches of prof_classes timed by their own events."""
+    for k in prof_classes:
+        dev.profile(k, True)
     dev.sync()
     t0 = time.perf_counter()
     res = None
@@ -110,10 +119,11 @@ def time_solves(dev, D, precision, steps, prof_class=None):
             raise RuntimeError(f"solve did not converge: {res}")
     dev.sync()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    prof = dev.profile_read(prof_class) if prof_class is not None else None
-    if prof_class is not None:
-        dev.profile(prof_class, False)
-    return ms, res, prof
+    profs = {}
+    for k in prof_classes:
+        profs[k] = dev.profile_read(k)
+        dev.profile(k, False)
+    return ms, res, profs
 
 
 def class_profile(dev, D, precision):
@@ -319,27 +329,34 @@ def output_leg(D, W, spec, tree, tb, sol):
                 regrid_blocks=n2, regrid_new_blocks=int((origin < 0).sum()), state_remap_ms=round(remap_ms, 3))
 
 
-def large_leg(D, W):
-    """16^3-block snapshot (139 M leaf cells): FP64 operator and smoother GB/s with the
-    fields far beyond the caches, and one mixed solve."""
+def build_large(D, W):
+    """The 16^3-block snapshot on the device: tree, sphere coefficients, Gaussian-blob rhs."""
     from sayakaamr_amd.tree import AmrTree
 
     spec = W.C4_SPEC
-    t0 = time.perf_counter()
     tb = AmrTree(3, max_blocks=45000)
     tree = tb.build_shell(spec, (0.5, 0.5, 0.5), W.C4_RADIUS, W.SHELL_FACTOR)
     n = tree["level"].shape[0]
+    dev = D.MacSolverGPU(3, spec.nb, n + 64, device=0)
+    dev.prepare(tree, spec)
+    dev.set_bc([W.WALL], [W.NEUMANN])
+    dxf = (spec.hi[0] - spec.lo[0]) / spec.ndiv[0] / spec.nb / 2 ** (spec.maxlevel - 1)
+    rng = np.random.default_rng(7)
+    blobs = dict(c=rng.uniform(0, 1, (8, 3)), w=rng.uniform(0.02, 0.1, 8), a=rng.normal(0, 1, 8))
+    dev.gen_sphere_blobs(tree, (0.5, 0.5, 0.5), W.C4_RADIUS, 1000.0, 1.5 * dxf, 4, blobs)
+    dev.sync()
+    return spec, tb, tree, dev
+
+
+def large_leg(D, W):
+    """16^3-block snapshot (139 M leaf cells): FP64 operator and smoother GB/s with the
+    fields far beyond the caches, and one mixed solve."""
+    t0 = time.perf_counter()
+    spec, tb, tree, dev = build_large(D, W)
+    n = tree["level"].shape[0]
     L = int(tree["level"].max())
     leaf = tree["node_type"] == 1
-    dev = D.MacSolverGPU(3, spec.nb, n + 64, device=0)
     try:
-        dev.prepare(tree, spec)
-        dev.set_bc([W.WALL], [W.NEUMANN])
-        dxf = (spec.hi[0] - spec.lo[0]) / spec.ndiv[0] / spec.nb / 2 ** (spec.maxlevel - 1)
-        rng = np.random.default_rng(7)
-        blobs = dict(c=rng.uniform(0, 1, (8, 3)), w=rng.uniform(0.02, 0.1, 8), a=rng.normal(0, 1, 8))
-        dev.gen_sphere_blobs(tree, (0.5, 0.5, 0.5), W.C4_RADIUS, 1000.0, 1.5 * dxf, 4, blobs)
-        dev.sync()
         log(f"large: {n} blocks, {int(leaf.sum())} leaves ({time.perf_counter() - t0:.1f} s)")
         out = dict(blocks=n, leaf_blocks=int(leaf.sum()), leaf_cells=int(leaf.sum()) * spec.nb ** 3,
                    workload="16^3 blocks, root 4^3, levels 1-6 around an r=0.15 sphere, density ratio 1000, "
@@ -361,6 +378,19 @@ def large_leg(D, W):
                         alg_bytes_per_launch=round(p["alg_bytes"] / p["launches"], 1), achieved=round(gbs, 1),
                         frac=round(gbs / HBM_PEAK_GBS, 4))
 
+        # copy peak: a device-to-device copy of one 1.27 GB field (the runtime's copy kernel), host-
+        # timed per copy including the synchronisation; reads + writes counted
+        nbytes = n * spec.nb ** 3 * 8
+        dev.copy(D.X, D.RHS)
+        cps = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            dev.copy(D.X, D.RHS)
+            cps.append(time.perf_counter() - t1)
+        cp = 2 * nbytes / min(cps) / 1e9
+        out["copy_peak"] = dict(bytes_moved=2 * nbytes, best_ms=round(min(cps) * 1e3, 3), gbs=round(cp, 1),
+                                frac_of_8tbs=round(cp / HBM_PEAK_GBS, 4))
+        log(f"large: device copy {cp:.0f} GB/s")
         out["apply"] = klass("apply", lambda: dev.apply(L, D.V, D.RHS), 5)
         out["smooth"] = klass("smooth", lambda: dev.smooth(L, D.SOL, D.RHS), 3)
         log(f"large: apply {out['apply']['achieved']:.0f} GB/s, smooth {out['smooth']['achieved']:.0f} GB/s")
@@ -378,14 +408,18 @@ def large_leg(D, W):
 
 
 def read_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter summary."""
+    """HBM bytes per launch of the coarsest-level solve and of the smoother, from the committed
+    counter summary."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             txt = f.read()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
-    return float(m.group(1)) if m else None
+    out = {}
+    for key, tag in (("smooth", "traffic_bytes_per_launch"), ("bottom", "bottom_traffic_bytes_per_launch")):
+        m = re.search(rf"^{tag}:\s*([0-9.eE+]+)", txt, re.M)
+        out[key] = float(m.group(1)) if m else None
+    return out
 
 
 # ------------------------------------------------------------------------- counter runs
@@ -416,20 +450,37 @@ def pmc_workload():
                           fill_write_bytes=n * spec.nb ** 3 * 8)), flush=True)
 
 
-def summarize_pmc(dirs):
-    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel from rocprofv3 csv
-    output, the calibration factors from the dot / fill kernels, and the smoother's HBM
-    bytes per launch, as a markdown summary on stdout."""
-    import csv
-
+def pmc_workload_large():
+    """The program the counter passes of the 16^3-block snapshot profile: 3 device-to-device
+    copies of a 1.27 GB field (the copy peak), 3 FP64 operator applications and 2 FP64 smoother
+    passes (every sub-level, both colours) over the finest
+    tower, then the calibration kernels (a two-field dot over the finest tower reads 2 x 8 B per
+    cell; a fill of all blocks writes 8 B per cell).  The fields are 1.27 GB each: nothing of
+    one launch is left in a cache for the next."""
+    from sayakaamr_amd import device as D
     from sayakaamr_amd import workloads as W
 
-    spec, tree, _, _ = build_workload()
+    spec, tb, tree, dev = build_large(D, W)
     L = int(tree["level"].max())
-    n = tree["level"].shape[0]
     lv, leaf = tree["level"], tree["node_type"] == 1
     tw = int((((lv < L) & leaf) | (lv == L)).sum())
-    dot_bytes, fill_bytes = 2 * tw * spec.nb ** 3 * 8, n * spec.nb ** 3 * 8
+    n = tree["level"].shape[0]
+    dev.zero(D.SOL)
+    for _ in range(3):
+        dev.copy(D.X, D.RHS)      # the runtime's device-to-device copy: the copy peak
+    for _ in range(3):
+        dev.apply(L, D.V, D.RHS)
+    for _ in range(2):
+        dev.smooth(L, D.SOL, D.RHS)
+    dev.zero(D.T)
+    dev.dot(L, D.X0, D.RHS)
+    dev.sync()
+    dev.close()
+    print(json.dumps(dict(blocks=n, tower_blocks=tw, dot_read_bytes=2 * tw * spec.nb ** 3 * 8,
+                          fill_write_bytes=n * spec.nb ** 3 * 8)), flush=True)
+
+
+def _read_pmc(dirs):
     vals = {}   # (kernel, counter) -> [(dispatch order, grid size, KiB)]
     for d in dirs:
         for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
@@ -439,6 +490,61 @@ def summarize_pmc(dirs):
                     order = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or i)
                     vals.setdefault(k, []).append((order, int(row.get("Grid_Size", 0) or 0),
                                                    float(row["Counter_Value"])))
+    return vals
+
+
+def summarize_pmc_large(dirs):
+    """Counter bytes against algorithmic bytes of the snapshot's FP64 operator and smoother:
+    per dispatch, blocks = grid size / 512 work-items, algorithmic bytes per block as coded in
+    AlgBytes (apply 3C + Fw, one colour pass (4C + Fw) / 2; C = 4096 x 8 B, Fw = 3 x 17 x 16 x 16
+    x 8 B), counter bytes scaled by the calibration kernels."""
+    C_ = 4096 * 8.0
+    Fw = 3 * 17 * 16 * 16 * 8.0
+    vals = _read_pmc(dirs)
+    dot = [v for (k, c), lst in vals.items() if c == "FETCH_SIZE" and "k_vec<" in k for v in lst]
+    fill = [v for (k, c), lst in vals.items() if c == "WRITE_SIZE" and "k_vec<" in k for v in lst]
+    dot.sort()
+    fill.sort()
+    # the last k_vec dispatch is the dot; the fill is the k_vec just before it
+    lines = []
+    kf = kw = None
+    if dot and fill:
+        gdot = dot[-1][1]
+        kf = (gdot / 512) * 4096 * 16 / (dot[-1][2] * 1024)
+        gfill = fill[-2][1] if len(fill) > 1 else fill[-1][1]
+        kw = (gfill / 512) * 4096 * 8 / (fill[-2][2] * 1024 if len(fill) > 1 else fill[-1][2] * 1024)
+        lines.append(f"calibration: dot over {gdot // 512} blocks -> FETCH factor {kf:.4f}; "
+                     f"fill of {gfill // 512} blocks -> WRITE factor {kw:.4f}")
+    for name, mode, per_block in (("apply", 0, 3 * C_ + Fw), ("smoother colour pass", 2, 0.5 * (4 * C_ + Fw))):
+        kre = f"k_stencil<3, 16, {mode}, double>"
+        fe = [v for (k, c), lst in vals.items() if c == "FETCH_SIZE" and kre in k for v in lst]
+        wr = [v for (k, c), lst in vals.items() if c == "WRITE_SIZE" and kre in k for v in lst]
+        if not fe or not wr or kf is None:
+            continue
+        alg = sum(g / 512 * per_block for _, g, _ in fe)
+        fb = sum(v for _, _, v in fe) * 1024 * kf
+        wb = sum(v for _, _, v in wr) * 1024 * kw
+        cells = sum(g / 512 * 4096 for _, g, _ in fe)
+        lines.append(f"{name}: {len(fe)} dispatches, algorithmic {alg / cells:.2f} B/cell, counter read "
+                     f"{fb / cells:.2f} B/cell + write {wb / cells:.2f} B/cell = {(fb + wb) / alg:.3f} x algorithmic")
+    for (k, c), lst in sorted(vals.items()):
+        lines.append(f"  {c:10s} {np.mean([v for _, _, v in lst]):14.1f} KiB x{len(lst):4d}  {k[:90]}")
+    print("\n".join(lines))
+
+
+def summarize_pmc(dirs):
+    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel from rocprofv3 csv
+    output, the calibration factors from the dot / fill kernels, and the smoother's HBM
+    bytes per launch, as a markdown summary on stdout."""
+    from sayakaamr_amd import workloads as W
+
+    spec, tree, _, _ = build_workload()
+    L = int(tree["level"].max())
+    n = tree["level"].shape[0]
+    lv, leaf = tree["level"], tree["node_type"] == 1
+    tw = int((((lv < L) & leaf) | (lv == L)).sum())
+    dot_bytes, fill_bytes = 2 * tw * spec.nb ** 3 * 8, n * spec.nb ** 3 * 8
+    vals = _read_pmc(dirs)
 
     def pick(kname_re, counter, grid=None, last=False):
         xs = sorted((o, v) for (k, c), lst in vals.items() if c == counter and re.search(kname_re, k)
@@ -463,6 +569,12 @@ def summarize_pmc(dirs):
         traffic = fetch_s * 1024 * kf + write_s * 1024 * kw
         lines.append(f"smoother: FETCH {fetch_s:.1f} KiB x{nf}, WRITE {write_s:.1f} KiB x{nw} per dispatch")
         lines.append(f"traffic_bytes_per_launch: {traffic:.0f}")
+    fetch_b, nfb = pick(r"k_bottom<3, 8, false>", "FETCH_SIZE")
+    write_b, nwb = pick(r"k_bottom<3, 8, false>", "WRITE_SIZE")
+    if fetch_b is not None and write_b is not None and kf and kw:
+        tb_ = fetch_b * 1024 * kf + write_b * 1024 * kw
+        lines.append(f"coarsest-level solve: FETCH {fetch_b:.1f} KiB x{nfb}, WRITE {write_b:.1f} KiB x{nwb} per dispatch")
+        lines.append(f"bottom_traffic_bytes_per_launch: {tb_:.0f}")
     print("\n".join(lines))
 
 
@@ -478,8 +590,12 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 139 M-cell snapshot")
     ap.add_argument("--no-moving", action="store_true", help="skip the moving-boundary cycle")
     ap.add_argument("--moving-steps", type=int, default=10)
-    ap.add_argument("--pmc-workload", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-fast", action="store_true", help="skip the order-free FP64 leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-output", action="store_true", help="skip the output / state-remap leg")
+    ap.add_argument("--pmc-workload", nargs="?", const="c1", choices=("c1", "large"), help=argparse.SUPPRESS)
     ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR", help=argparse.SUPPRESS)
+    ap.add_argument("--summarize-pmc-large", nargs="+", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the solve is "
@@ -487,9 +603,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.pmc_workload:
-        return pmc_workload()
+        return pmc_workload() if args.pmc_workload == "c1" else pmc_workload_large()
     if args.summarize_pmc:
         return summarize_pmc(args.summarize_pmc)
+    if args.summarize_pmc_large:
+        return summarize_pmc_large(args.summarize_pmc_large)
 
     from sayakaamr_amd import device as D
     from sayakaamr_amd import workloads as W
@@ -510,34 +628,40 @@ def main():
     log(f"fp64 classes: {classes64}")
     timed = {k: v["ms"] for k, v in classes64.items()}
     dom_name = max(timed, key=timed.get)
-    hbm_classes = {k: v for k, v in timed.items() if classes64[k]["alg_gbs"] is not None}
-    roof_name = dom_name if classes64[dom_name]["alg_gbs"] is not None else max(hbm_classes, key=hbm_classes.get)
-    roof = D.KCLASS_NAMES.index(roof_name)
 
-    # timed region: FP64 solves, events only around the roofline class's launches
-    ms64, r64, prof = time_solves(dev, D, D.FP64, args.steps, prof_class=roof)
+    # timed region: FP64 solves; the coarsest-level solve (the dominant kernel) and the smoother
+    # timed by their launches' own dispatch events on the solver's stream
+    kb, ks = D.KCLASS_NAMES.index("bottom"), D.KCLASS_NAMES.index("smooth")
+    ms64, r64, profs = time_solves(dev, D, D.FP64, args.steps, prof_classes=(kb, ks))
     sol64 = dev.download(D.SOL)
     log(f"fp64: {ms64:.2f} ms/solve, iters {r64['iters']}, rnorm {r64['rnorm']:.3e} / rnorm0 {r64['rnorm0']:.3e}")
-    achieved = prof["alg_bytes"] / (prof["total_ms"] * 1e-3) / 1e9
-    avg_launch_us = prof["total_ms"] * 1e3 / prof["launches"]
-    bytes_per_launch = prof["alg_bytes"] / prof["launches"]
+    pb, psm = profs[kb], profs[ks]
+    l1_cells = int((tree["level"] == 1).sum()) * spec.nb ** 3
+    bot_iters = args.steps * r64["bottom_iters"]
+    bot_bytes = BOTTOM_BYTES_PER_CELL_ITER * l1_cells * bot_iters
+    bot_gbs = bot_bytes / (pb["total_ms"] * 1e-3) / 1e9
+    sm_gbs = psm["alg_bytes"] / (psm["total_ms"] * 1e-3) / 1e9
+    traffic = read_traffic()
 
-    # FP64 with order-free sums
-    dev.zero(D.SOL)
-    dev.solve(precision=D.FP64_FAST)
-    msf, rf, _ = time_solves(dev, D, D.FP64_FAST, args.steps)
-    solf = dev.download(D.SOL)
-    _, classes_fast = class_profile(dev, D, D.FP64_FAST)
-    fast = dict(ms_per_solve=round(msf, 3), iters=rf["iters"], rel_resid=float(rf["rnorm"] / rf["rnorm0"]),
-                rel_l2_vs_fp64=rel_l2(gauge(solf, tree, spec), gauge(sol64, tree, spec), leaf),
-                kernels=classes_fast)
-    log(f"fp64 order-free sums: {msf:.2f} ms/solve, iters {rf['iters']}")
-
-    mixed_rows, ms_mixed = mixed_leg(dev, D, W, spec, tree, inp, args.seeds, args.steps)
-    rmx, classes_mixed = class_profile(dev, D, D.MIXED)
+    fast, ms_mixed, mixed_rows, classes_mixed, rmx, out_leg = None, None, None, None, None, None
+    if not args.no_fast:
+        # FP64 with order-free sums
+        dev.zero(D.SOL)
+        dev.solve(precision=D.FP64_FAST)
+        msf, rf, _ = time_solves(dev, D, D.FP64_FAST, args.steps)
+        solf = dev.download(D.SOL)
+        _, classes_fast = class_profile(dev, D, D.FP64_FAST)
+        fast = dict(ms_per_solve=round(msf, 3), iters=rf["iters"], rel_resid=float(rf["rnorm"] / rf["rnorm0"]),
+                    rel_l2_vs_fp64=rel_l2(gauge(solf, tree, spec), gauge(sol64, tree, spec), leaf),
+                    kernels=classes_fast)
+        log(f"fp64 order-free sums: {msf:.2f} ms/solve, iters {rf['iters']}")
+    if not args.no_mixed:
+        mixed_rows, ms_mixed = mixed_leg(dev, D, W, spec, tree, inp, args.seeds, args.steps)
+        rmx, classes_mixed = class_profile(dev, D, D.MIXED)
     dev.close()
-    out_leg = output_leg(D, W, spec, tree, tb, sol64)
-    log(f"output: {out_leg}")
+    if not args.no_output:
+        out_leg = output_leg(D, W, spec, tree, tb, sol64)
+        log(f"output: {out_leg}")
 
     learned = None if args.no_learned else learned_leg(D, args.learned)
     if learned:
@@ -593,7 +717,8 @@ def main():
                  "vcycle_leaf_cells_per_s": round(nleaf_cells / (vc_ms / max(nvc, 1) * 1e-3), 1),
                  "kernels": classes64},
         "fp64_fast": fast,
-        "mixed": {"ms_per_solve": round(ms_mixed, 3),
+        "mixed": None if ms_mixed is None else {
+                  "ms_per_solve": round(ms_mixed, 3),
                   "speedup_vs_fp64": round(ms64 / ms_mixed, 3),
                   "dtype": "f32 V(2,2)-cycle, f64 outer BiCGSTAB and coarsest solve",
                   "coarse_iters_per_call": round(rmx["bottom_iters"] / max(rmx["bottom_calls"], 1), 1),
@@ -602,11 +727,25 @@ def main():
         "hbm_regime": large,
         "moving": moving,
         "output": out_leg,
-        "roofline": {"bound": "hbm", "kernel": roof_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-                     "launches": prof["launches"], "avg_launch_us": round(avg_launch_us, 3),
-                     "alg_bytes_per_launch": round(bytes_per_launch, 1),
-                     "dominant_class": dom_name},
+        "roofline": {"bound": "hbm", "kernel": "k_bottom<3, 8, false> (coarsest-level BiCGSTAB, FP64)",
+                     "achieved": round(bot_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bot_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else traffic.get("bottom"),
+                     "launches": pb["launches"], "avg_launch_us": round(pb["total_ms"] * 1e3 / pb["launches"], 3),
+                     "alg_bytes_per_launch": round(bot_bytes / pb["launches"], 1),
+                     "iterations_per_launch": round(bot_iters / pb["launches"], 2),
+                     "us_per_iteration": round(pb["total_ms"] * 1e3 / bot_iters, 3),
+                     "alg_bytes_per_cell_iteration": BOTTOM_BYTES_PER_CELL_ITER, "level1_cells": l1_cells,
+                     "dominant_class": dom_name,
+                     "note": "latency-bound: three ordered 512-term block sums and three grid barriers per "
+                             "iteration (the reference's summation order); vectors stay in registers / LDS"},
+        "roofline_smoother": {"bound": "hbm", "kernel": "k_stencil<3, 8, 2, double> (FP64 red-black colour pass)",
+                              "achieved": round(sm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(sm_gbs / HBM_PEAK_GBS, 4),
+                              "traffic": None if traffic is None else traffic.get("smooth"),
+                              "launches": psm["launches"],
+                              "avg_launch_us": round(psm["total_ms"] * 1e3 / psm["launches"], 3),
+                              "alg_bytes_per_launch": round(psm["alg_bytes"] / psm["launches"], 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -493,11 +493,24 @@ __device__ inline void load_cell(CellIn<T>& in, const StencilArgsT<T>& a, int b,
     if (MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) in.dotv = a.dotv[gi];
 #pragma unroll
     for (int f = 0; f < G::NF; f++) {
-        const int d = f >> 1, s = f & 1;
+        const int d = f >> 1;
         in.w[f] = a.w[(b * D + d) * G::NFACE + G::fidx(d, i + (f == 1), j + (f == 3), k + (f == 5))];
+    }
+}
+
+// The coarse-side weights of the cell's coarse/fine faces (the only faces that use them), once
+// the block table is in LDS; issued beside the halo gathers, so they add no round trip.
+template <int D, int NB, class T>
+__device__ inline void load_wcf(CellIn<T>& in, const StencilArgsT<T>& a, int b, int c, const BlockInfo& bi) {
+    using G = Geo<D, NB>;
+    int i, j, k;
+    G::cell_ijk(c, i, j, k);
+#pragma unroll
+    for (int f = 0; f < G::NF; f++) {
+        const int d = f >> 1, s = f & 1;
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int ext = d == 2 ? G::NBZ : NB;
-        if (s ? idx == ext - 1 : idx == 0) {
+        if ((s ? idx == ext - 1 : idx == 0) && face_type<D, NB>(bi, f, a.m) == FT_CF) {
             int a1, a2;
             tang_axes<D>(d, a1, a2);
             const int ijk[3] = {i, j, k};
@@ -557,6 +570,12 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
         tile[G::tidx(i, j, k)] = xo[q];
     }
     load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
+    if (PRE) {
+        const int c = threadIdx.x;
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        if (c < G::NCELL && (!SMOOTHING || ((i + j + k) & 1) == a.colour)) load_wcf<D, NB, T>(in[0], a, b, c, bi);
+    }
     __syncthreads();
     double q0[CPT], q1[CPT], mx = 0.0;
 #pragma unroll
@@ -574,6 +593,7 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
                 continue;
             }
             load_cell<D, NB, MODE, T>(in[0], a, b, c, true);
+            load_wcf<D, NB, T>(in[0], a, b, c, bi);
         }
         const T vfv = in[0].vf;
         if (SMOOTHING) {

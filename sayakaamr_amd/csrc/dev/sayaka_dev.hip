@@ -178,6 +178,15 @@ namespace {
 
 void cnn_free(sayaka_dev_ctx::Cnn& n);
 
+// Streaming device copy (TreeData::Copy of a whole field): one 16-byte element per lane and a
+// grid that covers the field.  Measured on MI355X over 2 GiB: 6.2 TB/s (reads + writes), against
+// 4.5-5.2 TB/s for grid-stride loops of 1 024-8 192 workgroups and 5.0 TB/s for hipMemcpy.
+typedef double v2d __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_copy16(const v2d* __restrict__ src, v2d* __restrict__ dst, size_t n2) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n2) dst[i] = src[i];
+}
+
 __global__ void k_copy_dot_norm(const double* __restrict__ scal, double* __restrict__ out2) {
     if (threadIdx.x == 0) {
         out2[0] = scal[S_DOT];
@@ -438,12 +447,15 @@ struct Impl {
         if (nwg > fit)
             return fail(-2, "coarsest level too large for one resident grid: " + std::to_string(nwg) +
                                 " workgroups, " + std::to_string(fit) + " fit");
-        // iterative and latency-bound: no per-launch byte count
-        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0);
+        // iterative: its algorithmic bytes depend on the iteration count, which the caller
+        // reads back (bytes recorded as 0); timed by the launch's own start / stop events
+        ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0, true);
         if (two_level)
-            hipLaunchKernelGGL((k_bottom<D, NB, true>), dim3(nwg), dim3(BC::NT), 0, c->stream, A);
+            hipExtLaunchKernelGGL((k_bottom<D, NB, true>), dim3(nwg), dim3(BC::NT), 0, c->stream, ps.start(),
+                                  ps.stop(), 0, A);
         else
-            hipLaunchKernelGGL((k_bottom<D, NB, false>), dim3(nwg), dim3(BC::NT), 0, c->stream, A);
+            hipExtLaunchKernelGGL((k_bottom<D, NB, false>), dim3(nwg), dim3(BC::NT), 0, c->stream, ps.start(),
+                                  ps.stop(), 0, A);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -1863,9 +1875,15 @@ int sayaka_dev_solve(sayaka_dev_ctx* c, const sayaka_solve_params* p, sayaka_sol
 int sayaka_dev_copy(sayaka_dev_ctx* c, int dst, int src) {
     if (!c || !field_ok(dst) || !field_ok(src)) return fail(-1, "bad argument");
     HIPCHK(hipSetDevice(c->device));
-    if (dst != src)
-        HIPCHK(hipMemcpyAsync(c->d_field[dst], c->d_field[src], (size_t)c->nblk * c->ncell * sizeof(double),
-                              hipMemcpyDeviceToDevice, c->stream));
+    if (dst != src) {
+        const size_t n2 = (size_t)c->nblk * c->ncell / 2;   // nb^d is even
+        const size_t nwg = (n2 + 255) / 256;
+        if (n2 > 0)
+            hipLaunchKernelGGL(k_copy16, dim3((unsigned)nwg), dim3(256), 0, c->stream,
+                               reinterpret_cast<const v2d*>(c->d_field[src]),
+                               reinterpret_cast<v2d*>(c->d_field[dst]), n2);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
