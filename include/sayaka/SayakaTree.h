@@ -1,0 +1,4 @@
+// SayakaTree.h -- the reference header name of this part of the call surface (a driver written for
+// the reference includes it unchanged); the declarations are in amr_tree.h.
+#pragma once
+#include "amr_tree.h"

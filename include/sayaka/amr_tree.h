@@ -9,6 +9,7 @@
 
 #include <vector>
 
+#include "index_box.h"
 #include "sayaka_types.h"
 
 namespace sayaka {
@@ -16,8 +17,16 @@ inline namespace amd {
 
 struct RealBox {
     Vector3d vlo, vhi;
+    RealBox() = default;
+    RealBox(const Vector3d& lo, const Vector3d& hi) : vlo(lo), vhi(hi) {}
+    RealBox(const double lo[], const double hi[])
+        : vlo(Vector3d::VecMake(lo[0], lo[1], lo[2])), vhi(Vector3d::VecMake(hi[0], hi[1], hi[2])) {}
+    const Vector3d& lo() const { return vlo; }
+    const Vector3d& hi() const { return vhi; }
     double center(int d) const { return 0.5 * (vlo(d) + vhi(d)); }
     double length(int d) const { return vhi(d) - vlo(d); }
+    Vector3d center() const { return Vector3d::VecMake(center(0), center(1), center(2)); }
+    Vector3d length() const { return Vector3d::VecMake(length(0), length(1), length(2)); }
 };
 
 struct AmrTreeNode {
@@ -34,6 +43,14 @@ struct AmrTreeNode {
     RealBox boundBox;
 
     int getLevel() const { return refineLevel; }
+    void setLevel(int level) { refineLevel = level; }
+    // bounds of a root block set by hand (SayakaTree.h:140-150): box, centre and edge lengths
+    void setBoundBox(const RealBox& bb) {
+        boundBox = bb;
+        blockCenter = bb.center();
+        blockLength = bb.length();
+    }
+    void setBound(const double xlo[], const double xhi[]) { setBoundBox(RealBox(xlo, xhi)); }
     bool isLeaf() const { return nodeType == NodeIsLeaf; }
     bool isNotLeaf() const { return !isLeaf(); }
     bool isParentOfLeafChild() const { return nodeType == NodeHasLeafChild; }
@@ -51,6 +68,27 @@ struct SurroundingBlocks {
     int& at(int i, int j, int k) { return blocks[i + 1][j + 1][k + 1]; }
     int at(int i, int j, int k) const { return blocks[i + 1][j + 1][k + 1]; }
     void setUnused();
+};
+
+// The blocks of one tree level in ascending id (AmrTreeLevel, SayakaTree.h:154-189).
+struct AmrTreeLevel {
+    int level = 0;
+    int numLevelBlock = 0;
+    std::vector<int> levelBlock;
+    bool isEmptyLevel() const { return numLevelBlock == 0; }
+    const std::vector<int>& blockIds() const { return levelBlock; }
+    const int& operator[](int i) const { return levelBlock[i]; }
+    int size() const { return numLevelBlock; }
+    std::vector<int>::const_iterator begin() const { return levelBlock.begin(); }
+    std::vector<int>::const_iterator end() const { return levelBlock.end(); }
+    void addBlock(int b) {
+        levelBlock.push_back(b);
+        numLevelBlock++;
+    }
+    void clear() {
+        levelBlock.clear();
+        numLevelBlock = 0;
+    }
 };
 
 class TreeData;
@@ -71,9 +109,13 @@ struct TreeStateData {
 };
 
 struct AmrTree {
-    // configuration (set before init())
+    // configuration (set before init()): either ndim and nb, or the reference's index boxes
+    // (blockValidIndex {0,0,0}..{nb-1,nb-1,nb-1}, or ..{nb-1,nb-1,0} in 2D), from which init()
+    // takes ndim and nb
     int ndim = 3;
     int nb = 8;            // cells per block edge
+    IndexBox blockValidIndex;   // valid cells of a block (SayakaTree.h:207-209)
+    IndexBox blockGrownIndex;   // valid and ghost cells
     int blockNumGrow = 1;  // ghost layers of user data
     int minAmrLevel = 1;
     int maxAmrLevel = 1;
@@ -90,7 +132,7 @@ struct AmrTree {
     // SayakaRegrid.cpp:530-591, and fills the new blocks in initNewBlockDataAfterRegrid)
     std::vector<int> regridOrigin;
     std::vector<SurroundingBlocks> surrBlocks;
-    std::vector<std::vector<int>> levels;  // levels[l] = ids of level-l blocks, ascending
+    std::vector<AmrTreeLevel> levels;   // levels[l] = the level-l blocks, ascending id
     std::vector<TreeStateData> treeStateData;
 
     int numFace() const { return 2 * ndim; }
@@ -100,6 +142,11 @@ struct AmrTree {
     void init();
     void initUniformRootLevel(const int ndiv[3], const RealBox& probbox, const int bc_type[MAX_FACE],
                               const bool is_periodic[MAX_FACE]);
+    void initUniformRootLevel(const Vector3i& ndiv, const RealBox& probbox, const int bc_type[MAX_FACE],
+                              const bool is_periodic[MAX_FACE]) {
+        const int n[3] = {ndiv.x, ndiv.y, ndiv.z};
+        initUniformRootLevel(n, probbox, bc_type, is_periodic);
+    }
     void initRefineToMinLevel();
 
     // Returns 0 on success, -1 when the node pool overflows (the reference exits).  The
@@ -116,7 +163,11 @@ struct AmrTree {
     double getBlockCellVolume(int b) const;
     Vector3d getBlockCellArea(int b) const;
     int currentFinestLevel() const;
-    const std::vector<int>& getTreeLevel(int l) const { return levels[l]; }
+    int maxRefineLevel() const { return maxAmrLevel; }
+    int minRefineLevel() const { return minAmrLevel; }
+    const IndexBox& validBlockCellBox() const { return blockValidIndex; }
+    const IndexBox& grownBlockCellBox() const { return blockGrownIndex; }
+    const AmrTreeLevel& getTreeLevel(int l) const { return levels[l]; }
 
    private:
     void checkCoarsen();

@@ -8,6 +8,7 @@
 
 #include <map>
 
+#include "index_box.h"
 #include "sayaka_types.h"
 
 namespace sayaka {
@@ -27,6 +28,13 @@ struct BCRegister {
     }
     void setBCVal(int isurr, int jsurr, int ksurr, int phys_bc, double bc_val) {
         bcvals[surr(isurr, jsurr, ksurr)][phys_bc] = bc_val;
+    }
+    // the same by a surrounding position (SayakaBoundaryPatch.h:91-99)
+    void setBCMap(const SurroundingIndex& s, int phys_bc, int math_bc) {
+        setBCMap(s.ipos(), s.jpos(), s.kpos(), phys_bc, math_bc);
+    }
+    void setBCVal(const SurroundingIndex& s, int phys_bc, double bc_val) {
+        setBCVal(s.ipos(), s.jpos(), s.kpos(), phys_bc, bc_val);
     }
     // the same map on every face position (the common case of the reference's drivers)
     void setBCMapAllFaces(int phys_bc, int math_bc) {

@@ -28,10 +28,17 @@ struct InterpPatch {
         PROLONG_CENTERED_LINEAR,
         PROLONG_CENTERED_LINEAR_LIMITED,
     };
+    // restriction methods (SayakaInterpPatch.h:85-92); recorded by FillPatch::setRestriction
+    enum RestrictMethod { RESTRICT_IGNORE = -1, RESTRICT_AVERAGE = 0, RESTRICT_SUM };
 };
 
 struct BCPatch {
     std::vector<BCRegister> regs;
+    BCPatch() = default;
+    // BCPatch(tree, data, location) / define(tree, location, ncomp, ngrow)
+    // (SayakaBoundaryPatch.h:196-215): one BC register per component
+    BCPatch(const AmrTree&, const TreeData& data, VariableLocation) { defineByData(data); }
+    void define(const AmrTree&, VariableLocation, int ncomp, int) { regs.assign(ncomp, BCRegister{}); }
     void defineByData(const TreeData& data) { regs.assign(data.numComp(), BCRegister{}); }
     BCRegister& boundaryRegister(int comp) { return regs.at(comp); }
     const BCRegister& boundaryRegister(int comp) const { return regs.at(comp); }
@@ -49,6 +56,11 @@ class FillPatch {
     void setBC(const BCPatch& bc) { bc_ = &bc; }
     // default: injection for every component (SayakaInterpPatch.h:109)
     void setProlongation(int comp, int prolong);
+    // restriction method per component (InterpPatch::setRestriction, SayakaInterpPatch.h:119-122;
+    // default average); kept with the patch -- no fill here restricts
+    void setRestriction(int comp, int method) { restrict_.at(comp) = method; }
+    const std::vector<int>& prolongFlag() const { return prolong_; }
+    const std::vector<int>& restrictFlag() const { return restrict_; }
     // FillPatch::fillBoundary (SayakaFillPatch.cpp:17-30): levels 1..finest, every block
     void fillBoundary(int scomp, int ncomp, int nlayer);
     // initNewBlockDataAfterRegrid's work on this state (SayakaRegrid.cpp:1013-1103), after the
@@ -64,7 +76,7 @@ class FillPatch {
     const AmrTree* tree_;
     TreeData* data_;
     const BCPatch* bc_ = nullptr;
-    std::vector<int> prolong_;
+    std::vector<int> prolong_, restrict_;
     int device_;
     sayaka_dev_ctx* ctx_ = nullptr;
 };

@@ -42,6 +42,25 @@ struct SolveStats {
     std::vector<double> hist;   // -|s| at half steps, +|r| at full steps
 };
 
+// The blocks of one multigrid level's tower (MGLevel / MGLevelTower, SayakaTower.h:50-224):
+// for sub-levels l = m .. 1, the level-l blocks that are leaves or have l == m, each sub-level in
+// ascending id; [begin(l), end(l)) is sub-level l's range of the list.  Rebuilt by
+// prepareSolver.  The face registers stay on the device (its index tables).
+struct MGLevelTower : AmrTreeLevel {
+    enum TowerFluxSyncMode { FLUX_SYNC_AVG, FLUX_SYNC_SUM };
+    std::vector<int> level_begin, level_end;
+    int getMGLevel() const { return level; }
+    int begin(int l) const { return level_begin[l]; }
+    int end(int l) const { return level_end[l]; }
+    int size(int l) const { return end(l) - begin(l); }
+    int coversTreeLevel(int l) const { return l <= level; }
+    int hasTreeLevel(int l) const { return coversTreeLevel(l) && size(l) > 0; }
+    int isSubLevelEmpty(int l) const { return size(l) <= 0; }
+    using AmrTreeLevel::begin;
+    using AmrTreeLevel::end;
+    using AmrTreeLevel::size;
+};
+
 class MacSolver {
    public:
     enum Precision { FP64 = 0, Mixed = 1, FP64Fast = 2 };
@@ -123,11 +142,18 @@ class MacSolver {
     // MacSolver::mg_flux_level (SayakaMacSolver.h:275, SayakaMacOperator.cpp:161-239): the face
     // flux -d(phi)/dx of direction dir on tower mg_level into flux (face data of dir): in-block
     // and same-level faces over dx, the fine side of coarse/fine faces over 1.5 dx, the coarse
-    // side the average of the fine fluxes, walls 0.  The device context's X field carries phi;
-    // unlike the reference, phi's ghost cells are not filled as a side effect.
+    // side the average of the fine fluxes, walls 0.  Like the reference it also zeroes the
+    // tower blocks' flux ghosts, writes the restricted fine fluxes onto the faces of their
+    // parents, and leaves phi's first ghost layer as its fill sets it (injection, no corners, no
+    // walls); phi needs at least one ghost layer.  The device context's X field carries phi.
     void mg_flux_level(int mg_level, int dir, TreeData& flux, TreeData& phi, int fluxcomp, int phicomp);
+    // the flux of the tower only, packed [blk][face] (0 outside the tower)
     void fluxPacked(int mg_level, int dir, const double* phi, double* flux);
     const SolveStats& lastStats() const { return stats_; }
+    // the tower of multigrid level l (SayakaMacSolver.h:187-188), l = 1 .. maxAmrLevel; valid
+    // from construction on, filled by prepareSolver
+    const MGLevelTower& getTower(int l) const { return towers_.at(l); }
+    MGLevelTower& getTower(int l) { return towers_.at(l); }
 
     const AmrTree& tree() const { return *tree_; }
     sayaka_dev_ctx* deviceContext() const { return ctx_; }
@@ -148,6 +174,7 @@ class MacSolver {
     bool unitIB_ = true, unitB_ = true;
     std::vector<double> vf_, af_[3], b_[3];   // leaf values, packed valid [blk][...]
     SolveStats stats_;
+    std::vector<MGLevelTower> towers_;
 };
 
 }  // namespace amd

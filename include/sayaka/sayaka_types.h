@@ -20,7 +20,11 @@ constexpr int MAX_CHILD = 8;
 constexpr int MAX_LEVEL = 16;
 constexpr int PHYS_BNDRY = -20;  // neighbour id <= this is a physical boundary (SayakaTree.h:38-40)
 
-enum TreeNodeType { NodeInvalid = 0, NodeIsLeaf = 1, NodeHasLeafChild = 2, NodeNoLeafChild = 3 };
+enum TreeNodeType {
+    NodeInvalid = 0, NodeIsLeaf = 1, NodeHasLeafChild = 2, NodeNoLeafChild = 3,
+    // the reference's spellings (SayakaTree.h:27-32)
+    TreeNodeType_Invalid = 0, TreeNodeType_IsLeaf = 1, TreeNodeType_HasLeafChild = 2, TreeNodeType_NoLeafChild = 3
+};
 enum TowerFaceType { FACE_FINE_FINE = 0, FACE_FINE_BC = 1, FACE_FINE_CRSE = 2, FACE_CRSE_FINE = 3 };
 enum BCType { Neumann = 0, Dirichlet = 1, SimpleFill = 2 };
 

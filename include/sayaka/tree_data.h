@@ -16,9 +16,36 @@
 namespace sayaka {
 inline namespace amd {
 
+// The data of one block, over its grown box (GridData<double>, SayakaBlockData.h:12-215):
+// indexed (i, j, k, comp) with valid cells from 0 and ghosts from -ngrow.  TreeData owns the
+// storage; TreeData::operator[] hands out references to these views.
+class DoubleBlockData {
+   public:
+    double& operator()(int i, int j, int k, int comp = 0) { return p_[box_.offset(i, j, k, comp)]; }
+    double operator()(int i, int j, int k, int comp = 0) const { return p_[box_.offset(i, j, k, comp)]; }
+    const IndexBox& box() const { return box_; }
+    int numComp() const { return ncomp_; }
+    // every value of the grown box, every component / components [scomp, scomp + ncomp)
+    void setValue(double v);
+    void setValue(int scomp, int ncomp, double v);
+    double* dataPtr(int comp = 0) { return p_ + (std::size_t)comp * box_.stride(); }
+    const double* dataPtr(int comp = 0) const { return p_ + (std::size_t)comp * box_.stride(); }
+
+   private:
+    friend class TreeData;
+    double* p_ = nullptr;
+    IndexBox box_;
+    int ncomp_ = 0;
+};
+using GridDataD = DoubleBlockData;
+
 class TreeData {
    public:
     TreeData() = default;
+    TreeData(const TreeData& o);
+    TreeData(TreeData&& o) noexcept;
+    TreeData& operator=(const TreeData& o);
+    TreeData& operator=(TreeData&& o) noexcept;
 
     // TreeData::MakeCellData / MakeFaceData (SayakaTreeData.h:245-306)
     static TreeData MakeCellData(const AmrTree& tree, int ncomp, int ngrow);
@@ -34,6 +61,43 @@ class TreeData {
     static void Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow);
     static void AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp,
                          int ngrow);
+    static void SetValue(TreeData& dst, double val, int dcomp, int ncomp, int ngrow);
+    // the same over a list of blocks, iblocks[ibegin .. iend) (SayakaTreeData.h:256-294; e.g.
+    // a tower's blockIds())
+    static void Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow,
+                     const int iblocks[], int ibegin, int iend);
+    static void Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow,
+                     const std::vector<int>& iblocks) {
+        Copy(dst, src, dcomp, scomp, ncomp, ngrow, iblocks.data(), 0, (int)iblocks.size());
+    }
+    static void AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp,
+                         int ngrow, const int iblocks[], int ibegin, int iend);
+    static void AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp,
+                         int ngrow, const std::vector<int>& iblocks) {
+        AddEqual(dst, src, coef, dcomp, scomp, ncomp, ngrow, iblocks.data(), 0, (int)iblocks.size());
+    }
+    static void SetValue(TreeData& dst, double val, int dcomp, int ncomp, int ngrow, const int iblocks[],
+                         int ibegin, int iend);
+    static void SetValue(TreeData& dst, double val, int dcomp, int ncomp, int ngrow,
+                         const std::vector<int>& iblocks) {
+        SetValue(dst, val, dcomp, ncomp, ngrow, iblocks.data(), 0, (int)iblocks.size());
+    }
+
+    // block views (SayakaTreeData.h:176-177)
+    DoubleBlockData& operator[](int iblock) { return views_[iblock]; }
+    const DoubleBlockData& operator[](int iblock) const { return views_[iblock]; }
+    // valid box (cells, or faces staggered in the face direction) and grown box of a block
+    const IndexBox& validBox() const { return valid_; }
+    const IndexBox& indexBox() const { return grown_; }
+    VariableLocation indexType() const { return valid_.type(); }
+    int isCellData() const { return dir_ < 0; }
+    int isFaceData() const { return dir_ >= 0; }
+    int isFaceData(int d) const { return dir_ == d; }
+    int getFaceDataDir() const { return dir_; }
+    int blockNum() const { return tree_->numBlocks; }
+    int maxBlockNum() const { return nslot_; }
+    // every value of the tree's current blocks (TreeData::setValue, SayakaTreeData.h:187-192)
+    void setValue(double v);
 
     // the tree the data was made for (TreeData::tree, SayakaTreeData.h:149); it must outlive the data
     const AmrTree& tree() const { return *tree_; }
@@ -73,11 +137,14 @@ class TreeData {
         return (std::size_t)blk * blocksize_ + (std::size_t)comp * compsize_ +
                (std::size_t)((i + ngrow_) + lx_ * ((j + ngrow_) + ly_ * (k + kg_)));
     }
+    void rebind();   // point the block views at data_
     const AmrTree* tree_ = nullptr;
     int ndim_ = 3, nb_ = 8, ngrow_ = 0, ncomp_ = 0, dir_ = -1, nslot_ = 0;
     int lx_ = 0, ly_ = 0, lz_ = 0, kg_ = 0;
     std::size_t compsize_ = 0, blocksize_ = 0;
+    IndexBox valid_, grown_;
     std::vector<double> data_;
+    std::vector<DoubleBlockData> views_;
 };
 
 }  // namespace amd

@@ -130,6 +130,13 @@ int sayaka_dev_bottom(sayaka_dev_ctx* ctx, int sol, int rhs, double tol_rel, dou
  * nblk * (nb+1)*nb^(ndim-1) doubles; blocks outside tower m are 0). */
 int sayaka_dev_flux(sayaka_dev_ctx* ctx, int m, int dir, int phi, double* flux_out);
 
+/* The one-layer face ghosts that MacSolver::mg_flux_level leaves in its phi argument on tower m
+ * (its mg_fillbndry_level, SayakaMacOperator.cpp:181-182, SayakaMacUtil.cpp:210-251: injection, no
+ * corners, no physical boundary) for the field: ghost_out (host) [block][face][slab], slab = the
+ * face's nb^(ndim-1) cells with the lower tangential axis fastest; every block of tree levels
+ * 1..m except a level whose next tower sub-level is empty; wall faces and unvisited blocks 0. */
+int sayaka_dev_solver_ghosts(sayaka_dev_ctx* ctx, int m, int field, double* ghost_out);
+
 /* Regrid data movement of the device fields, after sayaka_dev_set_topology with the regridded
  * tree: block b takes the values block origin[b] had before (origin from sy_tree_regrid_origin;
  * nold = block count before the regrid), and blocks with origin -1 are filled from their

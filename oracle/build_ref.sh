@@ -52,4 +52,11 @@ for D in 2 3; do
     g++ -shared -Wl,-Bsymbolic -o "$OUT/libsayaka_ref$D.so.tmp" $OBJS "$OUT/obj/ref_harness_$D.o"
     mv "$OUT/libsayaka_ref$D.so.tmp" "$OUT/libsayaka_ref$D.so"
 done
-echo "build_ref.sh: built $OUT/libsayaka_ref2.so $OUT/libsayaka_ref3.so"
+# The projection driver (written against the reference's call surface; the product build
+# compiles the same file against include/sayaka) linked with the reference's 3D objects.
+DRV="$HERE/../sayakaamr_amd/csrc/drivers/projection_step.cpp"
+OBJS3=""
+for f in $SRCS; do OBJS3="$OBJS3 $OUT/obj/${f}_3.o"; done
+g++ $CXXFLAGS -DSAYAKA_SPACEDIM=3 "$DRV" $OBJS3 -o "$OUT/ref_projection_step.tmp"
+mv "$OUT/ref_projection_step.tmp" "$OUT/ref_projection_step"
+echo "build_ref.sh: built $OUT/libsayaka_ref2.so $OUT/libsayaka_ref3.so $OUT/ref_projection_step"

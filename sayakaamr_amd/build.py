@@ -25,7 +25,7 @@ DEV_SOURCES = ["dev/sayaka_dev.hip"]
 DEV_HEADERS = ["dev/dev_common.h", "dev/kernels.h", "dev/bottom.h", "dev/coef.h", "dev/mixed.h"]
 HOST_SOURCES = ["host/amr_tree.cpp", "host/capi_tree.cpp", "host/tree_data.cpp", "host/mac_solver.cpp",
                 "host/capi_mac.cpp", "host/h5lite.cpp", "host/writer_hdf5.cpp", "host/fill_patch.cpp"]
-DRIVERS = {"sayaka_c1_solve": "drivers/c1_solve.cpp"}
+DRIVERS = {"sayaka_c1_solve": "drivers/c1_solve.cpp", "sayaka_projection_step": "drivers/projection_step.cpp"}
 
 
 # Small per-thread index arrays that are indexed by a run-time axis (the halo gathers' i/j/k

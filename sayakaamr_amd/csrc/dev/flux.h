@@ -107,4 +107,42 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_flux(FluxArgs a) {
     }
 }
 
+// The one-layer face ghosts that mg_flux_level leaves in its phi argument: mg_fillbndry_level
+// (SayakaMacUtil.cpp:210-251) fills every block of the visited levels with injection and
+// neither corners nor physical boundaries (FillPatch::fillLevelBoundary, SayakaFillPatch.cpp:
+// 34-299): a same-level neighbour gives its boundary cell (leaf or not), a coarse neighbour the
+// cell of the parent's neighbour that the injection reads through the parent's ghost.  One
+// workgroup per block of the list, out[block][face][slab] with the lower tangential axis
+// fastest; wall faces are written as 0 (the fill leaves those ghosts alone).
+template <int D, int NB>
+__global__ void __launch_bounds__(256) k_solver_ghosts(const int* list, const BlockInfo* binfo, const double* x,
+                                                     double* out) {
+    using G = Geo<D, NB>;
+    const int b = list[blockIdx.x];
+    const BlockInfo& bi = binfo[b];
+    for (int g = threadIdx.x; g < G::NGHOST; g += 256) {
+        const int f = g / G::NSLAB, p = g % G::NSLAB;
+        const int d = f >> 1, s = f & 1;
+        int a1, a2;
+        tang_axes<D>(d, a1, a2);
+        const int ext = d == 2 ? G::NBZ : NB;
+        int q[3] = {0, 0, 0};
+        q[a1] = p % NB;
+        if (D == 3) q[a2] = p / NB;
+        const int n = bi.nbr[f];
+        double v = 0.0;
+        if (n >= 0) {
+            q[d] = s ? 0 : ext - 1;
+            v = x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
+        } else if (n > PHYS_BNDRY) {
+            int r[3] = {0, 0, 0};
+            r[d] = s ? 0 : ext - 1;
+            r[a1] = (q[a1] >> 1) + side_of(bi.which_child, a1) * (NB / 2);
+            if (D == 3) r[a2] = (q[a2] >> 1) + side_of(bi.which_child, a2) * (NB / 2);
+            v = x[bi.cnbr[f] * G::NCELL + G::cidx(r[0], r[1], r[2])];
+        }
+        out[(size_t)b * G::NGHOST + g] = v;
+    }
+}
+
 }  // namespace sydev

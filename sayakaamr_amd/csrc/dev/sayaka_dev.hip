@@ -154,6 +154,7 @@ struct sayaka_dev_ctx {
     float* d_pc_s0inv = nullptr;   // [PC_MAXBLK][PC_MAXBLK] (row-major nl x nl in use)
     int pc_enable = 1, pc_version = -1, pc_ok = 0;
     double* d_flux = nullptr;      // face flux of one direction, [cap][nface] (allocated on first use)
+    double* d_ghost = nullptr;     // one-layer face ghosts, [cap][2 ndim][nslab] (allocated on first use)
     // regrid data movement (allocated on first use): old field copy, origin map, new-block list
     double* d_remap_tmp = nullptr;
     int* d_remap_idx = nullptr;
@@ -786,6 +787,20 @@ struct Impl {
         return 0;
     }
 
+    // phi's one-layer face ghosts as mg_flux_level's fill leaves them (flux.h, k_solver_ghosts):
+    // tree levels 1..m, a level skipped when the tower's next sub-level is empty
+    static int solver_ghosts(sayaka_dev_ctx* c, int m, const double* x, double* out) {
+        for (int l = 1; l <= m; l++) {
+            if (l < m && c->tw_sub[m][l + 1].n <= 0) continue;
+            ListRef lst = c->lv_all[l];
+            if (lst.n <= 0) continue;
+            hipLaunchKernelGGL((k_solver_ghosts<D, NB>), dim3(lst.n), dim3(256), 0, c->stream, dlist(c, lst),
+                               c->d_binfo, x, out);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
     // regrid data movement of one field (flux.h): gather by origin, then inject the new blocks
     // level by level; idx = [origin (nblk)][new blocks sorted by level]; lv[l] their ranges
     static int remap(sayaka_dev_ctx* c, double* x, int nold, const int* d_origin, const int* d_new,
@@ -1174,6 +1189,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& p : c->d_field) if (p) hipFree(p);
     if (c->d_flux) hipFree(c->d_flux);
+    if (c->d_ghost) hipFree(c->d_ghost);
     if (c->d_remap_tmp) hipFree(c->d_remap_tmp);
     if (c->d_remap_idx) hipFree(c->d_remap_idx);
     if (c->d_vel) hipFree(c->d_vel);
@@ -1818,6 +1834,19 @@ int sayaka_dev_flux(sayaka_dev_ctx* c, int m, int dir, int phi, double* flux_out
     if (e) return e;
     HIPCHK(hipMemcpyAsync(flux_out, c->d_flux, (size_t)c->nblk * c->nface * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sayaka_dev_solver_ghosts(sayaka_dev_ctx* c, int m, int field, double* ghost_out) {
+    if (!c || !field_ok(field) || m < 1 || m > c->maxlevel || !ghost_out) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)c->nblk * 2 * c->ndim * c->nslab;
+    if (!c->d_ghost) HIPCHK(hipMalloc(&c->d_ghost, (size_t)c->cap * 2 * c->ndim * c->nslab * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->d_ghost, 0, n * sizeof(double), c->stream));
+    int e = dispatch(c, [&](auto I) { return decltype(I)::solver_ghosts(c, m, c->d_field[field], c->d_ghost); });
+    if (e) return e;
+    HIPCHK(hipMemcpyAsync(ghost_out, c->d_ghost, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -47,8 +47,19 @@ int AmrTree::surrIndex(int i, int j, int k) const {
     return m;
 }
 
-// Pool allocation (SayakaTree.cpp:34-91).
+// Pool allocation (SayakaTree.cpp:34-91).  A caller that set the reference's index boxes
+// gets ndim and nb from them; otherwise the boxes are made from ndim, nb and blockNumGrow.
 void AmrTree::init() {
+    if (blockValidIndex.size(0) > 1) {
+        nb = blockValidIndex.size(0);
+        ndim = blockValidIndex.size(2) > 1 ? 3 : 2;
+    } else {
+        blockValidIndex = IndexBox(Vector3i{0, 0, 0}, Vector3i{nb - 1, nb - 1, ndim == 3 ? nb - 1 : 0});
+    }
+    if (blockGrownIndex.size(0) <= blockValidIndex.size(0)) {
+        blockGrownIndex = blockValidIndex;
+        blockGrownIndex.extend(blockNumGrow, blockNumGrow, ndim == 3 ? blockNumGrow : 0);
+    }
     maxTreeBlockNum = maxBlockNum;
     numBlocks = 0;
     gridChanged = 0;
@@ -60,7 +71,8 @@ void AmrTree::init() {
     stayFlag.assign(maxTreeBlockNum, 1);
     surrBlocks.assign(maxTreeBlockNum, SurroundingBlocks{});
     for (auto& s : surrBlocks) s.setUnused();
-    levels.assign(maxAmrLevel + 1, {});
+    levels.assign(maxAmrLevel + 1, AmrTreeLevel{});
+    for (int l = 0; l <= maxAmrLevel; l++) levels[l].level = l;
 }
 
 // Root level: n0x*n0y*n0z leaves at level 1, i fastest; walls get the caller's boundary
@@ -520,7 +532,7 @@ void AmrTree::buildSurroundingBlocks() {
 // Block ids per level in ascending order (SayakaRegrid.cpp:962-976).
 void AmrTree::cacheTreeLevels() {
     for (int l = 1; l <= maxAmrLevel; l++) levels[l].clear();
-    for (int i = 0; i < numBlocks; i++) levels[blocks[i].refineLevel].push_back(i);
+    for (int i = 0; i < numBlocks; i++) levels[blocks[i].refineLevel].addBlock(i);
 }
 
 }  // namespace amd

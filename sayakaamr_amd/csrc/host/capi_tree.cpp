@@ -280,12 +280,12 @@ int sy_tree_flag_shell(sy_tree* h, const double* centre, double r0, double facto
 
 int sy_tree_level_count(const sy_tree* h, int l) {
     const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
-    return (l >= 1 && l <= t->maxAmrLevel) ? (int)t->levels[l].size() : 0;
+    return (l >= 1 && l <= t->maxAmrLevel) ? t->levels[l].numLevelBlock : 0;
 }
 void sy_tree_level_blocks(const sy_tree* h, int l, int* out) {
     const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
     if (l < 1 || l > t->maxAmrLevel) return;
-    std::memcpy(out, t->levels[l].data(), t->levels[l].size() * sizeof(int));
+    std::memcpy(out, t->levels[l].levelBlock.data(), t->levels[l].levelBlock.size() * sizeof(int));
 }
 
 }  // extern "C"

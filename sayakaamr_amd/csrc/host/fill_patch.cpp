@@ -18,7 +18,8 @@ void check(int rc, const char* what) {
 }  // namespace
 
 FillPatch::FillPatch(const AmrTree& tree, TreeData& data, int device)
-    : tree_(&tree), data_(&data), prolong_(data.numComp(), InterpPatch::PROLONG_INJECTION), device_(device) {}
+    : tree_(&tree), data_(&data), prolong_(data.numComp(), InterpPatch::PROLONG_INJECTION),
+      restrict_(data.numComp(), InterpPatch::RESTRICT_AVERAGE), device_(device) {}
 
 FillPatch::~FillPatch() {
     if (ctx_) sayaka_dev_destroy(ctx_);

@@ -1,6 +1,7 @@
 // mac_solver.cpp -- host MacSolver: the reference's call surface over the device C ABI.
 #include "mac_solver.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -15,6 +16,28 @@ MacSolver::MacSolver(const AmrTree& tree, int device) : tree_(&tree) {
     ncell_ = tree.cellsPerBlock();
     nface_ = (tree.nb + 1) * tree.nb * (tree.ndim == 3 ? tree.nb : 1);
     check(sayaka_dev_create(device, tree.ndim, tree.nb, tree.maxBlockNum, &ctx_), "MacSolver: device context");
+    towers_.assign(tree.maxAmrLevel + 1, MGLevelTower{});
+    for (int m = 0; m <= tree.maxAmrLevel; m++) {
+        towers_[m].level = m;
+        towers_[m].level_begin.assign(tree.maxAmrLevel + 1, 0);
+        towers_[m].level_end.assign(tree.maxAmrLevel + 1, 0);
+    }
+}
+
+// MGLevelTower::update (SayakaTower.cpp:28-68) for every multigrid level
+static void build_towers(const AmrTree& t, std::vector<MGLevelTower>& towers) {
+    for (int m = 1; m <= t.maxAmrLevel; m++) {
+        MGLevelTower& tw = towers[m];
+        tw.clear();
+        std::fill(tw.level_begin.begin(), tw.level_begin.end(), 0);
+        std::fill(tw.level_end.begin(), tw.level_end.end(), 0);
+        for (int l = m; l >= 1; l--) {
+            tw.level_begin[l] = tw.numLevelBlock;
+            for (int b : t.getTreeLevel(l).blockIds())
+                if (l == m || t[b].isLeaf()) tw.addBlock(b);
+            tw.level_end[l] = tw.numLevelBlock;
+        }
+    }
 }
 
 MacSolver::~MacSolver() {
@@ -64,6 +87,7 @@ void MacSolver::prepareSolver() {
     const AmrTree& t = *tree_;
     nblk_ = t.numBlocks;
     upload_tree_topology(ctx_, t);
+    build_towers(t, towers_);
     // prepareSolver resets the coefficients to unit values (SayakaMacSolver.cpp:12-26)
     unitIB_ = unitB_ = true;
     alpha_ = 0.0;
@@ -258,12 +282,79 @@ void MacSolver::fluxPacked(int mg_level, int dir, const double* phi, double* flu
     check(sayaka_dev_flux(ctx_, mg_level, dir, SAYAKA_X, flux), "MacSolver::mg_flux_level");
 }
 
+// mg_flux_level (SayakaMacOperator.cpp:161-239) with everything it writes into the caller's data:
+// the flux of the tower's blocks (their grown boxes zeroed first, then the valid faces), the
+// parents' faces the coarse/fine synchronisation restricts the fine fluxes onto, and phi's first
+// ghost layer as the solver's fill leaves it.  The arithmetic runs on the device (flux, ghost
+// gather); the host places the values.
 void MacSolver::mg_flux_level(int mg_level, int dir, TreeData& flux, TreeData& phi, int fluxcomp, int phicomp) {
     if (flux.faceDir() != dir) throw std::invalid_argument("mg_flux_level: flux must be face data of direction dir");
+    if (phi.numGrow() < 1) throw std::invalid_argument("mg_flux_level: phi needs a ghost layer (its fill writes one)");
+    const AmrTree& t = *tree_;
+    if (mg_level < 1 || mg_level > t.maxAmrLevel || t.getTreeLevel(mg_level).isEmptyLevel()) return;
     std::vector<double> p((std::size_t)nblk_ * ncell_), f((std::size_t)nblk_ * nface_);
     phi.packValid(phicomp, nblk_, p.data());
     fluxPacked(mg_level, dir, p.data(), f.data());
-    flux.unpackValid(fluxcomp, nblk_, f.data());
+
+    const int nb = t.nb, nz = t.ndim == 3 ? nb : 1;
+    const int fx = nb + (dir == 0), fy = nb + (dir == 1);
+    auto face_at = [&](int b, int i, int j, int k) { return f[(std::size_t)b * nface_ + i + fx * (j + fy * k)]; };
+    const MGLevelTower& tw = towers_[mg_level];
+    // mg_zero_level on the tower, then the tower's faces (SayakaMacOperator.cpp:175-204)
+    for (int b : tw.blockIds()) {
+        auto& d = flux[b];
+        d.setValue(fluxcomp, 1, 0.0);
+        const IndexBox& vb = flux.validBox();
+        BEGIN_FOR_BOX_RANGE(vb, i, j, k);
+        { d(i, j, k, fluxcomp) = face_at(b, i, j, k); }
+        END_FOR_BOX_RANGE;
+    }
+    // syncSubLevelCrseFineFlux (SayakaTower.cpp:360-413): the parent of a fine block with a
+    // coarse neighbour across a face of direction dir receives, on that face, the average of the
+    // fine fluxes -- the value its coarse neighbour's facing face holds
+    for (int l = mg_level; l >= 2; l--)
+        for (int q = tw.begin(l); q < tw.end(l); q++) {
+            const int b = tw[q];
+            for (int s = 0; s <= 1; s++) {
+                if (t[b].neighbor[2 * dir + s] != -1) continue;
+                const int par = t[b].parent, cn = t[par].neighbor[2 * dir + s];
+                if (par < 0 || cn < 0) continue;
+                // the parent's face plane on side s, the neighbour's on the other side
+                const int a1 = dir == 0 ? 1 : 0, a2 = dir == 2 ? 1 : 2;
+                auto& dp = flux[par];
+                for (int v = 0; v < (a2 == 2 ? nz : nb); v++)
+                    for (int u = 0; u < nb; u++) {
+                        int cp[3] = {0, 0, 0}, cc[3] = {0, 0, 0};
+                        cp[a1] = cc[a1] = u;
+                        cp[a2] = cc[a2] = v;
+                        cp[dir] = s ? nb : 0;
+                        cc[dir] = s ? 0 : nb;
+                        dp(cp[0], cp[1], cp[2], fluxcomp) = face_at(cn, cc[0], cc[1], cc[2]);
+                    }
+            }
+        }
+    // mg_fillbndry_level(mg_level, phi, phicomp, 1, 1) (SayakaMacOperator.cpp:181-182)
+    const int nf = 2 * t.ndim, nslab = nb * nz;
+    std::vector<double> g((std::size_t)nblk_ * nf * nslab);
+    check(sayaka_dev_solver_ghosts(ctx_, mg_level, SAYAKA_X, g.data()), "MacSolver::mg_flux_level");
+    for (int l = 1; l <= mg_level; l++) {
+        if (l < mg_level && tw.isSubLevelEmpty(l + 1)) continue;
+        for (int b : t.getTreeLevel(l).blockIds()) {
+            auto& d = phi[b];
+            for (int fc = 0; fc < nf; fc++) {
+                if (t[b].neighbor[fc] <= PHYS_BNDRY) continue;
+                const int dd = fc / 2, s = fc % 2;
+                const int a1 = dd == 0 ? 1 : 0, a2 = dd == 2 ? 1 : 2;
+                for (int pq = 0; pq < nslab; pq++) {
+                    int c[3] = {0, 0, 0};
+                    c[a1] = pq % nb;
+                    c[a2] = pq / nb;
+                    c[dd] = s ? (dd == 2 ? nz : nb) : -1;
+                    d(c[0], c[1], c[2], phicomp) = g[((std::size_t)b * nf + fc) * nslab + pq];
+                }
+            }
+        }
+    }
 }
 
 void MacSolver::getSol(TreeData& sol, int comp) const {

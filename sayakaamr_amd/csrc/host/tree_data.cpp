@@ -29,8 +29,67 @@ TreeData TreeData::MakeFaceData(const AmrTree& tree, int dir, int ncomp, int ngr
     t.lz_ = tree.ndim == 3 ? tree.nb + 2 * ngrow + (dir == 2) : 1;
     t.compsize_ = (std::size_t)t.lx_ * t.ly_ * t.lz_;
     t.blocksize_ = t.compsize_ * ncomp;
+    const int nz = tree.ndim == 3 ? tree.nb - 1 : 0;
+    t.valid_ = IndexBox(Vector3i{0, 0, 0}, Vector3i{tree.nb - 1, tree.nb - 1, nz});
+    if (dir >= 0) t.valid_.staggerInDir(dir);
+    t.grown_ = t.valid_;
+    t.grown_.extend(ngrow, ngrow, t.kg_);
     t.data_.assign(t.blocksize_ * t.nslot_, 0.0);
+    t.rebind();
     return t;
+}
+
+void DoubleBlockData::setValue(double v) { std::fill(p_, p_ + (std::size_t)ncomp_ * box_.stride(), v); }
+void DoubleBlockData::setValue(int scomp, int ncomp, double v) {
+    std::fill(p_ + (std::size_t)scomp * box_.stride(), p_ + (std::size_t)(scomp + ncomp) * box_.stride(), v);
+}
+
+void TreeData::rebind() {
+    views_.assign(nslot_, DoubleBlockData{});
+    for (int b = 0; b < nslot_; b++) {
+        views_[b].p_ = data_.data() + (std::size_t)b * blocksize_;
+        views_[b].box_ = grown_;
+        views_[b].ncomp_ = ncomp_;
+    }
+}
+
+TreeData::TreeData(const TreeData& o)
+    : tree_(o.tree_), ndim_(o.ndim_), nb_(o.nb_), ngrow_(o.ngrow_), ncomp_(o.ncomp_), dir_(o.dir_),
+      nslot_(o.nslot_), lx_(o.lx_), ly_(o.ly_), lz_(o.lz_), kg_(o.kg_), compsize_(o.compsize_),
+      blocksize_(o.blocksize_), valid_(o.valid_), grown_(o.grown_), data_(o.data_) {
+    rebind();
+}
+
+TreeData::TreeData(TreeData&& o) noexcept { *this = std::move(o); }
+
+TreeData& TreeData::operator=(const TreeData& o) {
+    if (this != &o) {
+        TreeData t(o);
+        *this = std::move(t);
+    }
+    return *this;
+}
+
+TreeData& TreeData::operator=(TreeData&& o) noexcept {
+    tree_ = o.tree_;
+    ndim_ = o.ndim_;
+    nb_ = o.nb_;
+    ngrow_ = o.ngrow_;
+    ncomp_ = o.ncomp_;
+    dir_ = o.dir_;
+    nslot_ = o.nslot_;
+    lx_ = o.lx_;
+    ly_ = o.ly_;
+    lz_ = o.lz_;
+    kg_ = o.kg_;
+    compsize_ = o.compsize_;
+    blocksize_ = o.blocksize_;
+    valid_ = o.valid_;
+    grown_ = o.grown_;
+    data_ = std::move(o.data_);   // the buffer moves: the views keep pointing into it
+    views_ = std::move(o.views_);
+    o.nslot_ = 0;
+    return *this;
 }
 
 TreeData* TreeData::CreateCellData(const AmrTree& tree, int ncomp, int ngrow) {
@@ -87,6 +146,53 @@ void TreeData::AddEqual(TreeData& dst, const TreeData& src, double coef, int dco
             });
 }
 
+template <class F>
+static void for_blocks(const int iblocks[], int ibegin, int iend, F&& f) {
+    for (int q = ibegin; q < iend; q++) f(iblocks[q]);
+}
+
+void TreeData::SetValue(TreeData& dst, double val, int dcomp, int ncomp, int ngrow) {
+    check_pair(dst, dst, dcomp, dcomp, ncomp, ngrow);
+    const int n = dst.tree().numBlocks;
+    for (int b = 0; b < n; b++)
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) { dst(b, i, j, k, dcomp + c) = val; });
+}
+
+void TreeData::Copy(TreeData& dst, const TreeData& src, int dcomp, int scomp, int ncomp, int ngrow,
+                    const int iblocks[], int ibegin, int iend) {
+    check_pair(dst, src, dcomp, scomp, ncomp, ngrow);
+    for_blocks(iblocks, ibegin, iend, [&](int b) {
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) { dst(b, i, j, k, dcomp + c) = src(b, i, j, k, scomp + c); });
+    });
+}
+
+void TreeData::AddEqual(TreeData& dst, const TreeData& src, double coef, int dcomp, int scomp, int ncomp,
+                        int ngrow, const int iblocks[], int ibegin, int iend) {
+    check_pair(dst, src, dcomp, scomp, ncomp, ngrow);
+    for_blocks(iblocks, ibegin, iend, [&](int b) {
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) {
+                dst(b, i, j, k, dcomp + c) += coef * src(b, i, j, k, scomp + c);
+            });
+    });
+}
+
+void TreeData::SetValue(TreeData& dst, double val, int dcomp, int ncomp, int ngrow, const int iblocks[], int ibegin,
+                        int iend) {
+    check_pair(dst, dst, dcomp, dcomp, ncomp, ngrow);
+    for_blocks(iblocks, ibegin, iend, [&](int b) {
+        for (int c = 0; c < ncomp; c++)
+            for_grown_box(dst, ngrow, [&](int i, int j, int k) { dst(b, i, j, k, dcomp + c) = val; });
+    });
+}
+
+void TreeData::setValue(double v) {
+    const int n = tree_->numBlocks;
+    std::fill(data_.begin(), data_.begin() + (std::size_t)n * blocksize_, v);
+}
+
 int TreeData::validExtent(int d) const {
     if (d == 2 && ndim_ == 2) return 1;
     return nb_ + (d == dir_ ? 1 : 0);
@@ -138,6 +244,7 @@ void TreeData::remapBlocks(const std::vector<int>& origin, int nblk) {
             std::memcpy(out.data() + (std::size_t)b * blocksize_, data_.data() + (std::size_t)origin[b] * blocksize_,
                         blocksize_ * sizeof(double));
     data_.swap(out);
+    rebind();
 }
 
 }  // namespace amd

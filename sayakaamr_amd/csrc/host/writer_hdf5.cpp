@@ -112,7 +112,7 @@ int WriteChomboPacked(const AmrTree& tree, int ncomp, const double* packed, cons
     std::vector<std::vector<int32_t>> boxes(num_levels);
     size_t off = 0;
     for (int il = 0; il < num_levels; il++) {
-        const std::vector<int>& lv = tree.getTreeLevel(il + 1);
+        const std::vector<int>& lv = tree.getTreeLevel(il + 1).blockIds();
         const Vector3d dh = dx_level[il];
         h5lite::Group g;
         g.name = "level_" + std::to_string(il);

@@ -10,6 +10,8 @@ against on machines where the reference is absent.
     python tests/golden/make_golden.py remap   (regrid data movement: regrid_remap{2,3}.npz)
     python tests/golden/make_golden.py fill    (general state fill: fill{2,3}.npz)
     python tests/golden/make_golden.py fixbc   (fixBC, valued walls, warm start: {c0,s3}_fixbc.npz)
+    python tests/golden/make_golden.py projstep (the projection driver built against the reference:
+                                                proj_step.npz)
 """
 from __future__ import annotations
 
@@ -369,6 +371,29 @@ def case_fixbc(name):
           f"{out['solve_iters'][0]} (zero guess) / {out['warm_iters'][0]} (warm)")
 
 
+def projection_step():
+    """The projection driver (sayakaamr_amd/csrc/drivers/projection_step.cpp) built against the
+    reference's sources (oracle/build_ref.sh): its tree, tower and a hash of every block of its
+    pressure, velocity and flux arrays."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.dirname(HERE))
+    import golden_cases as GC
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_projection_step")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "proj.bin")
+        r = subprocess.run([exe, path, str(GC.PROJ_MAXLEVEL)], capture_output=True, text=True, cwd=d)
+        assert r.returncode == 0, r.stdout + r.stderr
+        info, arrs = GC.read_projection_dump(path)
+    out = {k: np.array(v) for k, v in info.items()}
+    for name, a in arrs.items():
+        out[f"h_{name}"] = GC.block_hashes(a)
+        out[f"l2_{name}"] = np.array([np.linalg.norm(a)])
+    out["log_iters"] = np.array([r.stdout.count("iter=")])
+    np.savez_compressed(os.path.join(HERE, "proj_step.npz"), **out)
+    print(f"proj_step: {info['nblk']} blocks, tower {info['tower'].size}, {out['log_iters'][0]} iterations")
+
+
 def mg_relax(name, spec, crit, inp_fn, bc):
     """MacSolver::mg_solve (3 fixed V-cycles, then to 1e-8 from zero) and relax_solve (6
     passes) of the reference from a zero solution: the solutions of every block."""
@@ -459,6 +484,9 @@ def state_remap(name):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "projstep":
+        projection_step()
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "fixbc":
         case_fixbc("c0_fixbc")
         case_fixbc("s3_fixbc")

@@ -361,3 +361,39 @@ def fixbc_guess(name, tree):
     h = W._splitmix64(idx + np.uint64(0x6E550000))
     u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
     return (5.0 * u).reshape(n, -1)
+
+
+# ----------------------------------------------------- the projection driver, two builds
+# sayakaamr_amd/csrc/drivers/projection_step.cpp, compiled against the reference's sources
+# (oracle/_ref/ref_projection_step) and against include/sayaka (sayakaamr_amd/lib/
+# sayaka_projection_step), writes the grown boxes of its pressure, velocity and flux.  The golden
+# keeps a 64-bit hash of every block of every array of the reference build's file.
+PROJ_MAXLEVEL = 4
+
+
+def read_projection_dump(path):
+    """Header, tower blocks and the arrays [block][grown box] of a projection_step file."""
+    import numpy as _np
+    raw = open(path, "rb").read()
+    hdr = _np.frombuffer(raw[:24], dtype=_np.int32)
+    nblk, nb, nd, finest, ng, ntw = (int(v) for v in hdr)
+    tower = _np.frombuffer(raw[24:24 + 4 * ntw], dtype=_np.int32).copy()
+    vals = _np.frombuffer(raw[24 + 4 * ntw:], dtype=_np.float64)
+    g = nb + 2 * ng
+    cell = g ** nd
+    face = (g + 1) * g ** (nd - 1)
+    out, off = {}, 0
+    for name, size in [("pres", cell)] + [(f"vel{d}", face) for d in range(nd)] + [(f"gradp{d}", face) for d in range(nd)]:
+        out[name] = vals[off:off + nblk * size].reshape(nblk, size)
+        off += nblk * size
+    assert off == vals.size, "projection_step file size"
+    return dict(nblk=nblk, nb=nb, ndim=nd, finest=finest, ngrow=ng, tower=tower), out
+
+
+def block_hashes(a):
+    """64-bit BLAKE2b of every block's values (row a[b])."""
+    import hashlib
+
+    import numpy as _np
+    return _np.array([int.from_bytes(hashlib.blake2b(_np.ascontiguousarray(r).tobytes(), digest_size=8).digest(),
+                                     "little") for r in a], dtype=_np.uint64)
