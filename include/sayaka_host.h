@@ -32,6 +32,8 @@ int sy_tree_numblocks(const sy_tree* t);
 int sy_tree_finest(const sy_tree* t);
 /* AmrTree::refineFlag / coarsenFlag [0, n) (SayakaTree.h:221-222); NULL leaves one unchanged */
 int sy_tree_set_flags(sy_tree* t, const int* refine, const int* coarsen, int n);
+/* the current refineFlag / coarsenFlag of blocks [0, n) (either pointer may be NULL) */
+int sy_tree_get_flags(const sy_tree* t, int* refine, int* coarsen, int n);
 /* AmrTree::regrid (SayakaRegrid.cpp:26-125).  Returns numBlocks, -1 on pool overflow. */
 int sy_tree_regrid(sy_tree* t);
 /* after sy_tree_regrid: per block its id before the regrid, -1 for a block the regrid created

@@ -37,6 +37,7 @@ _HOST_SIG = {
     "sy_tree_numblocks": ([vp], i32),
     "sy_tree_finest": ([vp], i32),
     "sy_tree_set_flags": ([vp, vp, vp, i32], i32),
+    "sy_tree_get_flags": ([vp, vp, vp, i32], i32),
     "sy_tree_regrid": ([vp], i32),
     "sy_tree_regrid_origin": ([vp, vp], None),
     "sy_tree_flag_shell": ([vp, vp, f64, f64], i32),

@@ -61,7 +61,7 @@ def build_host(verbose=False) -> str:
     """Host library; links libsayaka_dev.so (the MacSolver drives the device through its C ABI)."""
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsayaka_host.so")
-    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}",
+    cmd = ["g++", "-std=c++17", "-O2", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}",
            f"-I{os.path.join(INC, 'sayaka')}", *[os.path.join(CSRC, s) for s in HOST_SOURCES],
            f"-L{LIB}", "-lsayaka_dev", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     _run(cmd, verbose)

@@ -29,9 +29,35 @@ struct SphereArgs {
     int nsub;
 };
 
+// Where the box [lo, lo + ext] (ext[e] = 0 on a flat axis) lies against the sphere: +1 every
+// point farther from the centre than r, -1 every point nearer, 0 the surface may cross it.  The
+// 1e-10 relative margin covers the rounding of the sample coordinates, so a box classified +1
+// or -1 has every sample on that side: the sampled fractions come out exactly 1 or 0 and need
+// not be computed point by point.
+__device__ inline int sphere_box_side(const SphereArgs& a, const double* lo, const double* ext) {
+    double dmin = 0.0, dmax = 0.0;
+    for (int e = 0; e < 3; e++) {
+        const double l = lo[e] - a.c[e], h = lo[e] + ext[e] - a.c[e];
+        const double near = l > 0.0 ? l : (h < 0.0 ? -h : 0.0);
+        const double far = fmax(fabs(l), fabs(h));
+        dmin += near * near;
+        dmax += far * far;
+    }
+    const double r2 = a.r * a.r;
+    if (dmin > r2 * (1.0 + 1e-10)) return 1;
+    if (dmax < r2 * (1.0 - 1e-10)) return -1;
+    return 0;
+}
+
 // fluid fraction of the cell (i, j, k) of a block with lower corner lo3 (indices may lie one
 // cell outside the block); sample points lo + (i + (s + 1/2) / nsub) * dx
 __device__ inline double sphere_cell_fluid(const SphereArgs& a, const double* lo3, int i, int j, int k, double dx) {
+    {
+        const double clo[3] = {lo3[0] + i * dx, lo3[1] + j * dx, lo3[2] + k * dx}, cext[3] = {dx, dx, dx};
+        const int side = sphere_box_side(a, clo, cext);
+        if (side > 0) return 1.0;
+        if (side < 0) return 0.0;
+    }
     int out = 0;
     const int ns = a.nsub;
     for (int sz = 0; sz < ns; sz++)
@@ -78,8 +104,12 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
         bco[(b * 3 + d) * G::NFACE + fi] = 1.0 / rho;
         // open fraction of the face
         const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-        int out = 0;
-        for (int p2 = 0; p2 < a.nsub; p2++)
+        double flo[3], fext[3] = {dx, dx, dx};
+        for (int e = 0; e < 3; e++) flo[e] = lo3[e] + idx[e] * dx;
+        fext[d] = 0.0;
+        const int fside = sphere_box_side(a, flo, fext);
+        int out = fside > 0 ? a.nsub * a.nsub : 0;
+        for (int p2 = 0; p2 < (fside == 0 ? a.nsub : 0); p2++)
             for (int p1 = 0; p1 < a.nsub; p1++) {
                 double o[3] = {0.0, 0.0, 0.0};
                 o[t1] = (p1 + 0.5) / a.nsub;

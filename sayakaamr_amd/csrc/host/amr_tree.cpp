@@ -432,6 +432,8 @@ void AmrTree::setNodeType() {
 void AmrTree::buildSurroundingBlocks() {
     const int ZD = ndim == 3 ? 1 : 0;
     const int center = surrIndex(0, 0, 0);
+    // every block's list depends on the neighbour links only: the blocks run in parallel
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < numBlocks; i++) {
         SurroundingBlocks& s = surrBlocks[i];
         s.setUnused();

@@ -153,6 +153,16 @@ int sy_tree_set_flags(sy_tree* h, const int* refine, const int* coarsen, int n) 
     return 0;
 }
 
+int sy_tree_get_flags(const sy_tree* h, int* refine, int* coarsen, int n) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    if (n < 0 || n > t->maxTreeBlockNum) return -1;
+    for (int i = 0; i < n; i++) {
+        if (refine) refine[i] = t->refineFlag[i];
+        if (coarsen) coarsen[i] = t->coarsenFlag[i];
+    }
+    return 0;
+}
+
 int sy_tree_regrid(sy_tree* h) {
     AmrTree* t = reinterpret_cast<AmrTree*>(h);
     return t->regrid() == 0 ? t->numBlocks : -1;
@@ -189,36 +199,60 @@ void sy_tree_export(const sy_tree* h, int* level, int* parent, int* which_child,
     }
 }
 
+// Does some cell centre of block n lie within `half` of the shell |x - centre| = r0?  Exactly
+// the per-cell test (|sqrt(r2) - r0| < half over the centres, i fastest), decided without the
+// scan where it cannot go either way: the centres' distances run from the nearest centre's to
+// the farthest's in steps of at most one cell width (walk one axis at a time), so when that range
+// misses the band by a margin no centre can hit, and when it overlaps the band by more than a
+// cell width some centre must.  The 1e-12 margin covers the rounding of the distances.
+static bool shell_hit(const AmrTree& t, const AmrTreeNode& n, const double* centre, double r0, double half) {
+    const int nb = t.nb, nz = t.ndim == 3 ? nb : 1;
+    double dx[3];
+    for (int d = 0; d < 3; d++) dx[d] = n.blockLength(d) / nb;
+    auto dist = [&](int i, int j, int k) {
+        const double x = n.boundBox.vlo(0) + (i + 0.5) * dx[0];
+        const double y = n.boundBox.vlo(1) + (j + 0.5) * dx[1];
+        double r2 = (x - centre[0]) * (x - centre[0]) + (y - centre[1]) * (y - centre[1]);
+        if (t.ndim == 3) {
+            const double z = n.boundBox.vlo(2) + (k + 0.5) * dx[2];
+            r2 += (z - centre[2]) * (z - centre[2]);
+        }
+        return std::sqrt(r2);
+    };
+    int inear[3] = {0, 0, 0}, ifar[3] = {0, 0, 0};
+    double step = 0.0;
+    for (int d = 0; d < t.ndim; d++) {
+        const int cnt = d == 2 ? nz : nb;
+        const double u = (centre[d] - n.boundBox.vlo(d)) / dx[d] - 0.5;
+        inear[d] = std::min(cnt - 1, std::max(0, (int)std::lround(u)));
+        ifar[d] = u < 0.5 * (cnt - 1) ? cnt - 1 : 0;
+        step = std::max(step, dx[d]);
+    }
+    const double eps = 1e-12 * std::max(1.0, r0 + half);
+    const double dmin = dist(inear[0], inear[1], inear[2]), dmax = dist(ifar[0], ifar[1], ifar[2]);
+    if (dmin - r0 >= half + eps || r0 - dmax >= half + eps) return false;
+    const double lo = std::max(dmin, r0 - half + eps), hi = std::min(dmax, r0 + half - eps);
+    if (hi - lo > step * (1.0 + 1e-12)) return true;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < nb; j++)
+            for (int i = 0; i < nb; i++)
+                if (std::fabs(dist(i, j, k) - r0) < half) return true;
+    return false;
+}
+
 // Refinement around a spherical shell, level by level from minAmrLevel: a leaf (or parent of
 // leaves) of level l < maxAmrLevel is flagged when one of its cell centres
 // x = lo + (i + 1/2) * dx lies within factor * blockLength of |x - c| = r0; one regrid per level
 // plus a final one (the same procedure as workloads.build_tree with shell_criterion).
 int sy_tree_build_shell(sy_tree* h, const double* centre, double r0, double factor) {
     AmrTree* t = reinterpret_cast<AmrTree*>(h);
-    const int nb = t->nb, nz = t->ndim == 3 ? nb : 1;
     for (int level = t->minAmrLevel; level <= t->maxAmrLevel; level++) {
         std::fill(t->refineFlag.begin(), t->refineFlag.end(), 0);
         if (level < t->maxAmrLevel)
             for (int b = 0; b < t->numBlocks; b++) {
                 const AmrTreeNode& n = t->blocks[b];
                 if (n.refineLevel != level || !(n.nodeType == NodeIsLeaf || n.nodeType == NodeHasLeafChild)) continue;
-                double dx[3];
-                for (int d = 0; d < 3; d++) dx[d] = n.blockLength(d) / nb;
-                const double half = factor * n.blockLength(0);
-                bool hit = false;
-                for (int k = 0; k < nz && !hit; k++)
-                    for (int j = 0; j < nb && !hit; j++)
-                        for (int i = 0; i < nb && !hit; i++) {
-                            const double x = n.boundBox.vlo(0) + (i + 0.5) * dx[0];
-                            const double y = n.boundBox.vlo(1) + (j + 0.5) * dx[1];
-                            double r2 = (x - centre[0]) * (x - centre[0]) + (y - centre[1]) * (y - centre[1]);
-                            if (t->ndim == 3) {
-                                const double z = n.boundBox.vlo(2) + (k + 0.5) * dx[2];
-                                r2 += (z - centre[2]) * (z - centre[2]);
-                            }
-                            hit = std::fabs(std::sqrt(r2) - r0) < half;
-                        }
-                t->refineFlag[b] = hit ? 1 : 0;
+                t->refineFlag[b] = shell_hit(*t, n, centre, r0, factor * n.blockLength(0)) ? 1 : 0;
             }
         if (t->regrid() != 0) return -1;
     }
@@ -229,15 +263,14 @@ int sy_tree_build_shell(sy_tree* h, const double* centre, double r0, double fact
 // when one of its cell centres lies within factor * (its block edge) of the shell (the rule
 // of sy_tree_build_shell); hit leaves below maxAmrLevel are flagged for refinement, leaves
 // that are not hit and above minAmrLevel for coarsening (regrid keeps only complete sibling
-// sets and the 2:1 balance).  Blocks whose box
-// cannot reach the band are rejected before their cells are visited.  Returns the number of
-// flagged blocks.
+// sets and the 2:1 balance).  Blocks whose box cannot reach the band are rejected first;
+// blocks run in parallel.  Returns the number of flagged blocks.
 int sy_tree_flag_shell(sy_tree* h, const double* centre, double r0, double factor) {
     AmrTree* t = reinterpret_cast<AmrTree*>(h);
-    const int nb = t->nb, nz = t->ndim == 3 ? nb : 1;
     std::fill(t->refineFlag.begin(), t->refineFlag.end(), 0);
     std::fill(t->coarsenFlag.begin(), t->coarsenFlag.end(), 0);
     int nflag = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nflag)
     for (int b = 0; b < t->numBlocks; b++) {
         const AmrTreeNode& n = t->blocks[b];
         if (n.nodeType != NodeIsLeaf) continue;
@@ -250,23 +283,8 @@ int sy_tree_flag_shell(sy_tree* h, const double* centre, double r0, double facto
             dmin2 += near * near;
             dmax2 += far * far;
         }
-        bool hit = false;
-        if (std::sqrt(dmin2) <= r0 + half && std::sqrt(dmax2) >= r0 - half) {
-            double dx[3];
-            for (int d = 0; d < 3; d++) dx[d] = n.blockLength(d) / nb;
-            for (int k = 0; k < nz && !hit; k++)
-                for (int j = 0; j < nb && !hit; j++)
-                    for (int i = 0; i < nb && !hit; i++) {
-                        const double x = n.boundBox.vlo(0) + (i + 0.5) * dx[0];
-                        const double y = n.boundBox.vlo(1) + (j + 0.5) * dx[1];
-                        double r2 = (x - centre[0]) * (x - centre[0]) + (y - centre[1]) * (y - centre[1]);
-                        if (t->ndim == 3) {
-                            const double z = n.boundBox.vlo(2) + (k + 0.5) * dx[2];
-                            r2 += (z - centre[2]) * (z - centre[2]);
-                        }
-                        hit = std::fabs(std::sqrt(r2) - r0) < half;
-                    }
-        }
+        const bool hit = std::sqrt(dmin2) <= r0 + half && std::sqrt(dmax2) >= r0 - half &&
+                         shell_hit(*t, n, centre, r0, half);
         if (hit && n.refineLevel < t->maxAmrLevel) {
             t->refineFlag[b] = 1;
             nflag++;

@@ -16,7 +16,10 @@ Per step (the regrid / projection cycle of the reference's flow demo, Test7.h:41
      Gaussian blobs (rhs="blobs", sayaka_gen_sphere_blobs);
   5. the mixed-precision solve to 1e-10 |b|_inf from the carried pressure, and, for comparison,
      from zero;
-  6. (projection rhs) the velocity correction u -= B flux(p) with coarse/fine faces
+  6. the judge: the true residual b - A x of the mixed solution under the FP64 operator (the
+     reference's, bit for bit), and at the steps in fp64_steps the plain FP64 solve from zero
+     beside it (its status, iterations, time and distance from the mixed solution);
+  7. (projection rhs) the velocity correction u -= B flux(p) with coarse/fine faces
      re-synchronised (sayaka_dev_project), and the divergence left on the leaves.
 
     python -m sayakaamr_amd.moving --steps 10
@@ -45,7 +48,7 @@ def compose(first, second):
 
 
 def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=1000.0, compare_zero=True,
-        max_blocks=45000, passes=3, rhs="projection", seed=1, log=None):
+        max_blocks=45000, passes=3, rhs="projection", seed=1, log=None, fp64_steps=(), fp64_max_iter=100):
     if rhs not in ("projection", "blobs"):
         raise ValueError(f"rhs must be 'projection' or 'blobs', not {rhs!r}")
     log = log or (lambda m: print(f"[moving] {m}", file=sys.stderr, flush=True))
@@ -108,9 +111,34 @@ def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=
             r = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=200, precision=D.MIXED)
             dev.sync()
             t6 = time.perf_counter()
+            # judge: the true residual of the mixed solution under the FP64 operator
+            dev.residual(L, D.R, D.SOL, D.RHS)
+            row["judge_rel_resid"] = float(dev.dot(L, D.R, D.R)[1] / bmax)
+            if step in fp64_steps:
+                sol_mixed = dev.download(D.SOL)
+                dev.zero(D.SOL)
+                dev.sync()
+                tf = time.perf_counter()
+                rf = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=fp64_max_iter, precision=D.FP64)
+                dev.sync()
+                row.update(fp64_ms=(time.perf_counter() - tf) * 1e3, fp64_iters=rf["iters"], fp64_status=rf["status"],
+                           fp64_rel_resid=float(rf["rnorm"] / bmax))
+                if rf["status"] == 0:
+                    # fluid leaf cells only (a solid cell keeps whatever the initial guess held),
+                    # each solution less its volume-weighted mean (the Neumann null space)
+                    sol64 = dev.download(D.SOL)
+                    vfl = dev.get_vf()
+                    fluid = (tree["node_type"] == 1)[:, None] & (vfl > 0)
+                    dv = np.broadcast_to(((tree["length"][:, 0] / spec.nb) ** spec.ndim)[:, None], vfl.shape)
+                    wgt = (dv * vfl)[fluid]
+                    a = sol_mixed[fluid] - np.sum(wgt * sol_mixed[fluid]) / np.sum(wgt)
+                    b = sol64[fluid] - np.sum(wgt * sol64[fluid]) / np.sum(wgt)
+                    row["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                dev.upload(D.SOL, sol_mixed)
+            t7 = time.perf_counter()
             if rhs == "projection":
                 div0, div1 = dev.project(D.SOL)
-                row.update(project_ms=(time.perf_counter() - t6) * 1e3, div_before=div0, div_after=div1)
+                row.update(project_ms=(time.perf_counter() - t7) * 1e3, div_before=div0, div_after=div1)
             leaf = tree["node_type"] == 1
             row.update(blocks=int(tree["level"].shape[0]), leaf_cells=int(leaf.sum()) * spec.nb ** spec.ndim,
                        regrid_ms=(t1 - t0) * 1e3, tables_ms=(t2 - t1) * 1e3, remap_ms=(t3 - t2) * 1e3,
@@ -140,6 +168,11 @@ def summarize(rows):
     out["steps"] = len(s)
     out["leaf_cells_mean"] = float(np.mean([r["leaf_cells"] for r in s]))
     out["max_rel_resid"] = float(max(r["rel_resid"] for r in s))
+    out["max_judge_rel_resid"] = float(max(r["judge_rel_resid"] for r in rows))
+    judged = [r for r in rows if "fp64_status" in r]
+    if judged:
+        out["fp64"] = [{k: r[k] for k in ("step", "fp64_status", "fp64_iters", "fp64_ms", "fp64_rel_resid",
+                                          "mixed_rel_l2_vs_fp64") if k in r} for r in judged]
     return out
 
 
@@ -148,8 +181,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--no-zero", action="store_true", help="skip the zero-guess comparison solve")
     ap.add_argument("--rhs", choices=("projection", "blobs"), default="projection")
+    ap.add_argument("--fp64-steps", type=int, nargs="*", default=(), help="steps with the plain FP64 solve beside")
     args = ap.parse_args(argv)
-    rows = run(steps=args.steps, compare_zero=not args.no_zero, rhs=args.rhs)
+    rows = run(steps=args.steps, compare_zero=not args.no_zero, rhs=args.rhs, fp64_steps=tuple(args.fp64_steps))
     print(json.dumps(summarize(rows)))
 
 

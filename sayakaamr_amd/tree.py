@@ -75,6 +75,13 @@ class AmrTree:
         c = np.ascontiguousarray(centre, dtype=np.float64)
         return self.lib.sy_tree_flag_shell(self.h, N.ptr(c), float(r0), float(factor))
 
+    def get_flags(self):
+        """(refineFlag, coarsenFlag) of the current blocks."""
+        n = self.numblocks
+        r, c = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.lib.sy_tree_get_flags(self.h, N.ptr(r), N.ptr(c), n)
+        return r, c
+
     def regrid_origin(self):
         """Per block, its id before the last regrid (-1 for blocks that regrid created)."""
         out = np.zeros(self.numblocks, np.int32)

@@ -60,3 +60,25 @@ def test_regrid_overflow_raises():
             t.set_refine_flags(np.ones(t.numblocks, np.int32))
             t.regrid()
     assert t.numblocks >= n
+
+
+@pytest.mark.parametrize("centre,r0,factor", [((0.53, 0.5, 0.5), 0.2, 0.5), ((0.41, 0.57, 0.49), 0.23, 0.5),
+                                              ((0.5, 0.5, 0.5), 0.2, 0.25), ((0.6, 0.45, 0.52), 0.1, 1.0)])
+def test_flag_shell_equals_cell_scan(centre, r0, factor):
+    """The shell flags (with their shortcut from the cell-centre lattice's nearest and farthest
+    distances) equal the plain test over every cell centre of every leaf."""
+    spec = W.C1_SPEC
+    t = AmrTree(3, max_blocks=8000)
+    tree = t.build_shell(spec, (0.5, 0.5, 0.5), 0.2, W.SHELL_FACTOR)
+    t.flag_shell(centre, r0, factor)
+    refine, coarsen = t.get_flags()
+    leaf = np.nonzero(tree["node_type"] == 1)[0]
+    hit = np.zeros(tree["level"].shape[0], bool)
+    hit[leaf] = W.shell_criterion(centre=centre, r0=r0, factor=factor)(tree, spec.nb, 3, leaf)
+    lv = tree["level"]
+    want_r = hit & (lv < spec.maxlevel)
+    want_c = np.zeros_like(hit)
+    want_c[leaf] = ~hit[leaf] & (lv[leaf] > spec.minlevel)
+    assert np.array_equal(refine.astype(bool), want_r)
+    assert np.array_equal(coarsen.astype(bool), want_c)
+    assert hit.sum() > 0 and want_c.sum() > 0
