@@ -85,26 +85,6 @@ def make_c1_device(D, W, spec, tree, inp):
     return dev
 
 
-def gauge(sol, tree, spec):
-    """Remove the Neumann null space: subtract the value of the pinned cell."""
-    from sayakaamr_amd import workloads as W
-
-    b, c = W.pin_cell(tree, spec)
-    return sol - sol[b, c]
-
-
-def gauge_mean(sol, tree, spec, vf):
-    """Remove the Neumann null space: subtract the volume-weighted fluid mean."""
-    leaf = tree["node_type"] == 1
-    dv = (tree["length"][:, 0] / spec.nb) ** 3
-    w = (dv[:, None] * vf)[leaf]
-    return sol - float(np.sum(w * sol[leaf]) / np.sum(w))
-
-
-def rel_l2(a, b, leaf):
-    return float(np.linalg.norm(a[leaf] - b[leaf]) / np.linalg.norm(b[leaf]))
-
-
 def time_solves(dev, D, precision, steps, prof_classes=()):
     """steps solves from a zero guess; the launches of prof_classes timed by their own events."""
     for k in prof_classes:
@@ -201,7 +181,6 @@ def cpu_model():
 # ------------------------------------------------------------------------------ legs
 def mixed_leg(dev, D, W, spec, tree, inp, seeds, steps):
     """Mixed-precision solves of seeded right-hand sides against FP64 solves of the same."""
-    leaf = tree["node_type"] == 1
     vf = inp["vf"]
     rows, ms_all = [], []
     for seed in range(seeds):
@@ -218,11 +197,9 @@ def mixed_leg(dev, D, W, spec, tree, inp, seeds, steps):
         rows.append(dict(seed=seed, ms=round(msm, 3), iters=rm["iters"], fp64_iters=rf["iters"],
                          rel_resid=float(rm["rnorm"] / rm["rnorm0"]),
                          coarse_iters_per_call=round(rm["bottom_iters"] / max(rm["bottom_calls"], 1), 1),
-                         rel_l2_vs_fp64=rel_l2(gauge(solm, tree, spec), gauge(ref_raw, tree, spec), leaf),
-                         rel_l2_vs_fp64_mean_gauge=rel_l2(gauge_mean(solm, tree, spec, vf),
-                                                          gauge_mean(ref_raw, tree, spec, vf), leaf)))
+                         rel_l2_vs_fp64=W.fluid_rel_l2(solm, ref_raw, tree, spec, vf)))
         log(f"mixed seed {seed}: {msm:.2f} ms, iters {rm['iters']} (fp64 {rf['iters']}), "
-            f"rel L2 vs fp64 {rows[-1]['rel_l2_vs_fp64']:.2e} (mean gauge {rows[-1]['rel_l2_vs_fp64_mean_gauge']:.2e})")
+            f"rel L2 vs fp64 {rows[-1]['rel_l2_vs_fp64']:.2e} (fluid cells, mean gauge)")
     dev.upload(D.RHS, inp["rhs"])
     return rows, float(np.mean(ms_all))
 
@@ -652,8 +629,7 @@ def main():
         solf = dev.download(D.SOL)
         _, classes_fast = class_profile(dev, D, D.FP64_FAST)
         fast = dict(ms_per_solve=round(msf, 3), iters=rf["iters"], rel_resid=float(rf["rnorm"] / rf["rnorm0"]),
-                    rel_l2_vs_fp64=rel_l2(gauge(solf, tree, spec), gauge(sol64, tree, spec), leaf),
-                    kernels=classes_fast)
+                    rel_l2_vs_fp64=W.fluid_rel_l2(solf, sol64, tree, spec, inp["vf"]), kernels=classes_fast)
         log(f"fp64 order-free sums: {msf:.2f} ms/solve, iters {rf['iters']}")
     if not args.no_mixed:
         mixed_rows, ms_mixed = mixed_leg(dev, D, W, spec, tree, inp, args.seeds, args.steps)
@@ -690,7 +666,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline: one FP64 solve on 1 core ...")
         cms, cres, csol, kind = cpu_baseline(spec, tree, inp)
-        err_cpu = rel_l2(gauge(sol64, tree, spec), gauge(csol, tree, spec), leaf)
+        err_cpu = W.fluid_rel_l2(sol64, csol, tree, spec, inp["vf"])
         cpu = dict(value=round(cms, 1), unit="ms/solve", cores=1, kind=kind,
                    sample=f"one full solve of the same workload (iters {cres['iters']}), "
                           f"{cpu_model()}, {os.cpu_count()} cpus visible, 1 used")

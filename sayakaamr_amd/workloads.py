@@ -400,6 +400,19 @@ def c1_rhs(tree: dict, spec: TreeSpec, seed: int, vf, af):
     return divergence_rhs(tree, spec, face_velocity(tree, spec, seed), vf, af)
 
 
+def fluid_rel_l2(a, b, tree: dict, spec: TreeSpec, vf):
+    """Relative L2 difference of two solutions of the pure-Neumann problem over the fluid leaf
+    cells (vf > 0), each less its volume-weighted mean there: the null space removed, and the
+    solid cells -- outside the system, 0 in both -- left out (with a gauge applied to every
+    cell they would carry the two solutions' different constants)."""
+    fluid = (tree["node_type"] == 1)[:, None] & (vf > 0)
+    dv = np.broadcast_to(((tree["length"][:, 0] / spec.nb) ** spec.ndim)[:, None], vf.shape)
+    w = (dv * vf)[fluid]
+    x = a[fluid] - np.sum(w * a[fluid]) / np.sum(w)
+    y = b[fluid] - np.sum(w * b[fluid]) / np.sum(w)
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
 def pin_cell(tree: dict, spec: TreeSpec, point=PIN_POINT):
     """(block, cell) of the leaf cell containing ``point``."""
     lo, hi = tree["lo"], tree["hi"]

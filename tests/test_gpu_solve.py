@@ -14,9 +14,10 @@ from sayakaamr_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 # Mixed precision (FP32 V-cycle inside the FP64 BiCGSTAB) converges to the same 1e-10
-# relative residual; its solution differs from the FP64 one by the conditioning of the
-# density-ratio-1000 operator times that residual (measured <= 6e-5 relative L2 over seeds 0-4).
-MIXED_REL_L2_TOL = 1e-3
+# relative residual; on the fluid cells (each solution less its mean: the Neumann null space)
+# it lands 3e-9 to 4e-8 relative L2 from the FP64 solution over right-hand-side seeds 0-4, the
+# same order as the order-free FP64 mode (2.9e-8), the same algorithm with other rounding.
+MIXED_REL_L2_TOL = 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -45,11 +46,6 @@ def true_residual(dev, spec, tree, sol):
     return np.max(np.abs(dev.download(D.R)[leaf]))
 
 
-def gauge(sol, tree, spec):
-    b, c = W.pin_cell(tree, spec)
-    return sol - sol[b, c]
-
-
 def test_c1_fp64_matches_reference(c1):
     g, spec, tree, inp, dev = c1
     dev.zero(D.SOL)
@@ -66,17 +62,16 @@ def test_c1_fp64_matches_reference(c1):
 
 def test_c1_mixed_precision(c1):
     g, spec, tree, inp, dev = c1
-    leaf = tree["node_type"] == 1
     dev.zero(D.SOL)
     r64 = dev.solve(precision=D.FP64)
-    ref = gauge(dev.download(D.SOL), tree, spec)
+    ref = dev.download(D.SOL)
     dev.zero(D.SOL)
     rm = dev.solve(tol_rel=1e-10, tol_abs=0.0, max_iter=200, precond=1, precision=D.MIXED,
                    pre_smooth=2, post_smooth=2)
     assert rm["status"] == 0 and rm["rnorm"] < 1e-10 * rm["rnorm0"]
     sol = dev.download(D.SOL)
     assert true_residual(dev, spec, tree, sol) <= 2e-10 * rm["rnorm0"]
-    err = np.linalg.norm((gauge(sol, tree, spec) - ref)[leaf]) / np.linalg.norm(ref[leaf])
+    err = W.fluid_rel_l2(sol, ref, tree, spec, inp["vf"])
     print(f"[gpu] mixed iters {rm['iters']} (fp64 {r64['iters']}), rel L2 vs FP64 {err:.3e}", flush=True)
     assert err <= MIXED_REL_L2_TOL
 
@@ -86,10 +81,9 @@ def test_c1_mixed_two_level_coarse_solve(c1):
     without it: both reach the outer tolerance with true residuals to match, agree with the
     FP64 solution, and the preconditioned coarse solves take fewer iterations."""
     g, spec, tree, inp, dev = c1
-    leaf = tree["node_type"] == 1
     dev.zero(D.SOL)
     dev.solve(precision=D.FP64)
-    ref = gauge(dev.download(D.SOL), tree, spec)
+    ref = dev.download(D.SOL)
     out = {}
     try:
         for two_level in (False, True):
@@ -99,7 +93,7 @@ def test_c1_mixed_two_level_coarse_solve(c1):
             assert rm["status"] == 0 and rm["rnorm"] < 1e-10 * rm["rnorm0"]
             sol = dev.download(D.SOL)
             assert true_residual(dev, spec, tree, sol) <= 2e-10 * rm["rnorm0"]
-            err = np.linalg.norm((gauge(sol, tree, spec) - ref)[leaf]) / np.linalg.norm(ref[leaf])
+            err = W.fluid_rel_l2(sol, ref, tree, spec, inp["vf"])
             assert err <= MIXED_REL_L2_TOL
             out[two_level] = rm["bottom_iters"] / rm["bottom_calls"]
             print(f"[gpu] mixed two-level={two_level}: iters {rm['iters']}, coarse its/call {out[two_level]:.1f}, "
@@ -114,16 +108,15 @@ def test_c1_fp64_fast_within_rounding(c1):
     iteration count within one; the solution agrees to the accuracy a 1e-10 relative
     residual fixes for this operator (the reference-order mode is the bit-exact one)."""
     g, spec, tree, inp, dev = c1
-    leaf = tree["node_type"] == 1
     dev.zero(D.SOL)
     dev.solve(precision=D.FP64)
-    ref = gauge(dev.download(D.SOL), tree, spec)
+    ref = dev.download(D.SOL)
     dev.zero(D.SOL)
     rf = dev.solve(tol_rel=1e-10, tol_abs=0.0, max_iter=2000, precond=1, precision=D.FP64_FAST)
     assert rf["status"] == 0 and abs(rf["iters"] - int(g["solve_iters"][0])) <= 1
     sol = dev.download(D.SOL)
     assert true_residual(dev, spec, tree, sol) <= 2e-10 * rf["rnorm0"]
-    err = np.linalg.norm((gauge(sol, tree, spec) - ref)[leaf]) / np.linalg.norm(ref[leaf])
+    err = W.fluid_rel_l2(sol, ref, tree, spec, inp["vf"])
     print(f"[gpu] fp64-fast iters {rf['iters']} (reference {int(g['solve_iters'][0])}), rel L2 {err:.3e}", flush=True)
     assert err <= 1e-6
 
