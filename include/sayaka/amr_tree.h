@@ -12,8 +12,32 @@
 #include "index_box.h"
 #include "sayaka_types.h"
 
+struct sayaka_dev_ctx;   // a device context of the C ABI (sayaka_amd.h)
+
 namespace sayaka {
 inline namespace amd {
+
+// The device side of a tree: one context per tree (on one GPU), shared by every MacSolver,
+// FillPatch and device-resident TreeData made on it, created on first use and destroyed with
+// the tree; and the topology versions its tables were last uploaded for.  A copied or
+// assigned tree starts without one (its solvers and states make a new one on first use).
+class TreeDevice {
+   public:
+    TreeDevice() = default;
+    TreeDevice(const TreeDevice&) {}
+    TreeDevice& operator=(const TreeDevice&) {
+        release();
+        return *this;
+    }
+    ~TreeDevice() { release(); }
+    sayaka_dev_ctx* ctx = nullptr;
+    int device = -1;
+    unsigned long long topo_version = ~0ull;   // topologyVersion the block tables are for
+    unsigned long long surr_version = ~0ull;   // ... and the surrounding tables
+
+   private:
+    void release();
+};
 
 struct RealBox {
     Vector3d vlo, vhi;
@@ -124,6 +148,9 @@ struct AmrTree {
     int maxTreeBlockNum = 0;
     int numBlocks = 0;
     int gridChanged = 0;
+    // bumped by every change of the block structure (init, root level, a regrid that changed
+    // the tree); the device tables are uploaded again only when it moved
+    unsigned long long topologyVersion = 0;
 
     std::vector<AmrTreeNode> blocks;
     std::vector<int> refineFlag, coarsenFlag, newChildFlag, stayFlag;
@@ -169,7 +196,18 @@ struct AmrTree {
     const IndexBox& grownBlockCellBox() const { return blockGrownIndex; }
     const AmrTreeLevel& getTreeLevel(int l) const { return levels[l]; }
 
+    // The tree's device context on GPU `device` (created on first use, sized by maxBlockNum;
+    // asking for another GPU than the first throws).
+    sayaka_dev_ctx* deviceContext(int device = 0) const;
+    // deviceContext with the block tables -- and, with surroundings, the 3^d surrounding
+    // tables -- of the current blocks uploaded if the topology changed since their last upload.
+    sayaka_dev_ctx* deviceTables(int device = 0, bool surroundings = true) const;
+    // the block tables were uploaded by the caller for the current topology (MacSolver's
+    // prepareSolver resets the solver's coefficients with them)
+    void markTopologyUploaded() const { device_.topo_version = topologyVersion; }
+
    private:
+    mutable TreeDevice device_;
     void checkCoarsen();
     int refineBlocks();
     int coarsenBlocks(int oldBlockNum);

@@ -5,12 +5,16 @@
 // (SayakaFillPatch.h:60-200: setBC, setProlongation, fillBoundary), so that a driver's
 //     FillPatch fp(tree, data); fp.setBC(bc); fp.setProlongation(c, m); fp.fillBoundary(0, n, 2);
 // compiles unchanged.  fillBoundary runs on the device (sayaka_dev_state_fill for cell data,
-// sayaka_dev_face_fill for face data): the tree's topology and surroundings and the data's
-// grown boxes go up, the filled ghosts come back.  The result is the reference's
+// sayaka_dev_face_fill for face data) in the tree's device context, shared by every patch and
+// solver of the tree; the tree's tables go up only when the tree changed since their last
+// upload, and the device staging buffers are kept between calls.  Data kept on the device
+// (TreeData::setResident) is filled in place with no host transfer; other data's grown boxes
+// go up and the filled ghosts come back.  The result is the reference's
 // FillPatch::fillBoundary (SayakaFillPatch.cpp:17-368) bit for bit.  Errors throw
 // std::runtime_error where the reference exits (an unmapped wall code, face data next to a wall).
 #pragma once
 
+#include <cstddef>
 #include <vector>
 
 #include "bc_register.h"
@@ -46,8 +50,8 @@ struct BCPatch {
 
 class FillPatch {
    public:
-    // The tree and the data must outlive the FillPatch.  A device context of the given GPU is
-    // made on the first fill and kept.
+    // The tree and the data must outlive the FillPatch.  The fills run in the tree's device
+    // context on the given GPU (AmrTree::deviceContext).
     FillPatch(const AmrTree& tree, TreeData& data, int device = 0);
     ~FillPatch();
     FillPatch(const FillPatch&) = delete;
@@ -71,6 +75,7 @@ class FillPatch {
 
    private:
     void prepareDevice();
+    double* stagingBuffer(int which, std::size_t bytes);
     void bcTables(int scomp, int ncomp, std::vector<int>& codes, std::vector<int>& mathbc,
                   std::vector<double>& bcval) const;
     const AmrTree* tree_;
@@ -78,7 +83,10 @@ class FillPatch {
     const BCPatch* bc_ = nullptr;
     std::vector<int> prolong_, restrict_;
     int device_;
-    sayaka_dev_ctx* ctx_ = nullptr;
+    sayaka_dev_ctx* ctx_ = nullptr;         // the tree's
+    double* d_buf_[2] = {nullptr, nullptr};  // device staging of host data, kept between calls
+    std::size_t buf_bytes_[2] = {0, 0};
+    std::vector<double> host_;               // host staging, kept between calls
 };
 
 }  // namespace amd

@@ -65,6 +65,9 @@ class MacSolver {
    public:
     enum Precision { FP64 = 0, Mixed = 1, FP64Fast = 2 };
 
+    // Works in the tree's device context (AmrTree::deviceContext, shared with the tree's
+    // FillPatches); the solver's device fields live there, so of several MacSolvers made on one
+    // tree (a driver that makes a new one every step) only one is used at a time.
     explicit MacSolver(const AmrTree& tree, int device = 0);
     ~MacSolver();
     MacSolver(const MacSolver&) = delete;

@@ -5,13 +5,17 @@
 // by ngrow cells (i fastest, SayakaBox.h:214-217); face data is staggered in one direction
 // (one more face than cells along it).  Valid cells are indexed from 0, ghosts from -ngrow.
 // The solver on the device keeps its own packed valid-only copies; these host objects are
-// what callers fill and read, exactly as with the reference.
+// what callers fill and read, exactly as with the reference.  A TreeData can also keep a copy
+// on the device (setResident): then the tree's ghost fills, regrid moves and new-block fills
+// run on that copy with no host transfer, and the host copy is synchronised on request.
 #pragma once
 
 #include <cstddef>
 #include <vector>
 
 #include "amr_tree.h"
+
+struct sayaka_dev_ctx;
 
 namespace sayaka {
 inline namespace amd {
@@ -42,7 +46,8 @@ using GridDataD = DoubleBlockData;
 class TreeData {
    public:
     TreeData() = default;
-    TreeData(const TreeData& o);
+    ~TreeData();
+    TreeData(const TreeData& o);   // copies the host data; the copy is not resident
     TreeData(TreeData&& o) noexcept;
     TreeData& operator=(const TreeData& o);
     TreeData& operator=(TreeData&& o) noexcept;
@@ -132,7 +137,29 @@ class TreeData {
     void remapBlocks(const std::vector<int>& origin, int nblk);
     std::size_t grownSize() const { return compsize_; }   // values per block and component
 
+    // ---- device copy (no reference counterpart: the reference's data lives in host memory)
+    // setResident(true) makes a copy in the tree's device context (AmrTree::deviceContext) in
+    // the C ABI's layout [comp][slot][grown box] and uploads the host data of the tree's
+    // current blocks.  From then on FillPatch::fillBoundary, AmrTree::regrid's block moves and
+    // initNewBlockDataAfterRegrid work on the device copy only, with no host transfer; the host
+    // copy is brought up to date by toHost(), and host changes go to the device by toDevice().
+    // setResident(false) brings the data back to the host and frees the device copy.  The
+    // tree must outlive the data.
+    void setResident(bool on, int device = 0);
+    bool resident() const { return d_data_ != nullptr; }
+    void toDevice();
+    void toHost();
+    double* deviceData() const { return d_data_; }
+    // values between two components of the device copy (slots x grown box)
+    std::size_t deviceCompStride() const { return (std::size_t)nslot_ * compsize_; }
+    // the device copy's second buffer, which a block move or a new-block fill writes before
+    // swapDeviceBuffers() makes it the data
+    double* deviceScratch() const { return d_tmp_; }
+    void swapDeviceBuffers();
+    sayaka_dev_ctx* deviceContext() const { return dctx_; }
+
    private:
+    void releaseDevice();
     std::size_t offset(int blk, int i, int j, int k, int comp) const {
         return (std::size_t)blk * blocksize_ + (std::size_t)comp * compsize_ +
                (std::size_t)((i + ngrow_) + lx_ * ((j + ngrow_) + ly_ * (k + kg_)));
@@ -145,6 +172,10 @@ class TreeData {
     IndexBox valid_, grown_;
     std::vector<double> data_;
     std::vector<DoubleBlockData> views_;
+    sayaka_dev_ctx* dctx_ = nullptr;   // the tree's context, while resident
+    double* d_data_ = nullptr;
+    double* d_tmp_ = nullptr;
+    std::vector<double> stage_;        // host staging of one component in the device layout
 };
 
 }  // namespace amd

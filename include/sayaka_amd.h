@@ -152,6 +152,22 @@ int sayaka_dev_alloc(sayaka_dev_ctx* ctx, size_t bytes, void** out);
 int sayaka_dev_free(sayaka_dev_ctx* ctx, void* ptr);
 int sayaka_dev_memcpy(sayaka_dev_ctx* ctx, void* dst, const void* src, size_t bytes, int to_device);
 
+/* The regrid's packing of a caller's state held on the device (SayakaRegrid.cpp:530-591, which
+ * moves every registered state's blocks with the tree's): block slot b of dst takes slot
+ * origin[b] of src for b < n, zeros where origin[b] == -1 and in slots n..nslot-1; both states
+ * [ncomp][comp_stride], a block slot block_len values, comp_stride >= nslot * block_len; src and
+ * dst distinct.  With sayaka_dev_state_remap's new-block fill it keeps a state resident on the
+ * device across regrids. */
+int sayaka_dev_block_move(sayaka_dev_ctx* ctx, const double* src, double* dst, int ncomp, long long comp_stride,
+                          long long block_len, int n, int nslot, const int* origin);
+
+/* Host <-> device traffic of a context since its creation (no reference counterpart; it lets a
+ * caller check that a state kept on the device is not copied back and forth): out4[0] bytes of
+ * state to the device and out4[1] to the host (sayaka_dev_memcpy, sayaka_dev_upload /
+ * download), out4[2] topology uploads (sayaka_dev_set_topology), out4[3] surroundings uploads
+ * (sayaka_dev_set_surroundings). */
+int sayaka_dev_transfer_stats(sayaka_dev_ctx* ctx, long long* out4);
+
 /* The tree's 3^d surrounding blocks for the general state fill, after every
  * sayaka_dev_set_topology: surr27[b][27] = SurroundingBlocks::blocks and bcref27[b][27] =
  * SurroundingBlocks::bcrefs of block b, both in the reference's [ii+1][jj+1][kk+1] order

@@ -66,6 +66,15 @@ int sy_tree_state_register(sy_tree* t, int dir, int ncomp, int ngrow, const int*
 int sy_tree_state_set_grown(sy_tree* t, int idx, const double* data);
 int sy_tree_state_get_grown(const sy_tree* t, int idx, double* data);
 int sy_tree_init_new_data(sy_tree* t);
+/* FillPatch::fillBoundary(scomp, ncomp, nlayer) (SayakaFillPatch.cpp:17-30) of a registered
+ * state, through its FillPatch */
+int sy_tree_state_fill(sy_tree* t, int idx, int scomp, int ncomp, int nlayer);
+/* TreeData::setResident (no reference counterpart): on != 0 keeps a copy of the state on GPU
+ * `device`, where regrid moves, new-block fills and ghost fills then run with no host transfer;
+ * set_grown / get_grown synchronise it; on == 0 brings it back and frees it. */
+int sy_tree_state_set_resident(sy_tree* t, int idx, int on, int device);
+/* host <-> device traffic of the tree's device context (sayaka_dev_transfer_stats) */
+int sy_tree_transfer_stats(const sy_tree* t, long long* out4);
 
 /* AmrTree::getTreeLevel (SayakaTree.h:317-318) */
 int sy_tree_level_count(const sy_tree* t, int level);

@@ -69,6 +69,9 @@ _HOST_SIG = {
     "sy_tree_state_set_grown": ([vp, i32, vp], i32),
     "sy_tree_state_get_grown": ([vp, i32, vp], i32),
     "sy_tree_init_new_data": ([vp], i32),
+    "sy_tree_state_fill": ([vp, i32, i32, i32, i32], i32),
+    "sy_tree_state_set_resident": ([vp, i32, i32, i32], i32),
+    "sy_tree_transfer_stats": ([vp, vp], i32),
     "sy_write_chombo": ([vp, i32, vp, C.c_char_p, f64, f64, i32, vp], i32),
 }
 

@@ -173,6 +173,10 @@ struct sayaka_dev_ctx {
     size_t fill_icap = 0, fill_vcap = 0;
     int* d_remap = nullptr;        // state remap: origin, then the per-level block lists
     size_t remap_cap = 0;
+    int* d_move = nullptr;         // block move: origin per slot
+    size_t move_cap = 0;
+    // host <-> device traffic: state bytes each way, topology and surroundings uploads
+    long long xfer[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -193,6 +197,22 @@ __global__ void k_copy_dot_norm(const double* __restrict__ scal, double* __restr
         out2[0] = scal[S_DOT];
         out2[1] = scal[S_NORM];
     }
+}
+
+// The regrid's packing of a caller's state: slot b of every component takes slot origin[b]
+// (zeros where origin[b] == -1 and for slots b >= n).  One workgroup per (slot, component).
+__global__ void __launch_bounds__(256) k_block_move(const double* __restrict__ src, double* __restrict__ dst,
+                                                    const int* __restrict__ origin, long long comp_stride,
+                                                    long long block_len, int n) {
+    const int b = blockIdx.x, comp = blockIdx.y;
+    const int o = b < n ? origin[b] : -1;
+    double* d = dst + comp * comp_stride + (long long)b * block_len;
+    if (o < 0) {
+        for (long long q = threadIdx.x; q < block_len; q += blockDim.x) d[q] = 0.0;
+        return;
+    }
+    const double* s = src + comp * comp_stride + (long long)o * block_len;
+    for (long long q = threadIdx.x; q < block_len; q += blockDim.x) d[q] = s[q];
 }
 
 __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
@@ -1192,6 +1212,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_ghost) hipFree(c->d_ghost);
     if (c->d_remap_tmp) hipFree(c->d_remap_tmp);
     if (c->d_remap_idx) hipFree(c->d_remap_idx);
+    if (c->d_move) hipFree(c->d_move);
     if (c->d_vel) hipFree(c->d_vel);
     if (c->d_div) hipFree(c->d_div);
     if (c->d_surr) hipFree(c->d_surr);
@@ -1342,6 +1363,7 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
         c->pc_version = -1;
     }
     c->vel_valid = 0;  // the face velocity belongs to the previous tree
+    c->xfer[2]++;
     // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32)
     c->alpha = 0.0;
     c->beta = 1.0;
@@ -1425,6 +1447,7 @@ int sayaka_dev_upload(sayaka_dev_ctx* c, int field, const double* h) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->d_field[field], h, (size_t)c->nblk * c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->xfer[0] += (long long)c->nblk * c->ncell * (long long)sizeof(double);
     return 0;
 }
 
@@ -1433,6 +1456,7 @@ int sayaka_dev_download(sayaka_dev_ctx* c, int field, double* h) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(h, c->d_field[field], (size_t)c->nblk * c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->xfer[1] += (long long)c->nblk * c->ncell * (long long)sizeof(double);
     return 0;
 }
 
@@ -1513,6 +1537,36 @@ int sayaka_dev_memcpy(sayaka_dev_ctx* c, void* dst, const void* src, size_t byte
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->xfer[to_device ? 0 : 1] += (long long)bytes;
+    return 0;
+}
+
+int sayaka_dev_transfer_stats(sayaka_dev_ctx* c, long long* out4) {
+    if (!c || !out4) return fail(-1, "bad argument");
+    for (int q = 0; q < 4; q++) out4[q] = c->xfer[q];
+    return 0;
+}
+
+int sayaka_dev_block_move(sayaka_dev_ctx* c, const double* src, double* dst, int ncomp, long long comp_stride,
+                          long long block_len, int n, int nslot, const int* origin) {
+    if (!c || !src || !dst || !origin || ncomp < 1 || block_len < 1 || n < 0 || nslot < n || nslot < 1 ||
+        comp_stride < (long long)nslot * block_len || ncomp > 65535)
+        return fail(-1, "bad argument");
+    if (src == dst) return fail(-1, "the source and the destination must be distinct buffers");
+    for (int b = 0; b < n; b++)
+        if (origin[b] < -1 || origin[b] >= nslot) return fail(-1, "origin out of range");
+    HIPCHK(hipSetDevice(c->device));
+    if ((size_t)nslot > c->move_cap) {
+        if (c->d_move) HIPCHK(hipFree(c->d_move));
+        c->d_move = nullptr;
+        HIPCHK(hipMalloc(&c->d_move, (size_t)nslot * sizeof(int)));
+        c->move_cap = nslot;
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(c->d_move, origin, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_block_move, dim3(nslot, ncomp), dim3(256), 0, c->stream, src, dst, c->d_move, comp_stride,
+                       block_len, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));   // origin is the caller's host array
     return 0;
 }
 
@@ -1533,6 +1587,7 @@ int sayaka_dev_set_surroundings(sayaka_dev_ctx* c, int nblk, const int* surr27, 
     HIPCHK(hipMemcpyAsync(c->d_bcref, bcref27, (size_t)nblk * 27 * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->surr_n = nblk;
+    c->xfer[3]++;
     return 0;
 }
 

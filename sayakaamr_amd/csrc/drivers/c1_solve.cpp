@@ -277,6 +277,7 @@ int main(int argc, char** argv) {
     double ms[2];
     int iters[2], status[2];
     std::vector<std::vector<double>> sols(2);
+    std::vector<double> wts;
     for (int pass = 0; pass < 2; pass++) {
         mac.setPrecision(pass == 0 ? MacSolver::FP64 : MacSolver::Mixed);
         mac.setZeroSol();
@@ -293,20 +294,27 @@ int main(int argc, char** argv) {
         if (pass == 0 && argc > 3 && (write_output(tree, phi, argv[3]) != 0 || write_filled(tree, phi, argv[3], WALL) != 0))
             return 3;
         auto& s = sols[pass];
+        wts.clear();
         for (int b = 0; b < tree.numBlocks; b++)
             if (tree[b].isLeaf())
                 for (int k = 0; k < nb; k++)
                     for (int j = 0; j < nb; j++)
-                        for (int i = 0; i < nb; i++) s.push_back(phi(b, i, j, k, 0));
+                        for (int i = 0; i < nb; i++)
+                            if (vf(b, i, j, k, 0) > 0) {   // fluid cells: the solid ones are outside the system
+                                s.push_back(phi(b, i, j, k, 0));
+                                wts.push_back(tree.getBlockCellVolume(b) * vf(b, i, j, k, 0));
+                            }
     }
-    // gauge: the Neumann problem fixes phi up to a constant -> compare mean-free solutions
-    double m0 = 0, m1 = 0;
+    // gauge: the Neumann problem fixes phi up to a constant -> compare solutions less their
+    // volume-weighted fluid means
+    double m0 = 0, m1 = 0, wsum = 0;
     for (size_t q = 0; q < sols[0].size(); q++) {
-        m0 += sols[0][q];
-        m1 += sols[1][q];
+        m0 += wts[q] * sols[0][q];
+        m1 += wts[q] * sols[1][q];
+        wsum += wts[q];
     }
-    m0 /= sols[0].size();
-    m1 /= sols[1].size();
+    m0 /= wsum;
+    m1 /= wsum;
     double num = 0, den = 0;
     for (size_t q = 0; q < sols[0].size(); q++) {
         const double a = sols[0][q] - m0, c = sols[1][q] - m1;

@@ -7,15 +7,55 @@
 // loop implements; the loops are written against this file's run-time dimension.
 #include "amr_tree.h"
 
+#include "dev_tree.h"
 #include "fill_patch.h"
+#include "sayaka_amd.h"
 #include "tree_data.h"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
+#include <string>
 
 namespace sayaka {
 inline namespace amd {
+
+void TreeDevice::release() {
+    if (ctx) sayaka_dev_destroy(ctx);
+    ctx = nullptr;
+    device = -1;
+    topo_version = surr_version = ~0ull;
+}
+
+sayaka_dev_ctx* AmrTree::deviceContext(int device) const {
+    if (device_.ctx) {
+        if (device != device_.device)
+            throw std::invalid_argument("AmrTree: the tree's device context is on GPU " +
+                                        std::to_string(device_.device) + ", not " + std::to_string(device));
+        return device_.ctx;
+    }
+    sayaka_dev_ctx* c = nullptr;
+    if (sayaka_dev_create(device, ndim, nb, maxBlockNum, &c) < 0)
+        throw std::runtime_error(std::string("AmrTree: device context: ") + sayaka_dev_last_error());
+    device_.ctx = c;
+    device_.device = device;
+    return c;
+}
+
+sayaka_dev_ctx* AmrTree::deviceTables(int device, bool surroundings) const {
+    sayaka_dev_ctx* c = deviceContext(device);
+    if (device_.topo_version != topologyVersion) {
+        upload_tree_topology(c, *this);
+        device_.topo_version = topologyVersion;
+        device_.surr_version = ~0ull;   // the surrounding tables follow the block count
+    }
+    if (surroundings && device_.surr_version != topologyVersion) {
+        upload_tree_surroundings(c, *this);
+        device_.surr_version = topologyVersion;
+    }
+    return c;
+}
 
 void AmrTreeNode::setUnused() {
     refineLevel = -1;
@@ -63,6 +103,8 @@ void AmrTree::init() {
     maxTreeBlockNum = maxBlockNum;
     numBlocks = 0;
     gridChanged = 0;
+    topologyVersion++;
+    device_ = TreeDevice{};   // the pool size may have changed: a new context on first use
     blocks.assign(maxTreeBlockNum, AmrTreeNode{});
     for (auto& b : blocks) b.setUnused();
     refineFlag.assign(maxTreeBlockNum, 0);
@@ -120,6 +162,7 @@ void AmrTree::initUniformRootLevel(const int ndiv[3], const RealBox& pb, const i
                     r.neighbor[f] = nbr;
                 }
             }
+    topologyVersion++;
 }
 
 // Regrid once per level up to minAmrLevel; regrid() itself flags every leaf below
@@ -180,6 +223,7 @@ int AmrTree::regrid() {
         int nsave = numBlocks;
         if (nrefine > 0 && refineBlocks() != 0) return -1;
         nsave = coarsenBlocks(nsave);
+        topologyVersion++;
     }
     buildSurroundingBlocks();
     cacheTreeLevels();

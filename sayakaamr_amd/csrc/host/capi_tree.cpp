@@ -12,6 +12,7 @@
 
 #include "amr_tree.h"
 #include "fill_patch.h"
+#include "sayaka_amd.h"
 #include "sayaka_host.h"
 #include "tree_data.h"
 
@@ -115,19 +116,75 @@ int sy_tree_state_set_grown(sy_tree* h, int idx, const double* data) {
     const int n = reinterpret_cast<AmrTree*>(h)->numBlocks;
     const size_t per = (size_t)n * d->grownSize();
     for (int c = 0; c < d->numComp(); c++) d->unpackGrown(c, n, data + c * per);
+    try {
+        if (d->resident()) d->toDevice();
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
     return 0;
 }
 
 int sy_tree_state_get_grown(const sy_tree* h, int idx, double* data) {
-    const TreeData* d = state_data(reinterpret_cast<const AmrTree*>(h), idx);
+    TreeData* d = state_data(reinterpret_cast<const AmrTree*>(h), idx);
     if (!d || !data) {
         g_terr = "bad state index";
+        return -1;
+    }
+    try {
+        if (d->resident()) d->toHost();
+    } catch (const std::exception& e) {
+        g_terr = e.what();
         return -1;
     }
     const int n = reinterpret_cast<const AmrTree*>(h)->numBlocks;
     const size_t per = (size_t)n * d->grownSize();
     for (int c = 0; c < d->numComp(); c++) d->packGrown(c, n, data + c * per);
     return 0;
+}
+
+int sy_tree_state_set_resident(sy_tree* h, int idx, int on, int device) {
+    TreeData* d = state_data(reinterpret_cast<AmrTree*>(h), idx);
+    if (!d) {
+        g_terr = "bad state index";
+        return -1;
+    }
+    try {
+        d->setResident(on != 0, device);
+        return 0;
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
+}
+
+int sy_tree_state_fill(sy_tree* h, int idx, int scomp, int ncomp, int nlayer) {
+    AmrTree* t = reinterpret_cast<AmrTree*>(h);
+    if (!state_data(t, idx) || !t->treeStateData[idx].fill_new_patch) {
+        g_terr = "bad state index";
+        return -1;
+    }
+    try {
+        t->treeStateData[idx].fill_new_patch->fillBoundary(scomp, ncomp, nlayer);
+        return 0;
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
+}
+
+int sy_tree_transfer_stats(const sy_tree* h, long long* out4) {
+    const AmrTree* t = reinterpret_cast<const AmrTree*>(h);
+    if (!t || !out4) {
+        g_terr = "bad argument";
+        return -1;
+    }
+    try {
+        return sayaka_dev_transfer_stats(t->deviceContext(0), out4) < 0 ? -1 : 0;
+    } catch (const std::exception& e) {
+        g_terr = e.what();
+        return -1;
+    }
 }
 
 int sy_tree_init_new_data(sy_tree* h) {

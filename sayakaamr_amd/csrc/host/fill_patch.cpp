@@ -1,6 +1,7 @@
 // fill_patch.cpp -- FillPatch on the GPU (see fill_patch.h).
 #include "fill_patch.h"
 
+#include <cstddef>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -22,17 +23,30 @@ FillPatch::FillPatch(const AmrTree& tree, TreeData& data, int device)
       restrict_(data.numComp(), InterpPatch::RESTRICT_AVERAGE), device_(device) {}
 
 FillPatch::~FillPatch() {
-    if (ctx_) sayaka_dev_destroy(ctx_);
+    // the context is the tree's (it outlives the patch); the staging buffers are the patch's
+    if (ctx_) {
+        if (d_buf_[0]) sayaka_dev_free(ctx_, d_buf_[0]);
+        if (d_buf_[1]) sayaka_dev_free(ctx_, d_buf_[1]);
+    }
 }
 
 void FillPatch::setProlongation(int comp, int prolong) { prolong_.at(comp) = prolong; }
 
 void FillPatch::prepareDevice() {
-    if (!ctx_)
-        check(sayaka_dev_create(device_, tree_->ndim, tree_->nb, tree_->maxBlockNum, &ctx_), "FillPatch: device context");
-    // the tree may have been regridded since the last fill: tables every time
-    upload_tree_topology(ctx_, *tree_);
-    upload_tree_surroundings(ctx_, *tree_);
+    // the tree's context; its tables go up only when the tree changed since the last upload
+    ctx_ = tree_->deviceTables(device_, true);
+}
+
+double* FillPatch::stagingBuffer(int which, std::size_t bytes) {
+    // grown only, kept between calls: no allocation per fill in steady state
+    if (bytes > buf_bytes_[which]) {
+        if (d_buf_[which]) sayaka_dev_free(ctx_, d_buf_[which]);
+        d_buf_[which] = nullptr;
+        buf_bytes_[which] = 0;
+        check(sayaka_dev_alloc(ctx_, bytes, (void**)&d_buf_[which]), "FillPatch: device state");
+        buf_bytes_[which] = bytes;
+    }
+    return d_buf_[which];
 }
 
 void FillPatch::bcTables(int scomp, int ncomp, std::vector<int>& codes, std::vector<int>& mathbc,
@@ -75,22 +89,27 @@ void FillPatch::initNewBlocks(const std::vector<int>& isnew) {
     std::vector<int> origin(nblk);
     for (int b = 0; b < nblk; b++) origin[b] = isnew[b] ? -1 : b;   // the blocks have moved already
     prepareDevice();
+    if (data_->resident()) {
+        // the device copy is the data: fill the new blocks there, no host transfer
+        const int nslot = data_->numBlockSlots();
+        check(sayaka_dev_state_remap(ctx_, data_->deviceData(), nslot, nblk, data_->deviceScratch(), ncomp, nslot,
+                                     data_->numGrow(), fdir, origin.data(), prolong_.data(), (int)codes.size(),
+                                     codes.data(), mathbc.data(), bcval.data()),
+              "FillPatch::initNewBlocks");
+        data_->swapDeviceBuffers();
+        return;
+    }
     const size_t per = (size_t)nblk * data_->grownSize();
-    std::vector<double> host(per * ncomp);
-    for (int c = 0; c < ncomp; c++) data_->packGrown(c, nblk, host.data() + c * per);
-    double *d_old = nullptr, *d_new = nullptr;
-    check(sayaka_dev_alloc(ctx_, host.size() * sizeof(double), (void**)&d_old), "FillPatch: device state");
-    int rc = sayaka_dev_alloc(ctx_, host.size() * sizeof(double), (void**)&d_new);
-    if (rc >= 0) rc = sayaka_dev_memcpy(ctx_, d_old, host.data(), host.size() * sizeof(double), 1);
-    if (rc >= 0)
-        rc = sayaka_dev_state_remap(ctx_, d_old, nblk, nblk, d_new, ncomp, nblk, data_->numGrow(), fdir, origin.data(),
-                                    prolong_.data(), (int)codes.size(), codes.data(), mathbc.data(), bcval.data());
-    if (rc >= 0) rc = sayaka_dev_memcpy(ctx_, host.data(), d_new, host.size() * sizeof(double), 0);
-    const std::string err = rc < 0 ? sayaka_dev_last_error() : "";
-    sayaka_dev_free(ctx_, d_old);
-    if (d_new) sayaka_dev_free(ctx_, d_new);
-    if (rc < 0) throw std::runtime_error("FillPatch::initNewBlocks: " + err);
-    for (int c = 0; c < ncomp; c++) data_->unpackGrown(c, nblk, host.data() + c * per);
+    host_.resize(per * ncomp);
+    for (int c = 0; c < ncomp; c++) data_->packGrown(c, nblk, host_.data() + c * per);
+    double* d_old = stagingBuffer(0, host_.size() * sizeof(double));
+    double* d_new = stagingBuffer(1, host_.size() * sizeof(double));
+    check(sayaka_dev_memcpy(ctx_, d_old, host_.data(), host_.size() * sizeof(double), 1), "FillPatch::initNewBlocks");
+    check(sayaka_dev_state_remap(ctx_, d_old, nblk, nblk, d_new, ncomp, nblk, data_->numGrow(), fdir, origin.data(),
+                                 prolong_.data(), (int)codes.size(), codes.data(), mathbc.data(), bcval.data()),
+          "FillPatch::initNewBlocks");
+    check(sayaka_dev_memcpy(ctx_, host_.data(), d_new, host_.size() * sizeof(double), 0), "FillPatch::initNewBlocks");
+    for (int c = 0; c < ncomp; c++) data_->unpackGrown(c, nblk, host_.data() + c * per);
 }
 
 void FillPatch::fillBoundary(int scomp, int ncomp, int nlayer) {
@@ -106,24 +125,26 @@ void FillPatch::fillBoundary(int scomp, int ncomp, int nlayer) {
     std::vector<double> bcval;
     bcTables(scomp, ncomp, codes, mathbc, bcval);
     prepareDevice();
-    const size_t per = (size_t)nblk * data_->grownSize();
-    std::vector<double> host(per * ncomp);
-    for (int c = 0; c < ncomp; c++) data_->packGrown(scomp + c, nblk, host.data() + c * per);
-    double* d = nullptr;
-    check(sayaka_dev_alloc(ctx_, host.size() * sizeof(double), (void**)&d), "FillPatch: device state");
-    int rc = sayaka_dev_memcpy(ctx_, d, host.data(), host.size() * sizeof(double), 1);
-    if (rc >= 0) {
+    auto fill = [&](double* state, int nc_total, int blocks_alloc, int sc) {
         if (fdir < 0)
-            rc = sayaka_dev_state_fill(ctx_, d, ncomp, nblk, data_->numGrow(), 0, ncomp, nlayer, prolong.data(),
-                                       (int)codes.size(), codes.data(), mathbc.data(), bcval.data());
-        else
-            rc = sayaka_dev_face_fill(ctx_, d, ncomp, nblk, fdir, data_->numGrow(), 0, ncomp, nlayer, prolong.data());
+            return sayaka_dev_state_fill(ctx_, state, nc_total, blocks_alloc, data_->numGrow(), sc, ncomp, nlayer,
+                                         prolong.data(), (int)codes.size(), codes.data(), mathbc.data(), bcval.data());
+        return sayaka_dev_face_fill(ctx_, state, nc_total, blocks_alloc, fdir, data_->numGrow(), sc, ncomp, nlayer,
+                                    prolong.data());
+    };
+    if (data_->resident()) {
+        // the device copy is the data: fill it in place, no host transfer
+        check(fill(data_->deviceData(), data_->numComp(), data_->numBlockSlots(), scomp), "FillPatch::fillBoundary");
+        return;
     }
-    if (rc >= 0) rc = sayaka_dev_memcpy(ctx_, host.data(), d, host.size() * sizeof(double), 0);
-    const std::string err = rc < 0 ? sayaka_dev_last_error() : "";
-    sayaka_dev_free(ctx_, d);
-    if (rc < 0) throw std::runtime_error("FillPatch::fillBoundary: " + err);
-    for (int c = 0; c < ncomp; c++) data_->unpackGrown(scomp + c, nblk, host.data() + c * per);
+    const size_t per = (size_t)nblk * data_->grownSize();
+    host_.resize(per * ncomp);
+    for (int c = 0; c < ncomp; c++) data_->packGrown(scomp + c, nblk, host_.data() + c * per);
+    double* d = stagingBuffer(0, host_.size() * sizeof(double));
+    check(sayaka_dev_memcpy(ctx_, d, host_.data(), host_.size() * sizeof(double), 1), "FillPatch::fillBoundary");
+    check(fill(d, ncomp, nblk, 0), "FillPatch::fillBoundary");
+    check(sayaka_dev_memcpy(ctx_, host_.data(), d, host_.size() * sizeof(double), 0), "FillPatch::fillBoundary");
+    for (int c = 0; c < ncomp; c++) data_->unpackGrown(scomp + c, nblk, host_.data() + c * per);
 }
 
 }  // namespace amd

@@ -15,7 +15,9 @@ inline namespace amd {
 MacSolver::MacSolver(const AmrTree& tree, int device) : tree_(&tree) {
     ncell_ = tree.cellsPerBlock();
     nface_ = (tree.nb + 1) * tree.nb * (tree.ndim == 3 ? tree.nb : 1);
-    check(sayaka_dev_create(device, tree.ndim, tree.nb, tree.maxBlockNum, &ctx_), "MacSolver: device context");
+    // the tree's context: one per tree, shared with its FillPatches (the solver's device fields
+    // live there, so one MacSolver per tree solves at a time)
+    ctx_ = tree.deviceContext(device);
     towers_.assign(tree.maxAmrLevel + 1, MGLevelTower{});
     for (int m = 0; m <= tree.maxAmrLevel; m++) {
         towers_[m].level = m;
@@ -40,9 +42,7 @@ static void build_towers(const AmrTree& t, std::vector<MGLevelTower>& towers) {
     }
 }
 
-MacSolver::~MacSolver() {
-    if (ctx_) sayaka_dev_destroy(ctx_);
-}
+MacSolver::~MacSolver() = default;   // the context belongs to the tree
 
 void MacSolver::check(int rc, const char* what) const {
     if (rc < 0) throw std::runtime_error(std::string(what) + ": " + sayaka_dev_last_error());
@@ -86,7 +86,9 @@ void upload_tree_surroundings(sayaka_dev_ctx* ctx, const AmrTree& t) {
 void MacSolver::prepareSolver() {
     const AmrTree& t = *tree_;
     nblk_ = t.numBlocks;
+    // always: the upload also resets the device coefficients and BC, as prepareSolver does
     upload_tree_topology(ctx_, t);
+    t.markTopologyUploaded();
     build_towers(t, towers_);
     // prepareSolver resets the coefficients to unit values (SayakaMacSolver.cpp:12-26)
     unitIB_ = unitB_ = true;

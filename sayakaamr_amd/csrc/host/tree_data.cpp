@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <string>
+
+#include "sayaka_amd.h"
 
 namespace sayaka {
 inline namespace amd {
@@ -62,6 +65,8 @@ TreeData::TreeData(const TreeData& o)
 
 TreeData::TreeData(TreeData&& o) noexcept { *this = std::move(o); }
 
+TreeData::~TreeData() { releaseDevice(); }
+
 TreeData& TreeData::operator=(const TreeData& o) {
     if (this != &o) {
         TreeData t(o);
@@ -71,6 +76,13 @@ TreeData& TreeData::operator=(const TreeData& o) {
 }
 
 TreeData& TreeData::operator=(TreeData&& o) noexcept {
+    if (this == &o) return *this;
+    releaseDevice();
+    dctx_ = o.dctx_;
+    d_data_ = o.d_data_;
+    d_tmp_ = o.d_tmp_;
+    o.dctx_ = nullptr;
+    o.d_data_ = o.d_tmp_ = nullptr;
     tree_ = o.tree_;
     ndim_ = o.ndim_;
     nb_ = o.nb_;
@@ -237,7 +249,78 @@ void TreeData::unpackGrown(int comp, int nblk, const double* in) {
                     in + (std::size_t)b * compsize_, compsize_ * sizeof(double));
 }
 
+void TreeData::releaseDevice() {
+    if (dctx_) {
+        if (d_data_) sayaka_dev_free(dctx_, d_data_);
+        if (d_tmp_) sayaka_dev_free(dctx_, d_tmp_);
+    }
+    dctx_ = nullptr;
+    d_data_ = d_tmp_ = nullptr;
+}
+
+static void dev_check(int rc, const char* what) {
+    if (rc < 0) throw std::runtime_error(std::string("TreeData::") + what + ": " + sayaka_dev_last_error());
+}
+
+void TreeData::setResident(bool on, int device) {
+    if (!defined()) throw std::logic_error("TreeData::setResident: undefined data");
+    if (on == resident()) return;
+    if (!on) {
+        toHost();
+        releaseDevice();
+        return;
+    }
+    sayaka_dev_ctx* c = tree_->deviceContext(device);
+    const std::size_t bytes = deviceCompStride() * ncomp_ * sizeof(double);
+    double *a = nullptr, *b = nullptr;
+    dev_check(sayaka_dev_alloc(c, bytes, (void**)&a), "setResident");
+    if (sayaka_dev_alloc(c, bytes, (void**)&b) < 0) {
+        const std::string err = sayaka_dev_last_error();
+        sayaka_dev_free(c, a);
+        throw std::runtime_error("TreeData::setResident: " + err);
+    }
+    dctx_ = c;
+    d_data_ = a;
+    d_tmp_ = b;
+    toDevice();
+}
+
+void TreeData::toDevice() {
+    if (!resident()) throw std::logic_error("TreeData::toDevice: the data is not resident");
+    const int n = tree_->numBlocks;
+    if (n < 1) return;
+    stage_.resize((std::size_t)n * compsize_);
+    for (int c = 0; c < ncomp_; c++) {
+        packGrown(c, n, stage_.data());
+        dev_check(sayaka_dev_memcpy(dctx_, d_data_ + c * deviceCompStride(), stage_.data(),
+                                    stage_.size() * sizeof(double), 1), "toDevice");
+    }
+}
+
+void TreeData::toHost() {
+    if (!resident()) throw std::logic_error("TreeData::toHost: the data is not resident");
+    const int n = tree_->numBlocks;
+    if (n < 1) return;
+    stage_.resize((std::size_t)n * compsize_);
+    for (int c = 0; c < ncomp_; c++) {
+        dev_check(sayaka_dev_memcpy(dctx_, stage_.data(), d_data_ + c * deviceCompStride(),
+                                    stage_.size() * sizeof(double), 0), "toHost");
+        unpackGrown(c, n, stage_.data());
+    }
+}
+
+void TreeData::swapDeviceBuffers() { std::swap(d_data_, d_tmp_); }
+
 void TreeData::remapBlocks(const std::vector<int>& origin, int nblk) {
+    if (resident()) {
+        // the device copy is the data: move its blocks there (the host copy is left as it was)
+        if ((int)origin.size() < nblk) throw std::invalid_argument("TreeData::remapBlocks: origin too short");
+        dev_check(sayaka_dev_block_move(dctx_, d_data_, d_tmp_, ncomp_, (long long)deviceCompStride(),
+                                        (long long)compsize_, nblk, nslot_, origin.data()),
+                  "remapBlocks");
+        swapDeviceBuffers();
+        return;
+    }
     std::vector<double> out(data_.size(), 0.0);
     for (int b = 0; b < nblk && b < nslot_; b++)
         if (origin[b] >= 0)

@@ -157,6 +157,24 @@ class AmrTree:
         """AmrTree::initNewBlockDataAfterRegrid for every registered state (on the GPU)."""
         self._tchk(self.lib.sy_tree_init_new_data(self.h))
 
+    def state_fill(self, idx, nlayer, scomp=0, ncomp=None):
+        """FillPatch::fillBoundary of a registered state (on the GPU)."""
+        nc = self._states[idx][1] - scomp if ncomp is None else ncomp
+        self._tchk(self.lib.sy_tree_state_fill(self.h, idx, scomp, nc, nlayer))
+
+    def state_set_resident(self, idx, on=True, device=0):
+        """Keep a registered state on the device (TreeData::setResident): regrid moves,
+        new-block fills and ghost fills then run there with no host transfer."""
+        self._tchk(self.lib.sy_tree_state_set_resident(self.h, idx, int(bool(on)), device))
+
+    def transfer_stats(self):
+        """Host <-> device traffic of the tree's device context: bytes of state to the device,
+        to the host, topology uploads, surroundings uploads."""
+        out = np.zeros(4, np.int64)
+        self._tchk(self.lib.sy_tree_transfer_stats(self.h, N.ptr(out)))
+        return dict(to_device=int(out[0]), to_host=int(out[1]), topology_uploads=int(out[2]),
+                    surroundings_uploads=int(out[3]))
+
     def level_blocks(self, level: int) -> np.ndarray:
         n = self.lib.sy_tree_level_count(self.h, level)
         out = np.zeros(n, np.int32)

@@ -39,7 +39,7 @@ def test_cpp_driver(tmp_path):
     assert kv["blocks"] == "4864" and kv["leaves"] == "4264"
     assert kv["fp64_status"] == "0" and kv["mixed_status"] == "0"
     assert int(kv["fp64_iters"]) < 50
-    assert float(kv["rel_l2_mixed_vs_fp64"]) < 1e-3
+    assert float(kv["rel_l2_mixed_vs_fp64"]) < 1e-6   # fluid cells, each solution less its mean
     check_fill(h5)
     check_output(h5)
 
