@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 HOST_LIB = os.path.join(LIBDIR, "libsayaka_host.so")
 DEV_LIB = os.path.join(LIBDIR, "libsayaka_dev.so")
+DEV_PHASES_LIB = os.path.join(LIBDIR, "libsayaka_dev_phases.so")   # the phase-timing build
 
 i32, f64, vp = C.c_int, C.c_double, C.c_void_p
 _host = None

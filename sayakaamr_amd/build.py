@@ -46,14 +46,35 @@ def _run(cmd, verbose):
     return r
 
 
-def build_dev(verbose=False) -> str:
+def _dev_cmd(out, defines=()):
+    return [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", *PROMOTE_FLAGS, *defines, f"-I{INC}",
+            *[os.path.join(CSRC, s) for s in DEV_SOURCES], "-o", out + ".tmp"]
+
+
+def build_dev(verbose=False, phases=True) -> str:
+    """libsayaka_dev.so and, beside it (phases), libsayaka_dev_phases.so: the same library with
+    the coarsest-level solve's phase timing compiled in (include/sayaka_phases.h), built
+    concurrently."""
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsayaka_dev.so")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", *PROMOTE_FLAGS, f"-I{INC}",
-           *[os.path.join(CSRC, s) for s in DEV_SOURCES], "-o", out + ".tmp"]
-    _run(cmd, verbose)
-    os.replace(out + ".tmp", out)
+    outp = os.path.join(LIB, "libsayaka_dev_phases.so")
+    procs = [(out, subprocess.Popen(_dev_cmd(out), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))]
+    if phases:
+        procs.append((outp, subprocess.Popen(_dev_cmd(outp, ["-DSAYAKA_BOTTOM_PHASES"]), stdout=subprocess.PIPE,
+                                             stderr=subprocess.PIPE, text=True)))
+    failed = []
+    for o, p in procs:
+        so, se = p.communicate()
+        if verbose:
+            print(" ".join(p.args), flush=True)
+        if p.returncode != 0:
+            sys.stderr.write(so + se)
+            failed.append(o)
+    if failed:
+        raise RuntimeError(f"build failed: {failed}")
+    for o, _ in procs:
+        os.replace(o + ".tmp", o)
     return out
 
 

@@ -41,6 +41,37 @@
 
 namespace sydev {
 
+// Phase timing (the library built with SAYAKA_BOTTOM_PHASES, lib/libsayaka_dev_phases.so; the
+// product build compiles the marks to nothing): thread 0 of workgroup 0 reads the wall clock at
+// the phase boundaries of every iteration -- after its workgroup barriers, so the phase ends
+// for the whole workgroup -- and adds the durations to g_bottom_phases at the end of a launch
+// (index = BottomPhase; [30] iterations, [31] launches).
+enum BottomPhase {
+    PH_OP_V = 0,            // p update, operand tile, v = A p, staging of rhat . v
+    PH_A = 1,               // reduction A: chains, grid barrier, totals (1, 2, 3)
+    PH_OP_T = 4,            // v ghosts, s / x update, operand tile, t = A s, staging of t.s, t.t
+    PH_B = 5,               // reduction B (5, 6, 7)
+    PH_UPD_R = 8,           // x / r update, staging of rhat . r
+    PH_C = 9,               // reduction C (9, 10, 11)
+    PH_R_GHOST = 12,        // r ghosts
+    PH_INIT = 13,           // the launch's first reduction (13, 14, 15)
+    PH_WAIT = 16,           // per reduction: the staging wait before its chains (16 + reduction)
+    PH_NUM = 20
+};
+#ifdef SAYAKA_BOTTOM_PHASES
+__device__ unsigned long long g_bottom_phases[32];
+__device__ inline void bottom_phase_mark(unsigned long long* pht, int k) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long t = wall_clock64();
+        pht[1 + k] += t - pht[0];
+        pht[0] = t;
+    }
+}
+#define SY_PH_MARK(pht, k) bottom_phase_mark(pht, k)
+#else
+#define SY_PH_MARK(pht, k) ((void)(pht), (void)(k))
+#endif
+
 struct BottomArgs {
     StencilArgs st;        // list = level-1 blocks (tower 1), nlist of them, m = 1
     const double* rhs;     // RESID
@@ -204,7 +235,8 @@ struct Bottom {
     // partials and forms the totals (in block order when exact).  a0/a1 are the thread's
     // running sums in fast mode.  Both sides of the barrier run in wave 0 alone after one
     // workgroup barrier, so the other waves only wait once on each side.
-    __device__ static void reduce(unsigned long long& ep, int* broken, const BottomArgs& A, double a0, double a1,
+    __device__ static void reduce(unsigned long long* pht, int ph, unsigned long long& ep, int* broken,
+                                  const BottomArgs& A, double a0, double a1,
                                   int nsum, double mxl, double* part, double* red, double* wl, int nvalid,
                                   double& s0, double& s1, double& mx) {
         const int w0 = blockIdx.x * BPW;
@@ -221,6 +253,7 @@ struct Bottom {
             }
         }
         __syncthreads();
+        SY_PH_MARK(pht, PH_WAIT + (ph - 1) / 4);
         if (wv == 0) {
             if (A.exact) {
                 // ordered per-block chains, one lane per block and sum
@@ -245,7 +278,9 @@ struct Bottom {
                 part[PS * (w0 + lane) + 2] = m;
             }
         }
+        SY_PH_MARK(pht, ph);
         grid_barrier(A, ++ep, (nl + BPW - 1) / BPW, broken);
+        SY_PH_MARK(pht, ph + 1);
         // totals (wave 0): the partials staged in LDS by lane in chunks of 64, the max reduced
         // across the wave, the sums chained in block order in lanes 0 / 1 -- every workgroup
         // forms the same totals in the same order
@@ -273,6 +308,7 @@ struct Bottom {
         s1 = wl[49];
         mx = wl[50];
         __syncthreads();
+        SY_PH_MARK(pht, ph + 2);
     }
 };
 
@@ -286,6 +322,12 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
     unsigned long long ep = A.episode0;   // barrier episodes of this launch
     __shared__ int s_broken;              // a grid barrier of this workgroup timed out
     if (threadIdx.x == 0) s_broken = 0;   // visible after the first __syncthreads below
+#ifdef SAYAKA_BOTTOM_PHASES
+    __shared__ unsigned long long pht[1 + PH_NUM];   // [0] last mark, [1 + k] phase k
+    if ((int)threadIdx.x <= PH_NUM) pht[threadIdx.x] = threadIdx.x == 0 ? wall_clock64() : 0;
+#else
+    unsigned long long* pht = nullptr;
+#endif
     __shared__ __attribute__((aligned(16))) double sh[PC ? C::NSHARED_FAST : C::NSHARED];   // operand tiles, then reduction staging
     __shared__ double wl[64];
     __shared__ BlockInfo bi[BPW];
@@ -438,7 +480,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
         block_terms(r);
         B::publish_block_sums(bq, partC, wq, w0, nvalid);
     }
-    B::reduce(ep, &s_broken, A, a0, a1, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
+    B::reduce(pht, PH_INIT, ep, &s_broken, A, a0, a1, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
     if constexpr (PC) B::read_block_sums(partC, sr_, nl);
     const double rnorm0 = mx;
     double rho = s0v, rho1 = 0.0, alpha = 0.0, omega = 0.0;
@@ -556,7 +598,8 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 block_terms(v);
                 B::publish_block_sums(bq, partA, wq, w0, nvalid);
             }
-            B::reduce(ep, &s_broken, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx);
+            SY_PH_MARK(pht, PH_OP_V);
+            B::reduce(pht, PH_A, ep, &s_broken, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx);
             if constexpr (PC) B::read_block_sums(partA, sv_, nl);
             load_ghosts(vG, A.vpub);
             if (s0v == 0.0) { stat = 1; break; }
@@ -593,7 +636,8 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 B::stage(sh, ex, 1, blk_of(q), cell_of(q), t[q] * t[q], a1);
             }
             double ts, tt;
-            B::reduce(ep, &s_broken, A, a0, a1, 2, mxl, partB, sh, wl, nvalid, ts, tt, mx);
+            SY_PH_MARK(pht, PH_OP_T);
+            B::reduce(pht, PH_B, ep, &s_broken, A, a0, a1, 2, mxl, partB, sh, wl, nvalid, ts, tt, mx);
             if (mx < rnorm0 * A.tol_rel || mx < A.tol_abs) { conv = 1; break; }
             if (tt == 0.0) { stat = 1; break; }
             omega = ts / tt;
@@ -615,9 +659,11 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 block_terms(r);
                 B::publish_block_sums(bq, partC, wq, w0, nvalid);
             }
-            B::reduce(ep, &s_broken, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
+            SY_PH_MARK(pht, PH_UPD_R);
+            B::reduce(pht, PH_C, ep, &s_broken, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
             if constexpr (PC) B::read_block_sums(partC, sr_, nl);
             load_ghosts(rG, A.rpub);
+            SY_PH_MARK(pht, PH_R_GHOST);
             if (mx < rnorm0 * A.tol_rel + A.tol_abs) { conv = 1; break; }
             if (omega == 0.0) { stat = 1; break; }
             rho1 = rho;
@@ -630,6 +676,13 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
         for (int q = 0; q < CPT; q++)
             if (blk_of(q) < nvalid) A.sol[gidx(q)] = 0.0 + x[q];
     }
+#ifdef SAYAKA_BOTTOM_PHASES
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < PH_NUM; k++) g_bottom_phases[k] += pht[1 + k];
+        g_bottom_phases[30] += iter;
+        g_bottom_phases[31] += 1;
+    }
+#endif
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.stats[0] += 1;
         A.stats[1] += iter;

@@ -2473,3 +2473,20 @@ int sayaka_dev_get_velocity(sayaka_dev_ctx* c, double* out) {
 }
 
 }  // extern "C"
+
+#ifdef SAYAKA_BOTTOM_PHASES
+// Phase times of the coarsest-level solve since the last reset (the phase-timing build only):
+// out32[k] wall-clock ticks of phase k (sydev::BottomPhase), [30] iterations, [31] launches.
+extern "C" int sayaka_dev_bottom_phases(int device, unsigned long long* out32, int reset, int* clock_khz) {
+    if (!out32) return fail(-1, "bad argument");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpyFromSymbol(out32, HIP_SYMBOL(sydev::g_bottom_phases), 32 * sizeof(unsigned long long)));
+    if (clock_khz) HIPCHK(hipDeviceGetAttribute(clock_khz, hipDeviceAttributeWallClockRate, device));
+    if (reset) {
+        const unsigned long long z[32] = {};
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(sydev::g_bottom_phases), z, sizeof(z)));
+    }
+    return 0;
+}
+#endif

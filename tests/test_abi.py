@@ -19,7 +19,8 @@ def declared(header):
     return sorted(set(re.findall(r"\b(sayaka_dev_\w+|sy_tree_\w+|sy_mac_\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("sayaka_amd.h", N.DEV_LIB), ("sayaka_host.h", N.HOST_LIB)])
+@pytest.mark.parametrize("header,lib", [("sayaka_amd.h", N.DEV_LIB), ("sayaka_host.h", N.HOST_LIB),
+                                        ("sayaka_amd.h", N.DEV_PHASES_LIB), ("sayaka_phases.h", N.DEV_PHASES_LIB)])
 def test_exports(header, lib):
     names = declared(header)
     assert names, header
