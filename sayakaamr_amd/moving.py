@@ -14,8 +14,11 @@ Per step (the regrid / projection cycle of the reference's flow demo, Test7.h:41
      resampled on the new tree (sayaka_gen_velocity + sayaka_gen_divergence_rhs, walls at 0;
      the same seed every step, so the field only moves with the tree and the sphere), or
      Gaussian blobs (rhs="blobs", sayaka_gen_sphere_blobs);
-  5. the mixed-precision solve to 1e-10 |b|_inf from the carried pressure, and, for comparison,
-     from zero;
+  5. the mixed-precision solve to 1e-10 |b|_inf from the initial guess the policy picks, and,
+     for comparison, from zero.  guess="carried" starts from the carried pressure, "zero" from
+     zero, "auto" from the carried pressure only when its Linf residual is below the zero
+     guess's (|b|_inf) -- the fallback rule of the learned-guess configuration, with the carried
+     pressure as the candidate; the residual test is inside the timed solve;
   6. the judge: the true residual b - A x of the mixed solution under the FP64 operator (the
      reference's, bit for bit), and at the steps in fp64_steps the plain FP64 solve from zero
      beside it (its status, iterations, time and distance from the mixed solution);
@@ -48,9 +51,12 @@ def compose(first, second):
 
 
 def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=1000.0, compare_zero=True,
-        max_blocks=45000, passes=3, rhs="projection", seed=1, log=None, fp64_steps=(), fp64_max_iter=100):
+        max_blocks=45000, passes=3, rhs="projection", seed=1, log=None, fp64_steps=(), fp64_max_iter=100,
+        guess="auto"):
     if rhs not in ("projection", "blobs"):
         raise ValueError(f"rhs must be 'projection' or 'blobs', not {rhs!r}")
+    if guess not in ("carried", "zero", "auto"):
+        raise ValueError(f"guess must be 'carried', 'zero' or 'auto', not {guess!r}")
     log = log or (lambda m: print(f"[moving] {m}", file=sys.stderr, flush=True))
     tb = AmrTree(spec.ndim, max_blocks=max_blocks)
     tree = tb.build_shell(spec, (x0, 0.5, 0.5), radius, W.SHELL_FACTOR)
@@ -108,6 +114,15 @@ def run(steps=10, spec=W.C4_SPEC, radius=W.C4_RADIUS, x0=0.5, shift=0.01, ratio=
                 dev.copy(D.SOL, D.TMP)
             dev.sync()
             t5 = time.perf_counter()
+            used = "zero"
+            if step > 0 and guess != "zero":
+                dev.residual(L, D.R, D.SOL, D.RHS)
+                rc = dev.dot(L, D.R, D.R)[1]
+                row["carried_resid_rel"] = float(rc / bmax)
+                used = "carried" if guess == "carried" or rc < bmax else "zero"
+            if used == "zero":
+                dev.zero(D.SOL)
+            row["guess"] = used
             r = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=200, precision=D.MIXED)
             dev.sync()
             t6 = time.perf_counter()
@@ -166,6 +181,10 @@ def summarize(rows):
         out["step_ms"] += out["project_ms"]
         out["max_div_ratio"] = float(max(r["div_after"] / r["div_before"] for r in rows))
     out["steps"] = len(s)
+    out["carried_used_frac"] = float(np.mean([r.get("guess") == "carried" for r in s])) if s else 0.0
+    cr = [r["carried_resid_rel"] for r in s if "carried_resid_rel" in r]
+    if cr:
+        out["carried_resid_rel_median"] = float(np.median(cr))
     out["leaf_cells_mean"] = float(np.mean([r["leaf_cells"] for r in s]))
     out["max_rel_resid"] = float(max(r["rel_resid"] for r in s))
     out["max_judge_rel_resid"] = float(max(r["judge_rel_resid"] for r in rows))
@@ -182,8 +201,10 @@ def main(argv=None):
     ap.add_argument("--no-zero", action="store_true", help="skip the zero-guess comparison solve")
     ap.add_argument("--rhs", choices=("projection", "blobs"), default="projection")
     ap.add_argument("--fp64-steps", type=int, nargs="*", default=(), help="steps with the plain FP64 solve beside")
+    ap.add_argument("--guess", choices=("carried", "zero", "auto"), default="auto")
     args = ap.parse_args(argv)
-    rows = run(steps=args.steps, compare_zero=not args.no_zero, rhs=args.rhs, fp64_steps=tuple(args.fp64_steps))
+    rows = run(steps=args.steps, compare_zero=not args.no_zero, rhs=args.rhs, fp64_steps=tuple(args.fp64_steps),
+               guess=args.guess)
     print(json.dumps(summarize(rows)))
 
 

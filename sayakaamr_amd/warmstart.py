@@ -16,10 +16,19 @@ zero volume-weighted mean over the fluid.  A guess is used only when its initial
 Every solve, with or without the guess, runs to the same absolute target 1e-10 * |b|_inf.
 
     python -m sayakaamr_amd.warmstart train --n-train 2000 --n-test 200 --epochs 20 --out params.npz
+    python -m sayakaamr_amd.warmstart study --n-test 200 --out study.json
+
+`study` measures what any initial guess can gain in front of this solver, on the held-out
+problems: the outer iterations from zero, from one V-cycle of the rhs (x1, the preconditioner's
+own first step, about 1 % from the solution), from x1 plus exact parts of its remaining error
+e1 = x* - x1 (the per-block means of e1; the per-block mean-free part of e1, which is what a
+per-block network can see; 90 % of e1), and from the trained network's guess, forced and with
+the fallback.  x* is the mixed solve taken to 1e-13 |b|_inf.
 """
 from __future__ import annotations
 
 import argparse
+import json
 import os
 import sys
 import time
@@ -156,6 +165,54 @@ def evaluate(dev, seeds, precision=D.MIXED):
     return out
 
 
+def study_problem(dev, pb):
+    """Iterations (and initial residual / |b|_inf) of the mixed solve to 1e-10 |b|_inf from the
+    guesses of the module docstring, for one problem set up on dev."""
+    tol = 1e-10 * pb.bmax
+    leaf = pb.tree["node_type"] == 1
+    fl = leaf[:, None] & (dev.get_vf() > 0)
+
+    def run(g):
+        dev.upload(D.SOL, g)
+        r0 = pb.residual_inf(D.SOL) / pb.bmax
+        st = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=400, precision=D.MIXED)
+        return dict(iters=int(st["iters"]), status=int(st["status"]), r0=float(r0))
+
+    dev.zero(D.SOL)
+    dev.solve(tol_rel=0.0, tol_abs=1e-3 * tol, max_iter=60, precision=D.MIXED)
+    xs = dev.download(D.SOL)
+    dev.zero(D.X)
+    dev.precond(pb.L, D.X, D.RHS)
+    x1 = dev.download(D.X)
+    e1 = np.where(fl, xs - x1, 0.0)
+    cnt = np.maximum(fl.sum(1), 1)
+    bmean = np.where(fl, (e1.sum(1) / cnt)[:, None], 0.0)
+    out = dict(zero=run(np.zeros_like(xs)), x1=run(x1))
+    out["x1+block_means"] = run(x1 + bmean)
+    out["x1+block_detail"] = run(x1 + (e1 - bmean))
+    out["x1+0.9e1"] = run(x1 + 0.9 * e1)
+    dev.cnn_guess(D.SOL, 1.0 / pb.bmax, pb.bmax * LEN2)
+    g = dev.download(D.SOL)
+    out["net"] = run(g)
+    out["net_fallback"] = out["net"] if out["net"]["r0"] < 1.0 else dict(out["zero"])
+    return out
+
+
+def study(dev, seeds):
+    rows = []
+    for q, seed in enumerate(seeds):
+        pb = Problem(dev, sample(np.random.default_rng(seed)))
+        rows.append(study_problem(dev, pb))
+        if (q + 1) % 20 == 0:
+            log(f"study {q + 1}/{len(seeds)}")
+    keys = list(rows[0])
+    return dict(n=len(rows),
+                mean_iters={k: float(np.mean([r[k]["iters"] for r in rows])) for k in keys},
+                median_initial_resid_rel={k: float(np.median([r[k]["r0"] for r in rows])) for k in keys},
+                all_converged=bool(all(r[k]["status"] == 0 for r in rows for k in keys)),
+                net_used_frac=float(np.mean([r["net"]["r0"] < 1.0 for r in rows])))
+
+
 def summarize(res):
     zi = np.array([r["zero_iters"] for r in res])
     gi = np.array([r["guess_iters"] for r in res])
@@ -170,7 +227,7 @@ def summarize(res):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "study"])
     ap.add_argument("--n-train", type=int, default=2000)
     ap.add_argument("--n-test", type=int, default=200)
     ap.add_argument("--per-solve", type=int, default=64)
@@ -178,11 +235,21 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--params", default=DEFAULT_PARAMS)
-    ap.add_argument("--out", default=DEFAULT_PARAMS)
+    ap.add_argument("--out", default=None, help="trained parameters (train) / JSON record (eval, study)")
     args = ap.parse_args(argv)
     dev = make_device()
     test_seeds = list(range(1_000_000, 1_000_000 + args.n_test))
+    if args.cmd == "study":
+        dev.cnn_set_params(np.load(args.params)["params"])
+        res = study(dev, test_seeds)
+        log(f"study {json.dumps(res)}")
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(res, f, indent=1)
+        dev.close()
+        return res
     if args.cmd == "train":
+        args.out = args.out or DEFAULT_PARAMS
         dev.cnn_set_params(init_params(0))
         dev.cnn_store_reset(args.n_train * args.per_solve)
         nwin = generate(dev, list(range(args.n_train)), args.per_solve, np.random.default_rng(1))
@@ -195,8 +262,13 @@ def main(argv=None):
     else:
         dev.cnn_set_params(np.load(args.params)["params"])
     res = evaluate(dev, test_seeds)
-    log(f"held-out {summarize(res)}")
+    summ = summarize(res)
+    log(f"held-out {summ}")
+    if args.cmd == "eval" and args.out:
+        with open(args.out, "w") as f:
+            json.dump(summ, f, indent=1)
     dev.close()
+    return summ
 
 
 if __name__ == "__main__":

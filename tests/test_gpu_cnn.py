@@ -168,3 +168,25 @@ def test_learned_guess_solve_reaches_target():
         assert pb.residual_inf(D.SOL) <= 2e-10 * pb.bmax
     finally:
         dev.close()
+
+
+def test_guess_study_on_held_out_problems():
+    """The trained network and the exact-correction bounds on three held-out problems
+    (warmstart.study_problem): every solve reaches its target; the fallback never costs
+    iterations against the zero guess (deterministic solves of the same system); one V-cycle of
+    the rhs as the guess saves iterations; an exact per-block mean-free correction of that
+    guess -- the most a per-block network could add to it -- saves at most one more."""
+    dev = WS.make_device(max_blocks=12000, windows=4096)
+    try:
+        dev.cnn_set_params(np.load(WS.DEFAULT_PARAMS)["params"])
+        for seed in (1_000_000, 1_000_001, 1_000_002):
+            pb = WS.Problem(dev, WS.sample(np.random.default_rng(seed)))
+            r = WS.study_problem(dev, pb)
+            print(f"[gpu] guess study seed {seed}: " +
+                  ", ".join(f"{k} {v['iters']} ({v['r0']:.3g})" for k, v in r.items()), flush=True)
+            assert all(v["status"] == 0 for v in r.values())
+            assert r["net_fallback"]["iters"] <= r["zero"]["iters"]
+            assert r["x1"]["iters"] < r["zero"]["iters"]
+            assert r["x1+block_detail"]["iters"] >= r["x1"]["iters"] - 1
+    finally:
+        dev.close()

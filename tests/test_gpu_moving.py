@@ -32,6 +32,13 @@ def test_moving_cycle_converges_every_step(rhs):
             # the corrected velocity is divergence-free to the solve's residual
             assert r["div_after"] <= 1e-8 * r["div_before"]
     assert rows[1]["blocks"] > 0 and "zero_iters" in rows[1]
+    # the initial-guess fallback: the carried pressure is used only when its Linf residual is
+    # below the zero guess's; when it is not, the solve is the zero-guess solve, iteration for
+    # iteration (deterministic kernels, same rhs)
+    for r in rows[1:]:
+        assert r["guess"] == ("carried" if r["carried_resid_rel"] < 1.0 else "zero")
+        if r["guess"] == "zero":
+            assert r["iters"] == r["zero_iters"]
 
 
 def test_carried_pressure_is_the_remapped_solution():
@@ -73,4 +80,4 @@ def test_moving_cycle_at_large_block_shape():
         assert r["leaf_cells"] > 100_000_000
     r1 = rows[1]
     assert r1["fp64_status"] == 0 and r1["fp64_rel_resid"] <= 1e-10
-    assert r1["mixed_rel_l2_vs_fp64"] < 1e-2
+    assert r1["mixed_rel_l2_vs_fp64"] < 1e-6   # fluid cells, each solution less its mean
