@@ -20,6 +20,7 @@ namespace sydev {
 namespace cnn {
 
 constexpr double VF_MIN = 0.02;
+constexpr int MAXSUB = 8;   // samples per cell edge the generator supports
 constexpr int MAXBLOB = 16;
 
 struct SphereArgs {
@@ -49,9 +50,19 @@ __device__ inline int sphere_box_side(const SphereArgs& a, const double* lo, con
     return 0;
 }
 
+// Sample offsets (s + 1/2) / nsub within a cell edge, s < nsub: computed once per thread, the
+// same divisions the sampling loops would do.
+struct SubOffsets {
+    double o[MAXSUB];
+    __device__ explicit SubOffsets(int ns) {
+        for (int q = 0; q < MAXSUB; q++) o[q] = q < ns ? (q + 0.5) / ns : 0.0;
+    }
+};
+
 // fluid fraction of the cell (i, j, k) of a block with lower corner lo3 (indices may lie one
 // cell outside the block); sample points lo + (i + (s + 1/2) / nsub) * dx
-__device__ inline double sphere_cell_fluid(const SphereArgs& a, const double* lo3, int i, int j, int k, double dx) {
+__device__ inline double sphere_cell_fluid(const SphereArgs& a, const SubOffsets& so, const double* lo3, int i, int j,
+                                           int k, double dx) {
     {
         const double clo[3] = {lo3[0] + i * dx, lo3[1] + j * dx, lo3[2] + k * dx}, cext[3] = {dx, dx, dx};
         const int side = sphere_box_side(a, clo, cext);
@@ -63,7 +74,7 @@ __device__ inline double sphere_cell_fluid(const SphereArgs& a, const double* lo
     for (int sz = 0; sz < ns; sz++)
         for (int sy = 0; sy < ns; sy++)
             for (int sx = 0; sx < ns; sx++) {
-                const double ox = (sx + 0.5) / ns, oy = (sy + 0.5) / ns, oz = (sz + 0.5) / ns;
+                const double ox = so.o[sx], oy = so.o[sy], oz = so.o[sz];
                 const double x = lo3[0] + (i + ox) * dx, y = lo3[1] + (j + oy) * dx, z = lo3[2] + (k + oz) * dx;
                 const double r2 = (x - a.c[0]) * (x - a.c[0]) + (y - a.c[1]) * (y - a.c[1]) + (z - a.c[2]) * (z - a.c[2]);
                 out += r2 > a.r * a.r;
@@ -78,6 +89,19 @@ __device__ inline double heaviside(double s, double eps) {
     return 0.5 * (1.0 + s / eps + sin(M_PI * s / eps) / M_PI);
 }
 
+// B = 1 / rho at a face centre at squared distance d2 from the sphere's centre, rho =
+// 1 + (ratio - 1) H_eps(r - |x - c|).  Outside the band r - eps < |x - c| < r + eps, widened by
+// a 1e-12 relative margin on d2 (far beyond the rounding of the distance), H is exactly 0 or 1
+// and rho exactly 1 or 1 + (ratio - 1) * 1: such faces need neither the square root nor the
+// division (b_in is that division, done once), and get the same bits.
+__device__ inline double face_b(const SphereArgs& a, double d2, double b_in) {
+    const double ro = a.r + a.eps, ri = a.r - a.eps;
+    if (d2 > ro * ro * (1.0 + 1e-12)) return 1.0;
+    if (ri > 0.0 && d2 < ri * ri * (1.0 - 1e-12)) return b_in;
+    const double rho = 1.0 + (a.ratio - 1.0) * heaviside(-(sqrt(d2) - a.r), a.eps);
+    return 1.0 / rho;
+}
+
 // vf [blk][512]; af, b [blk][3][NFACE]; one workgroup per block
 template <int NB>
 __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __restrict__ vf, double* __restrict__ af,
@@ -86,11 +110,45 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
     const int b = blockIdx.x;
     const double lo3[3] = {a.lo[3 * b], a.lo[3 * b + 1], a.lo[3 * b + 2]};
     const double dx = a.dx[b];
-    for (int c = threadIdx.x; c < G::NCELL; c += blockDim.x) {
-        int i, j, k;
-        G::cell_ijk(c, i, j, k);
-        vf[b * G::NCELL + c] = sphere_cell_fluid(a, lo3, i, j, k, dx);
+    const double b_in = 1.0 / (1.0 + (a.ratio - 1.0) * 1.0);
+    // The block with one cell around it (the cells a boundary face's closure test reads) wholly
+    // outside the sphere and its density band, or wholly inside both, by the margins of
+    // sphere_box_side and face_b: every sample point, cell and face lands on that side, so the
+    // block's values are the constants the per-point code would produce.
+    {
+        double dmin = 0.0, dmax = 0.0;
+        for (int e = 0; e < 3; e++) {
+            const double l = lo3[e] - dx - a.c[e], h = lo3[e] + (NB + 1) * dx - a.c[e];
+            const double near = l > 0.0 ? l : (h < 0.0 ? -h : 0.0);
+            const double far = fmax(fabs(l), fabs(h));
+            dmin += near * near;
+            dmax += far * far;
+        }
+        const double ro = a.r + a.eps, ri = a.r - a.eps;
+        const bool out = dmin > a.r * a.r * (1.0 + 1e-10) && dmin > ro * ro * (1.0 + 1e-12);
+        const bool in = ri > 0.0 && dmax < a.r * a.r * (1.0 - 1e-10) && dmax < ri * ri * (1.0 - 1e-12);
+        if (out || in) {
+            for (int c = threadIdx.x; c < G::NCELL; c += blockDim.x) vf[b * G::NCELL + c] = out ? 1.0 : 0.0;
+            for (int q = threadIdx.x; q < 3 * G::NFACE; q += blockDim.x) {
+                af[b * 3 * G::NFACE + q] = out ? 1.0 : 0.0;
+                bco[b * 3 * G::NFACE + q] = out ? 1.0 : b_in;
+            }
+            return;
+        }
     }
+    // fractions of the block's cells and of the one-cell layer around it, once each: the
+    // closure test of a face reads its two cells from here
+    constexpr int E = NB + 2;
+    __shared__ double vloc[E * E * E];
+    const SubOffsets so(a.nsub);
+    for (int q = threadIdx.x; q < E * E * E; q += blockDim.x) {
+        const int i = q % E - 1, j = (q / E) % E - 1, k = q / (E * E) - 1;
+        const double v = sphere_cell_fluid(a, so, lo3, i, j, k, dx);
+        vloc[q] = v;
+        if (i >= 0 && i < NB && j >= 0 && j < NB && k >= 0 && k < NB) vf[b * G::NCELL + G::cidx(i, j, k)] = v;
+    }
+    __syncthreads();
+    auto vat = [&](int i, int j, int k) { return vloc[(i + 1) + E * ((j + 1) + E * (k + 1))]; };
     for (int q = threadIdx.x; q < 3 * G::NFACE; q += blockDim.x) {
         const int d = q / G::NFACE, fi = q % G::NFACE;
         const int fx = G::fx(d), fy = G::fy(d);
@@ -98,10 +156,9 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
         // face centre: the face index along d, cell centres across
         double ctr[3];
         for (int e = 0; e < 3; e++) ctr[e] = lo3[e] + (idx[e] + (e == d ? 0.0 : 0.5)) * dx;
-        const double rr = sqrt((ctr[0] - a.c[0]) * (ctr[0] - a.c[0]) + (ctr[1] - a.c[1]) * (ctr[1] - a.c[1]) +
-                               (ctr[2] - a.c[2]) * (ctr[2] - a.c[2]));
-        const double rho = 1.0 + (a.ratio - 1.0) * heaviside(-(rr - a.r), a.eps);
-        bco[(b * 3 + d) * G::NFACE + fi] = 1.0 / rho;
+        const double d2 = (ctr[0] - a.c[0]) * (ctr[0] - a.c[0]) + (ctr[1] - a.c[1]) * (ctr[1] - a.c[1]) +
+                          (ctr[2] - a.c[2]) * (ctr[2] - a.c[2]);
+        bco[(b * 3 + d) * G::NFACE + fi] = face_b(a, d2, b_in);
         // open fraction of the face
         const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
         double flo[3], fext[3] = {dx, dx, dx};
@@ -112,8 +169,8 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
         for (int p2 = 0; p2 < (fside == 0 ? a.nsub : 0); p2++)
             for (int p1 = 0; p1 < a.nsub; p1++) {
                 double o[3] = {0.0, 0.0, 0.0};
-                o[t1] = (p1 + 0.5) / a.nsub;
-                o[t2] = (p2 + 0.5) / a.nsub;
+                o[t1] = so.o[p1];
+                o[t2] = so.o[p2];
                 double pt[3];
                 for (int e = 0; e < 3; e++) pt[e] = lo3[e] + (idx[e] + o[e]) * dx;
                 out += (pt[0] - a.c[0]) * (pt[0] - a.c[0]) + (pt[1] - a.c[1]) * (pt[1] - a.c[1]) +
@@ -121,8 +178,7 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
             }
         int lc[3] = {idx[0], idx[1], idx[2]};
         lc[d] -= 1;
-        const bool closed = sphere_cell_fluid(a, lo3, lc[0], lc[1], lc[2], dx) == 0.0 ||
-                            sphere_cell_fluid(a, lo3, idx[0], idx[1], idx[2], dx) == 0.0;
+        const bool closed = vat(lc[0], lc[1], lc[2]) == 0.0 || vat(idx[0], idx[1], idx[2]) == 0.0;
         af[(b * 3 + d) * G::NFACE + fi] = closed ? 0.0 : (double)out / (a.nsub * a.nsub);
     }
 }

@@ -146,6 +146,10 @@ struct sayaka_dev_ctx {
     std::vector<ProfRec> prof[SAYAKA_KCLASS_N];
     std::vector<hipEvent_t> ev_pool;
     int coef_version = 0;
+    // set_topology leaves the coefficients unit (the reference's prepareSolver) without
+    // computing them: the first entry point that reads them fills and derives them
+    // (ensure_coefs), unless set_coefs or the generators write their own first
+    int coef_pending = 0;
     int exact = 1;  // reference-order reductions (FP64 solves); the mixed solve clears it
     // two-level preconditioner of the mixed solve's coarsest problem (bottom.h)
     int* d_l1pos = nullptr;        // block id -> level-1 list position, -1 elsewhere
@@ -1127,8 +1131,12 @@ struct Impl {
     }
 };
 
+int ensure_coefs(sayaka_dev_ctx* c);
+
 template <class F>
 int dispatch(sayaka_dev_ctx* c, F&& f) {
+    if (c->coef_pending)
+        if (int e = ensure_coefs(c)) return e;
     if (c->ndim == 3 && c->nb == 8) return f(Impl<3, 8>{});
     if (c->ndim == 2 && c->nb == 8) return f(Impl<2, 8>{});
     if (c->ndim == 3 && c->nb == 16) return f(Impl<3, 16>{});
@@ -1138,6 +1146,24 @@ int dispatch(sayaka_dev_ctx* c, F&& f) {
 }
 
 bool field_ok(int f) { return f >= 0 && f < SAYAKA_NFIELD; }
+
+// Unit coefficients (vf = a_f = B = 1) and the weights derived from them for the current
+// topology and BCs: what set_topology left pending.
+int ensure_coefs(sayaka_dev_ctx* c) {
+    c->coef_pending = 0;
+    const size_t cells = (size_t)c->nblk * c->ncell;
+    const size_t faces = (size_t)c->nblk * c->ndim * c->nface;
+    HIPCHK(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->d_vf, cells, 1.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->d_b, faces, 1.0);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->d_af, faces, 1.0);
+    HIPCHK(hipGetLastError());
+    int e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
+    if (e) return e;
+    c->coef_version++;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
 
 }  // namespace
 
@@ -1364,10 +1390,14 @@ int sayaka_dev_set_topology(sayaka_dev_ctx* c, int nblk, int minlevel, int maxle
     }
     c->vel_valid = 0;  // the face velocity belongs to the previous tree
     c->xfer[2]++;
-    // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32)
+    // prepareSolver: unit coefficients (SayakaMacCoef.cpp:8-18, SayakaMacIB.cpp:16-32), filled
+    // and turned into weights by the first entry point that reads them (ensure_coefs) unless
+    // the caller sets its own first
     c->alpha = 0.0;
     c->beta = 1.0;
-    return sayaka_dev_set_coefs(c, 0.0, 1.0, nullptr, nullptr, nullptr);
+    c->coef_pending = 1;
+    c->coef_version++;
+    return 0;
 }
 
 int sayaka_dev_set_bc(sayaka_dev_ctx* c, const int* bc6) {
@@ -1376,6 +1406,11 @@ int sayaka_dev_set_bc(sayaka_dev_ctx* c, const int* bc6) {
     for (int b = 0; b < c->nblk; b++)
         for (int f = 0; f < c->nf; f++) c->h_binfo[b].bct[f] = c->h_binfo[b].nbr[f] <= PHYS_BNDRY ? bc6[b * 6 + f] : -1;
     HIPCHK(hipMemcpy(c->d_binfo, c->h_binfo.data(), c->nblk * sizeof(BlockInfo), hipMemcpyHostToDevice));
+    c->pc_version = -1;
+    if (c->coef_pending) {   // the weights are derived later, with these BCs
+        c->coef_version++;
+        return 0;
+    }
     // Neumann / Dirichlet change the wall weights
     int e = dispatch(c, [&](auto I) { return decltype(I)::set_coefs(c); });
     c->coef_version++;
@@ -1387,6 +1422,7 @@ int sayaka_dev_set_coefs(sayaka_dev_ctx* c, double alpha, double beta, const dou
     if (!c) return fail(-1, "null ctx");
     if (alpha != 0.0) return fail(-1, "alpha must be 0 (the reference asserts it, SayakaMacOperator.cpp:1268)");
     HIPCHK(hipSetDevice(c->device));
+    c->coef_pending = 0;   // every coefficient array is written below
     c->alpha = alpha;
     c->beta = beta;
     const size_t cells = (size_t)c->nblk * c->ncell;
@@ -1423,6 +1459,8 @@ int sayaka_dev_set_coefs(sayaka_dev_ctx* c, double alpha, double beta, const dou
 
 int sayaka_dev_get_coefs(sayaka_dev_ctx* c, double* vf, double* const* af, double* const* b, double* w, double* wcf) {
     if (!c) return fail(-1, "null ctx");
+    if (c->coef_pending)
+        if (int e = ensure_coefs(c)) return e;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     const size_t cells = (size_t)c->nblk * c->ncell;
@@ -2219,6 +2257,8 @@ int sayaka_cnn_store_reset(sayaka_dev_ctx* c, long long cap) {
 long long sayaka_cnn_collect(sayaka_dev_ctx* c, const int* leaf_ids, int n, double s_rhs, double shift,
                              double t_scale) {
     if (!c || !leaf_ids || n < 1 || !c->cnn.store) return fail(-1, "bad argument");
+    if (c->coef_pending)
+        if (int e = ensure_coefs(c)) return e;
     auto& N = c->cnn;
     if (N.store_n + n > N.store_cap) return fail(-1, "training store full");
     if (n > N.cap) return fail(-1, "more windows than the capacity");
@@ -2337,7 +2377,9 @@ static int gen_sphere_coefs(sayaka_dev_ctx* c, const double* lo3, const double* 
     int e = gen_scratch(c, lo3, dx);
     if (e) return e;
     const int n = c->nblk;
+    if (nsub < 1 || nsub > cnn::MAXSUB) return fail(-1, "nsub must be 1 .. 8");
     cnn::SphereArgs sa{c->d_gen, c->d_gen + 3 * n, {centre[0], centre[1], centre[2]}, radius, ratio, eps, nsub};
+    c->coef_pending = 0;   // the generator writes vf, a_f and B of every block
     if (c->nb == 8)
         hipLaunchKernelGGL(cnn::k_gen_sphere<8>, dim3(n), dim3(256), 0, c->stream, sa, c->d_vf, c->d_af, c->d_b);
     else

@@ -253,7 +253,9 @@ void AmrTree::checkCoarsen() {
         if (coarsenFlag[i] && !blocks[i].isLeaf()) coarsenFlag[i] = 0;
     for (int i = 0; i < numBlocks; i++) nonleaf[i] = blocks[i].hasChildren() ? 1 : 0;
     auto spread = [&](const std::vector<int>& src, std::vector<int>& dst) {
-        for (int i = 0; i < numBlocks; i++)
+        const int n = numBlocks;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++)
             for (int f = 0; f < numFace(); f++) {
                 const int n = blocks[i].neighbor[f];
                 if (n >= 0 && src[n]) { dst[i] = 1; break; }
@@ -284,8 +286,13 @@ int AmrTree::refineBlocks() {
     std::vector<int> t1(M, 0), t2(M, 0), t3(M, 0);
     std::fill(newChildFlag.begin(), newChildFlag.end(), 0);
     const int NF = numFace(), NC = numChild();
+    // The per-block passes below write only block i's own entries from values that earlier
+    // passes wrote: they run in parallel with the sequential result.  Child creation and
+    // linking (ids handed out in order, links written across blocks) stay sequential.
     for (int cycle = 0; cycle < 99999; cycle++) {
-        for (int i = 0; i < numBlocks; i++)
+        const int nb0 = numBlocks;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nb0; i++)
             if (refineFlag[i] && !blocks[i].isLeaf()) refineFlag[i] = 0;
 
         // children take the next free ids, parent by parent, child index order
@@ -343,7 +350,9 @@ int AmrTree::refineBlocks() {
 
         // face marks: 2 on every face of a parent, 3 on face (d, 1-side_d(c)) of a parent
         // whose child c has children of its own (SayakaRegrid.cpp:350-382)
-        for (int i = 0; i < numBlocks; i++) {
+        const int nb1 = numBlocks;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nb1; i++) {
             int* t2f = blocks[i].nodeType2;
             for (int f = 0; f < NF; f++) t2f[f] = 1;
             for (int c = 0; c < NC; c++) {
@@ -363,7 +372,8 @@ int AmrTree::refineBlocks() {
         std::fill(t3.begin(), t3.end(), 0);
         std::fill(refineFlag.begin(), refineFlag.end(), 0);
         // rule 1: a face neighbour whose mark on that same face index is 3
-        for (int i = 0; i < numBlocks; i++)
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nb1; i++)
             for (int f = 0; f < NF; f++) {
                 const int n = blocks[i].neighbor[f];
                 if (n >= 0 && blocks[n].nodeType2[f] == 3) { t1[i] = 1; break; }
@@ -378,26 +388,30 @@ int AmrTree::refineBlocks() {
             }
         };
         if (ndim >= 2) {
-            for (int i = 0; i < numBlocks; i++) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < nb1; i++) {
                 int c[3];
                 dirs_hit(i, t1, c);
                 if ((c[0] >= 1 && c[1] >= 1) || (c[0] >= 1 && c[2] >= 1) || (c[1] >= 1 && c[2] >= 1)) t2[i] = 1;
             }
-            for (int i = 0; i < numBlocks; i++)
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < nb1; i++)
                 if (t2[i] || t1[i]) t2[i] = 1;
         }
         if (ndim == 3) {
-            for (int i = 0; i < numBlocks; i++) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < nb1; i++) {
                 int c[3];
                 dirs_hit(i, t2, c);
                 if (c[0] >= 1 && c[1] >= 1 && c[2] >= 1) t3[i] = 1;
             }
         }
-        bool repeat = false;
-        for (int i = 0; i < numBlocks; i++) {
+        int repeat = 0;
+#pragma omp parallel for schedule(static) reduction(| : repeat)
+        for (int i = 0; i < nb1; i++) {
             refineFlag[i] = 0;
             if (blocks[i].isLeaf() && (t1[i] || t2[i] || t3[i])) {
-                repeat = true;
+                repeat = 1;
                 refineFlag[i] = 1;
             }
         }
@@ -458,7 +472,9 @@ int AmrTree::coarsenBlocks(int oldBlockNum) {
 
 // Leaf / parent of leaves / parent of parents (SayakaRegrid.cpp:634-653).
 void AmrTree::setNodeType() {
-    for (int i = 0; i < numBlocks; i++) {
+    const int n = numBlocks;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
         AmrTreeNode& b = blocks[i];
         b.nodeType = NodeNoLeafChild;
         for (int c = 0; c < numChild(); c++) {
