@@ -16,9 +16,11 @@ namespace sydev {
 // Fine blocks' FINE_CRSE faces -> parent's boundary face sub-box, average of the
 // 2^(d-1) fine faces (restrictFaceBlockData_Average, SayakaInterpPatch.cpp:627-691).
 template <int D, int NB>
-__global__ void k_sync_fine(const int* list, const BlockInfo* binfo, double* fdata, int d) {
+__global__ void k_sync_fine(const int* list, const BlockInfo* binfo, double* f0, double* f1) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
+    const int d = blockIdx.y % D;                          // direction; array f0 or f1
+    double* fdata = (int)blockIdx.y < D ? f0 : f1;
     __shared__ BlockInfo bi;
     if (threadIdx.x == 0) bi = binfo[b];
     __syncthreads();
@@ -53,9 +55,11 @@ __global__ void k_sync_fine(const int* list, const BlockInfo* binfo, double* fda
 
 // Coarse leaves' CRSE_FINE faces <- the neighbour's opposite face slab (SayakaTower.cpp:388-413).
 template <int D, int NB>
-__global__ void k_sync_crse(const int* list, const BlockInfo* binfo, double* fdata, int d, int m) {
+__global__ void k_sync_crse(const int* list, const BlockInfo* binfo, double* f0, double* f1, int m) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
+    const int d = blockIdx.y % D;
+    double* fdata = (int)blockIdx.y < D ? f0 : f1;
     __shared__ BlockInfo bi;
     if (threadIdx.x == 0) bi = binfo[b];
     __syncthreads();
@@ -82,9 +86,11 @@ __global__ void k_sync_crse(const int* list, const BlockInfo* binfo, double* fda
 // (FillPatch::restrictBlock, SayakaFillPatch.cpp:420-447).  Children are processed in index
 // order there, so on the middle plane the child on the high side wins.
 template <int D, int NB>
-__global__ void k_restrict_face(const int* list, const BlockInfo* binfo, double* fdata, int d) {
+__global__ void k_restrict_face(const int* list, const BlockInfo* binfo, double* f0, double* f1) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
+    const int d = blockIdx.y % D;
+    double* fdata = (int)blockIdx.y < D ? f0 : f1;
     const int h = NB / 2;
     const double scale = 1.0 / ((2 * 2 * (D == 3 ? 2 : 1)) / 2);
     int a1, a2;

@@ -745,30 +745,31 @@ struct Impl {
     // coarse/fine synchronisation of one face field [blk][D][nface] on the finest tower (the
     // coarse side of every coarse/fine face gets the average of the fine faces), then the faces
     // of non-leaf blocks averaged from their children, coarsest last (coef.h)
-    static void sync_faces(sayaka_dev_ctx* c, double* fd) {
+    // Coarse/fine face sync and average-down of one or two face arrays (fd1 may be null).
+    // Directions and arrays are independent, so each launch covers all of them (grid y =
+    // array * D + direction); the levels keep their order.
+    static void sync_faces(sayaka_dev_ctx* c, double* fd0, double* fd1 = nullptr) {
         const int L = c->finest;
-        for (int d = 0; d < D; d++) {
-            for (int l = L; l >= 2; l--) {
-                ListRef fine = c->tw_sub[L][l], crse = c->tw_sub[L][l - 1];
-                if (c->tw[L].n == 0 || fine.n <= 0 || crse.n <= 0) continue;
-                hipLaunchKernelGGL((k_sync_fine<D, NB>), dim3(fine.n), dim3(64), 0, c->stream, dlist(c, fine),
-                                   c->d_binfo, fd, d);
-                hipLaunchKernelGGL((k_sync_crse<D, NB>), dim3(crse.n), dim3(64), 0, c->stream, dlist(c, crse),
-                                   c->d_binfo, fd, d, L);
-            }
-            for (int l = L - 1; l >= 1; l--) {
-                ListRef nl = c->lv_nonleaf[l];
-                if (nl.n <= 0) continue;
-                hipLaunchKernelGGL((k_restrict_face<D, NB>), dim3(nl.n), dim3(256), 0, c->stream, dlist(c, nl),
-                                   c->d_binfo, fd, d);
-            }
+        const dim3 gy(1, fd1 ? 2 * D : D);
+        for (int l = L; l >= 2; l--) {
+            ListRef fine = c->tw_sub[L][l], crse = c->tw_sub[L][l - 1];
+            if (c->tw[L].n == 0 || fine.n <= 0 || crse.n <= 0) continue;
+            hipLaunchKernelGGL((k_sync_fine<D, NB>), dim3(fine.n, gy.y), dim3(64), 0, c->stream, dlist(c, fine),
+                               c->d_binfo, fd0, fd1);
+            hipLaunchKernelGGL((k_sync_crse<D, NB>), dim3(crse.n, gy.y), dim3(64), 0, c->stream, dlist(c, crse),
+                               c->d_binfo, fd0, fd1, L);
+        }
+        for (int l = L - 1; l >= 1; l--) {
+            ListRef nl = c->lv_nonleaf[l];
+            if (nl.n <= 0) continue;
+            hipLaunchKernelGGL((k_restrict_face<D, NB>), dim3(nl.n, gy.y), dim3(256), 0, c->stream, dlist(c, nl),
+                               c->d_binfo, fd0, fd1);
         }
     }
 
     static int set_coefs(sayaka_dev_ctx* c) {
         const int L = c->finest;
-        sync_faces(c, c->d_b);
-        sync_faces(c, c->d_af);
+        sync_faces(c, c->d_b, c->d_af);
         for (int l = L - 1; l >= 1; l--) {
             ListRef nl = c->lv_nonleaf[l];
             if (nl.n <= 0) continue;
