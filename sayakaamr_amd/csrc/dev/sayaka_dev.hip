@@ -1134,6 +1134,18 @@ struct Impl {
 
 int ensure_coefs(sayaka_dev_ctx* c);
 
+// The (dimension, block size) instance of an entry point that reads no coefficient.
+template <class F>
+int dispatch_nc(sayaka_dev_ctx* c, F&& f) {
+    if (c->ndim == 3 && c->nb == 8) return f(Impl<3, 8>{});
+    if (c->ndim == 2 && c->nb == 8) return f(Impl<2, 8>{});
+    if (c->ndim == 3 && c->nb == 16) return f(Impl<3, 16>{});
+    if (c->ndim == 2 && c->nb == 16) return f(Impl<2, 16>{});
+    if (c->ndim == 3 && c->nb == 4) return f(Impl<3, 4>{});
+    return fail(-2, "unsupported (ndim, nb)");
+}
+
+// ... of an entry point that may read them: unit coefficients left pending are derived first.
 template <class F>
 int dispatch(sayaka_dev_ctx* c, F&& f) {
     if (c->coef_pending)
@@ -1714,7 +1726,7 @@ int sayaka_dev_state_fill(sayaka_dev_ctx* c, double* state, int ncomp, int block
     a.nc = nc;
     a.nphys = nphys;
     for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
-    int e = dispatch(c, [&](auto I) { return decltype(I)::state_fill(c, a, lds); });
+    int e = dispatch_nc(c, [&](auto I) { return decltype(I)::state_fill(c, a, lds); });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1760,7 +1772,7 @@ int sayaka_dev_face_fill(sayaka_dev_ctx* c, double* state, int ncomp, int blocks
     a.scomp = scomp;
     a.nc = nc;
     for (int q = 0; q < nc; q++) a.prolong[q] = prolong[q];
-    int e = dispatch(c, [&](auto I) { return decltype(I)::face_fill(c, a, lds); });
+    int e = dispatch_nc(c, [&](auto I) { return decltype(I)::face_fill(c, a, lds); });
     if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1862,7 +1874,7 @@ int sayaka_dev_state_remap(sayaka_dev_ctx* c, const double* old_state, int old_b
     pa.nc = ncomp;
     pa.fd = fdir;
     for (int q = 0; q < ncomp; q++) fa.prolong[q] = ffa.prolong[q] = pa.prolong[q] = prolong[q];
-    int e = dispatch(c, [&](auto I) {
+    int e = dispatch_nc(c, [&](auto I) {
         return decltype(I)::state_remap(c, old_state, (long long)old_blocks_alloc * ntile, state, cstride, ntile,
                                         ncomp, ngrow, fdir, po, pn, no, nn, fa, ffa, pa, lds_fill, lds_face);
     });
@@ -1879,7 +1891,7 @@ int sayaka_dev_pack_levels(sayaka_dev_ctx* c, const double* state, int ncomp, in
     const long long gx = c->nb + 2 * ngrow;
     const long long ntile = gx * gx * (c->ndim == 3 ? c->nb + 2 * ngrow : 1);
     HIPCHK(hipSetDevice(c->device));
-    int e = dispatch(c, [&](auto I) {
+    int e = dispatch_nc(c, [&](auto I) {
         return decltype(I)::pack_levels(c, state, (long long)blocks_alloc * ntile, ncomp, ngrow, out);
     });
     if (e) return e;
@@ -1911,7 +1923,7 @@ int sayaka_dev_remap(sayaka_dev_ctx* c, const int* origin, int nold, const int* 
     if (!c->d_remap_idx) HIPCHK(hipMalloc(&c->d_remap_idx, (size_t)2 * c->cap * sizeof(int)));
     HIPCHK(hipMemcpyAsync(c->d_remap_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     for (int q = 0; q < nfields; q++) {
-        int e = dispatch(c, [&](auto I) {
+        int e = dispatch_nc(c, [&](auto I) {
             return decltype(I)::remap(c, c->d_field[fields[q]], nold, c->d_remap_idx, c->d_remap_idx, lv);
         });
         if (e) return e;
@@ -1938,7 +1950,7 @@ int sayaka_dev_solver_ghosts(sayaka_dev_ctx* c, int m, int field, double* ghost_
     const size_t n = (size_t)c->nblk * 2 * c->ndim * c->nslab;
     if (!c->d_ghost) HIPCHK(hipMalloc(&c->d_ghost, (size_t)c->cap * 2 * c->ndim * c->nslab * sizeof(double)));
     HIPCHK(hipMemsetAsync(c->d_ghost, 0, n * sizeof(double), c->stream));
-    int e = dispatch(c, [&](auto I) { return decltype(I)::solver_ghosts(c, m, c->d_field[field], c->d_ghost); });
+    int e = dispatch_nc(c, [&](auto I) { return decltype(I)::solver_ghosts(c, m, c->d_field[field], c->d_ghost); });
     if (e) return e;
     HIPCHK(hipMemcpyAsync(ghost_out, c->d_ghost, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2014,7 +2026,7 @@ int sayaka_dev_copy(sayaka_dev_ctx* c, int dst, int src) {
 int sayaka_dev_add(sayaka_dev_ctx* c, int m, int dst, int src) {
     if (!c || !field_ok(dst) || !field_ok(src) || m < 1 || m > c->finest) return fail(-1, "bad argument");
     HIPCHK(hipSetDevice(c->device));
-    int e = dispatch(c, [&](auto I) {
+    int e = dispatch_nc(c, [&](auto I) {
         using Im = decltype(I);
         VecArgs a = vec_args(c, c->tw[m]);
         a.dst = c->d_field[dst];
@@ -2030,7 +2042,7 @@ int sayaka_dev_fill(sayaka_dev_ctx* c, int field, double value) {
     if (!c || !field_ok(field)) return fail(-1, "bad argument");
     HIPCHK(hipSetDevice(c->device));
     if (value != 0.0) return fail(-1, "only zero fill is supported");
-    int e = dispatch(c, [&](auto I) {
+    int e = dispatch_nc(c, [&](auto I) {
         using Im = decltype(I);
         VecArgs a = vec_args(c, c->all_blocks);
         a.dst = c->d_field[field];
@@ -2459,7 +2471,7 @@ int sayaka_gen_velocity(sayaka_dev_ctx* c, const double* lo3, const double* dlo3
     int e = vel_alloc(c);
     if (!e) e = gen_scratch(c, lo3, nullptr);
     if (!e)
-        e = dispatch(c, [&](auto I) {
+        e = dispatch_nc(c, [&](auto I) {
             return decltype(I)::gen_velocity(c, c->d_gen, dlo3, root3, seed, zero_walls ? 1 : 0);
         });
     if (e) return e;
