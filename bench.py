@@ -383,6 +383,10 @@ def large_leg(D, W):
         out["mixed"] = dict(ms=round((time.perf_counter() - t1) * 1e3, 2), iters=r["iters"], status=r["status"],
                             rel_resid=float(r["rnorm"] / max(r["rnorm0"], 1e-300)))
         log(f"large: mixed solve {out['mixed']['ms']:.0f} ms, iters {r['iters']}")
+        # the same solve once more with every kernel class timed by events (per-class time and
+        # algorithmic GB/s of the FP32 cycle at this size)
+        _, out["mixed_kernels"] = class_profile(dev, D, D.MIXED)
+        log(f"large: mixed classes {out['mixed_kernels']}")
         return out
     finally:
         dev.close()

@@ -50,13 +50,11 @@ __device__ inline int sphere_box_side(const SphereArgs& a, const double* lo, con
     return 0;
 }
 
-// Sample offsets (s + 1/2) / nsub within a cell edge, s < nsub: computed once per thread, the
-// same divisions the sampling loops would do.
+// Sample offsets (s + 1/2) / nsub within a cell edge, s < nsub: computed once per workgroup
+// (the same divisions the sampling loops would do) into LDS, where a run-time index costs no
+// scratch memory.
 struct SubOffsets {
-    double o[MAXSUB];
-    __device__ explicit SubOffsets(int ns) {
-        for (int q = 0; q < MAXSUB; q++) o[q] = q < ns ? (q + 0.5) / ns : 0.0;
-    }
+    const double* o;
 };
 
 // fluid fraction of the cell (i, j, k) of a block with lower corner lo3 (indices may lie one
@@ -140,7 +138,10 @@ __global__ void __launch_bounds__(256) k_gen_sphere(SphereArgs a, double* __rest
     // closure test of a face reads its two cells from here
     constexpr int E = NB + 2;
     __shared__ double vloc[E * E * E];
-    const SubOffsets so(a.nsub);
+    __shared__ double soff[MAXSUB];
+    if ((int)threadIdx.x < MAXSUB) soff[threadIdx.x] = (int)threadIdx.x < a.nsub ? (threadIdx.x + 0.5) / a.nsub : 0.0;
+    __syncthreads();
+    const SubOffsets so{soff};
     for (int q = threadIdx.x; q < E * E * E; q += blockDim.x) {
         const int i = q % E - 1, j = (q / E) % E - 1, k = q / (E * E) - 1;
         const double v = sphere_cell_fluid(a, so, lo3, i, j, k, dx);

@@ -742,12 +742,11 @@ struct Impl {
         return mix(c, c->tw[L], M_F32_TO_F64, nullptr, ms.u[L], phat64, nullptr);
     }
 
-    // coarse/fine synchronisation of one face field [blk][D][nface] on the finest tower (the
-    // coarse side of every coarse/fine face gets the average of the fine faces), then the faces
-    // of non-leaf blocks averaged from their children, coarsest last (coef.h)
-    // Coarse/fine face sync and average-down of one or two face arrays (fd1 may be null).
-    // Directions and arrays are independent, so each launch covers all of them (grid y =
-    // array * D + direction); the levels keep their order.
+    // Coarse/fine synchronisation of one or two face arrays [blk][D][nface] (fd1 may be null)
+    // on the finest tower (the coarse side of every coarse/fine face gets the average of the
+    // fine faces), then the faces of non-leaf blocks averaged from their children, coarsest
+    // last (coef.h).  Directions and arrays are independent, so each launch covers all of them
+    // (grid y = array * D + direction); the levels keep their order.
     static void sync_faces(sayaka_dev_ctx* c, double* fd0, double* fd1 = nullptr) {
         const int L = c->finest;
         const dim3 gy(1, fd1 ? 2 * D : D);
