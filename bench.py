@@ -439,7 +439,7 @@ def pmc_workload_large():
     """The program the counter passes of the 16^3-block snapshot profile: 3 device-to-device
     copies of a 1.27 GB field (the copy peak), 3 FP64 operator applications and 2 FP64 smoother
     passes (every sub-level, both colours) over the finest
-    tower, then the calibration kernels (a two-field dot over the finest tower reads 2 x 8 B per
+    tower, one mixed-precision solve (the FP32 cycle's kernels), then the calibration kernels (a two-field dot over the finest tower reads 2 x 8 B per
     cell; a fill of all blocks writes 8 B per cell).  The fields are 1.27 GB each: nothing of
     one launch is left in a cache for the next."""
     from sayakaamr_amd import device as D
@@ -457,6 +457,8 @@ def pmc_workload_large():
         dev.apply(L, D.V, D.RHS)
     for _ in range(2):
         dev.smooth(L, D.SOL, D.RHS)
+    dev.zero(D.SOL)
+    dev.solve(tol_rel=1e-10, tol_abs=0.0, max_iter=200, precond=1, precision=D.MIXED)   # the FP32 cycle's kernels
     dev.zero(D.T)
     dev.dot(L, D.X0, D.RHS)
     dev.sync()

@@ -630,6 +630,89 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
                                              reinterpret_cast<double*>(tile), lds);
 }
 
+// One colour of a red-black Gauss-Seidel pass (MODE ST_SMOOTH in place, ST_SMOOTH_DB from x
+// into out with the other colour's cells copied), the per-cell arithmetic of k_stencil's
+// smoothing modes (cell_op) with the updates packed by colour.  Thread t updates the colour's
+// cells t, t + NT, ... in the order i/2 fastest, then j, then k; with k_stencil's mapping
+// (cell t + q NT) every cell of a thread has one parity, so half the lanes of every wave sat
+// masked through a colour pass.  The block and its one-cell halo are staged in LDS as there.
+template <int NB>
+__device__ inline void colour_cell(int a, int colour, int& i, int& j, int& k) {
+    constexpr int H = NB / 2;
+    j = (a / H) % NB;
+    k = a / (H * NB);
+    i = 2 * (a % H) + ((colour + j + k) & 1);
+}
+
+template <int D, int NB, int MODE, class T>
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_smooth(StencilArgsT<T> a) {
+    using G = Geo<D, NB>;
+    static_assert(MODE == ST_SMOOTH || MODE == ST_SMOOTH_DB, "smoothing modes only");
+    static_assert(NB % 2 == 0, "even block edge");
+    constexpr int NT = G::THREADS;
+    constexpr int CPT = (G::NCELL + NT - 1) / NT;   // cells per thread (tile fill, copies)
+    constexpr int NACT = G::NCELL / 2;               // cells of one colour
+    constexpr int APT = (NACT + NT - 1) / NT;        // updates per thread
+    constexpr bool PRE = APT == 1;                   // one update per thread: its inputs at entry
+    __shared__ __attribute__((aligned(16))) T tile[G::NTILE];
+    __shared__ int biw[BI_WORDS];
+    const int b = a.list[blockIdx.x];
+    if ((int)threadIdx.x < BI_WORDS) biw[threadIdx.x] = reinterpret_cast<const int*>(a.binfo + b)[threadIdx.x];
+    T xo[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = threadIdx.x + q * NT;
+        xo[q] = c < G::NCELL ? a.x[b * G::NCELL + c] : T(0);
+    }
+    CellIn<T> in[1];
+    if (PRE) {
+        const int ia = threadIdx.x;
+        if (ia < NACT) {
+            int i, j, k;
+            colour_cell<NB>(ia, a.colour, i, j, k);
+            load_cell<D, NB, MODE, T>(in[0], a, b, G::cidx(i, j, k), true);
+        }
+    }
+    const T dv = (T)a.met[b].dv;
+    __syncthreads();
+    const BlockInfo& bi = *reinterpret_cast<const BlockInfo*>(biw);
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = threadIdx.x + q * NT;
+        if (c >= G::NCELL) continue;
+        int i, j, k;
+        G::cell_ijk(c, i, j, k);
+        tile[G::tidx(i, j, k)] = xo[q];
+        if (MODE == ST_SMOOTH_DB && ((i + j + k) & 1) != a.colour) a.out[b * G::NCELL + c] = xo[q];
+    }
+    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
+    if (PRE) {
+        const int ia = threadIdx.x;
+        if (ia < NACT) {
+            int i, j, k;
+            colour_cell<NB>(ia, a.colour, i, j, k);
+            load_wcf<D, NB, T>(in[0], a, b, G::cidx(i, j, k), bi);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < APT; q++) {
+        const int ia = threadIdx.x + q * NT;
+        if (ia >= NACT) continue;
+        int i, j, k;
+        colour_cell<NB>(ia, a.colour, i, j, k);
+        const int c = G::cidx(i, j, k);
+        CellIn<T>& ci = in[0];
+        if (!PRE) {
+            load_cell<D, NB, MODE, T>(ci, a, b, c, true);
+            load_wcf<D, NB, T>(ci, a, b, c, bi);
+        }
+        const T u = cell_op<D, NB, true, T>(tile, ci.w, ci.wcf, i, j, k, bi, a.m, a.alpha, dv, ci.vf, ci.rhs);
+        if (MODE == ST_SMOOTH_DB) a.out[b * G::NCELL + c] = u;
+        else const_cast<T*>(a.x)[b * G::NCELL + c] = u;
+    }
+}
+
 // --------------------------------------------------------------- restriction / prolongation
 // mg_restict_wgt onto the non-leaf blocks of one level (SayakaMacIB.cpp:157-232 with
 // restrictCellBlockData_Average, SayakaInterpPatch.cpp:398-444): coarse =

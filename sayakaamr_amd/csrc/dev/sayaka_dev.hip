@@ -336,8 +336,8 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
         case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_SMOOTH:
-            hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, ps.start(), ps.stop(),
-                                  0, a, rb, fin);
+            hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, ps.start(),
+                                  ps.stop(), 0, a);
             break;
         case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2, double>), grid, block, 0, c->stream, a, rb, fin); break;
@@ -671,8 +671,8 @@ struct Impl {
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL,
                              T.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
-                hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0,
-                                      c->stream, ps.start(), ps.stop(), 0, a, RedBuf{}, FIN_NONE);
+                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0,
+                                      c->stream, ps.start(), ps.stop(), 0, a);
             }
             HIPCHK(hipGetLastError());
             return 0;
@@ -687,8 +687,8 @@ struct Impl {
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL,
                              sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
-                hipExtLaunchKernelGGL((k_stencil<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0,
-                                      c->stream, ps.start(), ps.stop(), 0, a, RedBuf{}, FIN_NONE);
+                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0,
+                                      c->stream, ps.start(), ps.stop(), 0, a);
             }
         }
         HIPCHK(hipGetLastError());
