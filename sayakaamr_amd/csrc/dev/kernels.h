@@ -521,10 +521,10 @@ __device__ inline void load_wcf(CellIn<T>& in, const StencilArgsT<T>& a, int b, 
 
 constexpr int BI_WORDS = sizeof(BlockInfo) / sizeof(int);
 
-// Operator / residual / smoother over one block list (one workgroup per block).  Dependent
-// memory round trips per workgroup: block id -> {table, own cells, weights, vf, rhs in
-// parallel} -> halo (needs the table) -> store.  FP64 modes with a fused dot product finish
-// with the grid reduction (reference order when rb.exact).
+// Operator / residual over one block list (one workgroup per block; the colour passes are
+// k_smooth's).  Dependent memory round trips per workgroup: block id -> {table, own cells,
+// weights, vf, rhs in parallel} -> halo (needs the table) -> store.  FP64 modes with a fused
+// dot product finish with the grid reduction (reference order when rb.exact).
 template <int D, int NB, int MODE, class T>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
 k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
@@ -532,7 +532,7 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
     constexpr int NT = G::THREADS;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;
     constexpr bool REDUCE = MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT;
-    constexpr bool SMOOTHING = MODE == ST_SMOOTH || MODE == ST_SMOOTH_DB;
+    static_assert(MODE != ST_SMOOTH && MODE != ST_SMOOTH_DB, "the colour passes are k_smooth's");
     static_assert(!REDUCE || sizeof(T) == 8, "fused reductions are FP64 only");
     __shared__ __attribute__((aligned(16))) T tile[G::NTILE > 2 * G::NCELL + 4 ? G::NTILE : 2 * G::NCELL + 4];
     __shared__ double lds[3 * ((NT + 63) / 64)];
@@ -550,9 +550,8 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
         if (c >= G::NCELL) continue;
         int i, j, k;
         G::cell_ijk(c, i, j, k);
-        const bool active = !SMOOTHING || ((i + j + k) & 1) == a.colour;
         if (PRE) {
-            load_cell<D, NB, MODE, T>(in[0], a, b, c, active);
+            load_cell<D, NB, MODE, T>(in[0], a, b, c, true);
             xo[q] = in[0].x;
         } else {
             xo[q] = a.x[b * G::NCELL + c];
@@ -574,7 +573,7 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
         const int c = threadIdx.x;
         int i, j, k;
         G::cell_ijk(c, i, j, k);
-        if (c < G::NCELL && (!SMOOTHING || ((i + j + k) & 1) == a.colour)) load_wcf<D, NB, T>(in[0], a, b, c, bi);
+        if (c < G::NCELL) load_wcf<D, NB, T>(in[0], a, b, c, bi);
     }
     __syncthreads();
     double q0[CPT], q1[CPT], mx = 0.0;
@@ -587,24 +586,11 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
         G::cell_ijk(c, i, j, k);
         const int gi = b * G::NCELL + c;
         if (!PRE) {
-            const bool active = !SMOOTHING || ((i + j + k) & 1) == a.colour;
-            if (!active) {
-                if (MODE == ST_SMOOTH_DB) a.out[gi] = xo[q];
-                continue;
-            }
             load_cell<D, NB, MODE, T>(in[0], a, b, c, true);
             load_wcf<D, NB, T>(in[0], a, b, c, bi);
         }
         const T vfv = in[0].vf;
-        if (SMOOTHING) {
-            if (((i + j + k) & 1) != a.colour) {
-                if (MODE == ST_SMOOTH_DB) a.out[gi] = xo[q];
-                continue;
-            }
-            const T u = cell_op<D, NB, true, T>(tile, in[0].w, in[0].wcf, i, j, k, bi, a.m, a.alpha, dv, vfv, in[0].rhs);
-            if (MODE == ST_SMOOTH_DB) a.out[gi] = u;
-            else const_cast<T*>(a.x)[gi] = u;
-        } else {
+        {
             const T Lp = cell_op<D, NB, false, T>(tile, in[0].w, in[0].wcf, i, j, k, bi, a.m, a.alpha, dv, vfv, T(0));
             if (MODE == ST_APPLY || MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2) {
                 a.out[gi] = Lp;
