@@ -85,7 +85,7 @@ struct RedBuf {
                         //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
 };
 
-// Lane l of the calling wave sums the N values staged for chain l, in index order.  Chain
+// Lane l of the calling wave adds the N values staged for chain l onto s, in index order.  Chain
 // l occupies red[l*(N+2) .. l*(N+2)+N): the stride keeps every chain 16-byte aligned for
 // 128-bit LDS reads and puts concurrent chains on distinct LDS banks.  A sequential FP64 chain
 // is bound by the dependent-add latency (~10 cycles on MI355X); the chain is unrolled with two
@@ -93,10 +93,9 @@ struct RedBuf {
 // cycles per add with W = 8, against 22 for a rolled loop with register rotation).  W = 4
 // keeps the register footprint of kernels that chain only at their end small.
 template <int N, int W = 8>
-__device__ inline double lane_chain(const double* red, int l) {
+__device__ inline double lane_chain(const double* red, int l, double s = 0.0) {
     static_assert(N % (4 * W) == 0, "chain length must be a multiple of 4W");
     const double2* p = reinterpret_cast<const double2*>(red + l * (N + 2));
-    double s = 0.0;
     double2 A[W], B[W];
 #pragma unroll
     for (int u = 0; u < W; u++) A[u] = p[u];
@@ -165,23 +164,26 @@ __device__ inline void wave_lds_barrier() {
 __device__ inline void stage_chain(double* red, int n, int l, int c, double v) { red[l * (n + 2) + c] = v; }
 
 // Sums, in cell order, of nsum (1 or 2) per-cell values held by the threads (cell
-// c = tid + q*NT); the two chains run concurrently in lanes 0 and 1.  `scratch` is LDS of
-// at least 2*NCELL+2 doubles.  Results valid in thread 0.
+// c = tid + q*NT); the two chains run concurrently in lanes 0 and 1.  The cells of one q are
+// the contiguous run q*NT .. q*NT+NT-1, so the chains go through LDS one run at a time, the
+// sums carried from run to run: `scratch` is 16-byte aligned LDS of at least 2*min(NCELL,
+// NT)+4 doubles.  Results valid in thread 0.
 template <int NT, int NCELL, int CPT>
 __device__ inline void seq_block_sums(const double (&v0)[CPT], const double (&v1)[CPT], int nsum, double* scratch,
                                       double& s0, double& s1) {
-    __syncthreads();
+    static_assert(CPT == 1 || NCELL == CPT * NT, "runs of NT cells");
+    constexpr int RUN = CPT == 1 ? NCELL : NT;
+    double r = 0.0;
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-        const int c = threadIdx.x + q * NT;
-        if (c < NCELL) {
-            stage_chain(scratch, NCELL, 0, c, v0[q]);
-            if (nsum > 1) stage_chain(scratch, NCELL, 1, c, v1[q]);
+        __syncthreads();
+        if ((int)threadIdx.x < RUN) {
+            stage_chain(scratch, RUN, 0, threadIdx.x, v0[q]);
+            if (nsum > 1) stage_chain(scratch, RUN, 1, threadIdx.x, v1[q]);
         }
+        __syncthreads();
+        if ((int)threadIdx.x < nsum) r = lane_chain<RUN, 4>(scratch, threadIdx.x, r);
     }
-    __syncthreads();
-    double r = 0.0;
-    if ((int)threadIdx.x < nsum) r = lane_chain<NCELL, 4>(scratch, threadIdx.x);
     __syncthreads();
     if (threadIdx.x == 1) scratch[0] = r;
     __syncthreads();
@@ -272,7 +274,7 @@ __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsu
 }
 
 // Block totals of nsum per-cell sums (q0, q1) and a max, stored as the workgroup's
-// partials.  `scratch` is 16-byte aligned LDS of at least 2*NCELL+4 doubles.
+// partials.  `scratch` is 16-byte aligned LDS of at least 2*min(NCELL, NT)+4 doubles.
 template <int NT, int NCELL, int CPT>
 __device__ inline void grid_reduce_cells(const double (&q0)[CPT], const double (&q1)[CPT], double mx, int nsum,
                                          const RedBuf& rb, double* scratch, double* lds) {
@@ -534,7 +536,10 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
     constexpr bool REDUCE = MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT;
     static_assert(MODE != ST_SMOOTH && MODE != ST_SMOOTH_DB, "the colour passes are k_smooth's");
     static_assert(!REDUCE || sizeof(T) == 8, "fused reductions are FP64 only");
-    __shared__ __attribute__((aligned(16))) T tile[G::NTILE > 2 * G::NCELL + 4 ? G::NTILE : 2 * G::NCELL + 4];
+    // the tile doubles as the reduction's staging (2*min(NCELL, NT)+4 doubles): sized by the
+    // larger only when the mode reduces, so that 16^3 blocks keep 3 workgroups per CU
+    constexpr int NRED = REDUCE ? 2 * (G::NCELL < NT ? G::NCELL : NT) + 4 : 0;
+    __shared__ __attribute__((aligned(16))) T tile[G::NTILE > NRED ? G::NTILE : NRED];
     __shared__ double lds[3 * ((NT + 63) / 64)];
     __shared__ int biw[BI_WORDS];
     const int b = a.list[blockIdx.x];
@@ -784,7 +789,7 @@ template <int NCELL, int NT>
 __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fin) {
     constexpr int CPT = (NCELL + NT - 1) / NT;
     __shared__ double lds[3 * ((NT + 63) / 64)];
-    __shared__ __attribute__((aligned(16))) double scratch[2 * NCELL + 4];
+    __shared__ __attribute__((aligned(16))) double scratch[2 * (NCELL < NT ? NCELL : NT) + 4];
     const int b = a.list[blockIdx.x];
     double q0[CPT], q1[CPT], mx = 0.0;
     const double* S = a.scal;
