@@ -168,6 +168,18 @@ int sayaka_dev_block_move(sayaka_dev_ctx* ctx, const double* src, double* dst, i
  * (sayaka_dev_set_surroundings). */
 int sayaka_dev_transfer_stats(sayaka_dev_ctx* ctx, long long* out4);
 
+/* Placement of the coarsest-level solve's persistent launch (no reference counterpart; the
+ * reference solves on one CPU thread).  one_xcd = 1 (the default): when its workgroups fit on
+ * one XCD they run there, and the grid barriers use that XCD's L2 as their coherence point
+ * (no L2 write-back or invalidation); 0: spread over the device with device-wide barriers.
+ * The results are identical either way. */
+int sayaka_dev_set_bottom_placement(sayaka_dev_ctx* ctx, int one_xcd);
+
+/* Coarsest-solve launches since the context's creation: out3[0] all, out3[1] placed on one XCD,
+ * out3[2] of those, launches that found their workgroups on more than one XCD and kept the
+ * device-wide barriers. */
+int sayaka_dev_bottom_placement(sayaka_dev_ctx* ctx, long long* out3);
+
 /* The tree's 3^d surrounding blocks for the general state fill, after every
  * sayaka_dev_set_topology: surr27[b][27] = SurroundingBlocks::blocks and bcref27[b][27] =
  * SurroundingBlocks::bcrefs of block b, both in the reference's [ii+1][jj+1][kk+1] order

@@ -9,8 +9,9 @@ run in a process that has not loaded libsayaka_dev.so:
     python -m sayakaamr_amd.bottom_phases [--solves 5] [--out phases.json]
 
 Prints, per precision (FP64: reference-order sums; mixed: the two-level preconditioned FP32
-cycle's FP64 coarse solve), the microseconds per iteration of every phase (workgroup 0's wall
-clock, 100 MHz on MI355X) and per launch of the set-up.
+cycle's FP64 coarse solve) and placement (the launch on one XCD, the default, and spread over
+the device: "_spread"), the microseconds per iteration of every phase (workgroup 0's wall clock,
+100 MHz on MI355X) and per launch of the set-up.
 """
 from __future__ import annotations
 
@@ -62,7 +63,10 @@ def main(argv=None):
         fn.restype = C.c_int
         out = np.zeros(32, np.uint64)
         khz = C.c_int(0)
-        for name, prec in (("fp64", D.FP64), ("mixed", D.MIXED)):
+        runs = [(name + ("" if one else "_spread"), prec, one) for one in (True, False)
+                for name, prec in (("fp64", D.FP64), ("mixed", D.MIXED))]
+        for name, prec, one in runs:
+            dev.set_bottom_placement(one)
             dev.zero(D.SOL)
             dev.solve(precision=prec)   # warm-up
             if fn(0, out.ctypes.data, 1, C.byref(khz)) != 0:
@@ -76,7 +80,8 @@ def main(argv=None):
             iters, launches = int(out[30]), int(out[31])
             per_it = {v: round(float(out[k]) * us / iters, 3) for k, v in NAMES.items()}
             per_launch = {v: round(float(out[k]) * us / launches, 3) for k, v in LAUNCH.items()}
-            res[name] = dict(solves=args.solves, launches=launches, iterations=iters, wall_clock_khz=khz.value,
+            res[name] = dict(placement="one XCD" if one else "spread", bottom_placement=dev.bottom_placement(),
+                             solves=args.solves, launches=launches, iterations=iters, wall_clock_khz=khz.value,
                              us_per_iteration=round(sum(per_it.values()), 3), per_iteration_us=per_it,
                              per_launch_us=per_launch)
             print(name, json.dumps(res[name], indent=1), flush=True)

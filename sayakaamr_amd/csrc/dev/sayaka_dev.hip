@@ -113,6 +113,9 @@ struct sayaka_dev_ctx {
     unsigned long long* d_bot_flags = nullptr;   // grid-barrier flags of the bottom solve, one per workgroup
     unsigned long long bot_launches = 0;
     int bot_fit[2] = {0, 0};                     // workgroups of k_bottom<.., PC> resident at once
+    int bot_fit_xcd[2] = {0, 0};                 // ... on one XCD
+    int bot_one_xcd = 1;                         // place the coarsest solve on one XCD when it fits
+    long long bot_place[3] = {0, 0, 0};          // launches; on one XCD; of those, fallen back
     int* h_bot_stats = nullptr;  // pinned
 
     MixedState mixed;
@@ -186,6 +189,16 @@ struct sayaka_dev_ctx {
 namespace {
 
 void cnn_free(sayaka_dev_ctx::Cnn& n);
+
+constexpr int NUM_XCD = 8;   // MI355X: 8 XCDs of 32 CUs, each with its own L2
+
+// The coarsest solve's launch counters of the last readback (h_bot_stats) into the context's
+// running totals.
+void note_bottom_stats(sayaka_dev_ctx* c) {
+    c->bot_place[0] += c->h_bot_stats[0];
+    c->bot_place[1] += c->h_bot_stats[5];
+    c->bot_place[2] += c->h_bot_stats[6];
+}
 
 // Streaming device copy (TreeData::Copy of a whole field): one 16-byte element per lane and a
 // grid that covers the field.  Measured on MI355X over 2 GiB: 6.2 TB/s (reads + writes), against
@@ -463,23 +476,31 @@ struct Impl {
         // GPU) into error -5 instead of a hang.  A cooperative launch would add nothing but a
         // device-wide lock that keeps a second process from solving on the same GPU.
         int& fit = c->bot_fit[two_level ? 1 : 0];
+        int& fit_xcd = c->bot_fit_xcd[two_level ? 1 : 0];
         if (fit == 0) {
             int per_cu = 0, ncu = 0;
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BC::NT, 0));
             HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
             fit = per_cu * ncu;
+            fit_xcd = per_cu * (ncu / NUM_XCD);
         }
         if (nwg > fit)
             return fail(-2, "coarsest level too large for one resident grid: " + std::to_string(nwg) +
                                 " workgroups, " + std::to_string(fit) + " fit");
+        // One XCD when the workers fit there: the dispatcher deals workgroups to the XCDs in
+        // turn (workgroup i to XCD i mod 8), so a grid of 8 x nwg puts the workers
+        // blockIdx.x % 8 == 0 on one XCD and the others exit at once.  The kernel checks the
+        // placement at its first barrier and keeps the device-wide barrier if it does not hold.
+        A.confine = c->bot_one_xcd && nwg <= fit_xcd ? 1 : 0;
+        const dim3 grid(A.confine ? NUM_XCD * nwg : nwg);
         // iterative: its algorithmic bytes depend on the iteration count, which the caller
         // reads back (bytes recorded as 0); timed by the launch's own start / stop events
         ProfScope ps(c, SAYAKA_KCLASS_BOTTOM, (long long)lst.n * G::NCELL, 0.0, true);
         if (two_level)
-            hipExtLaunchKernelGGL((k_bottom<D, NB, true>), dim3(nwg), dim3(BC::NT), 0, c->stream, ps.start(),
+            hipExtLaunchKernelGGL((k_bottom<D, NB, true>), grid, dim3(BC::NT), 0, c->stream, ps.start(),
                                   ps.stop(), 0, A);
         else
-            hipExtLaunchKernelGGL((k_bottom<D, NB, false>), dim3(nwg), dim3(BC::NT), 0, c->stream, ps.start(),
+            hipExtLaunchKernelGGL((k_bottom<D, NB, false>), grid, dim3(BC::NT), 0, c->stream, ps.start(),
                                   ps.stop(), 0, A);
         HIPCHK(hipGetLastError());
         return 0;
@@ -1123,6 +1144,7 @@ struct Impl {
         }
         HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        note_bottom_stats(c);
         if (c->h_bot_stats[4]) return fail(-5, "coarsest-level solve: a grid barrier timed out");
         st->bottom_calls = c->h_bot_stats[0];
         st->bottom_iters = c->h_bot_stats[1];
@@ -1591,6 +1613,18 @@ int sayaka_dev_memcpy(sayaka_dev_ctx* c, void* dst, const void* src, size_t byte
     return 0;
 }
 
+int sayaka_dev_set_bottom_placement(sayaka_dev_ctx* c, int one_xcd) {
+    if (!c || (one_xcd != 0 && one_xcd != 1)) return fail(-1, "bad argument");
+    c->bot_one_xcd = one_xcd;
+    return 0;
+}
+
+int sayaka_dev_bottom_placement(sayaka_dev_ctx* c, long long* out3) {
+    if (!c || !out3) return fail(-1, "bad argument");
+    for (int q = 0; q < 3; q++) out3[q] = c->bot_place[q];
+    return 0;
+}
+
 int sayaka_dev_transfer_stats(sayaka_dev_ctx* c, long long* out4) {
     if (!c || !out4) return fail(-1, "bad argument");
     for (int q = 0; q < 4; q++) out4[q] = c->xfer[q];
@@ -1991,6 +2025,7 @@ int sayaka_dev_bottom(sayaka_dev_ctx* c, int sol, int rhs, double tol_rel, doubl
     if (e) return e;
     HIPCHK(hipMemcpyAsync(c->h_bot_stats, c->d_bot_stats, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    note_bottom_stats(c);
     if (c->h_bot_stats[4]) return fail(-5, "coarsest-level solve: a grid barrier timed out");
     if (iters) *iters = c->h_bot_stats[2];
     if (status) *status = c->h_bot_stats[3];

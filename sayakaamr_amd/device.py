@@ -86,6 +86,8 @@ _SIG = {
     "sayaka_dev_project": ([vp, i32, C.POINTER(f64), C.POINTER(f64)], i32),
     "sayaka_dev_get_velocity": ([vp, vp], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
+    "sayaka_dev_set_bottom_placement": ([vp, i32], i32),
+    "sayaka_dev_bottom_placement": ([vp, vp], i32),
     "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                  C.POINTER(f64)], i32),
 }
@@ -529,6 +531,17 @@ class MacSolverGPU:
 
     def profile(self, kclass, enable=True):
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")
+
+    def set_bottom_placement(self, one_xcd=True):
+        """The coarsest solve on one XCD when it fits (True, the default) or spread over the device."""
+        _chk(self.lib.sayaka_dev_set_bottom_placement(self.h, int(bool(one_xcd))), "set_bottom_placement")
+
+    def bottom_placement(self):
+        """Coarsest-solve launches so far: all, placed on one XCD, and of those the launches that
+        found their workgroups on several XCDs (device-wide barriers kept)."""
+        out = np.zeros(3, np.int64)
+        _chk(self.lib.sayaka_dev_bottom_placement(self.h, N.ptr(out)), "bottom_placement")
+        return dict(launches=int(out[0]), one_xcd=int(out[1]), fallen_back=int(out[2]))
 
     def profile_read(self, kclass):
         t = f64()

@@ -60,6 +60,35 @@ def test_c1_fp64_matches_reference(c1):
     assert true_residual(dev, spec, tree, sol) <= 2e-10 * res["rnorm0"]
 
 
+def test_c1_coarsest_solve_placement(c1):
+    """The coarsest-level solve on one XCD (the default: its grid barriers use that XCD's L2 as
+    the coherence point) and spread over the device (device-wide barriers): the same iterations
+    and the same bits, in FP64 and in mixed precision; every one-XCD launch found its workgroups
+    on one XCD."""
+    g, spec, tree, inp, dev = c1
+    out = {}
+    try:
+        for one in (True, False):
+            dev.set_bottom_placement(one)
+            p0 = dev.bottom_placement()
+            for prec in (D.FP64, D.MIXED):
+                dev.zero(D.SOL)
+                r = dev.solve(precision=prec)
+                assert r["status"] == 0
+                out[one, prec] = (r["iters"], r["bottom_iters"], dev.download(D.SOL))
+            p1 = dev.bottom_placement()
+            n = p1["launches"] - p0["launches"]
+            print(f"[gpu] placement one_xcd={one}: {p1}", flush=True)
+            assert n > 0 and p1["fallen_back"] == p0["fallen_back"]
+            assert p1["one_xcd"] - p0["one_xcd"] == (n if one else 0)
+    finally:
+        dev.set_bottom_placement(True)
+    assert out[True, D.FP64][0] == g["solve_iters"][0]
+    for prec in (D.FP64, D.MIXED):
+        assert out[True, prec][:2] == out[False, prec][:2]
+        assert np.array_equal(out[True, prec][2], out[False, prec][2])
+
+
 def test_c1_mixed_precision(c1):
     g, spec, tree, inp, dev = c1
     dev.zero(D.SOL)
