@@ -105,6 +105,7 @@ struct BarState {
     int* broken;             // (LDS) a barrier of this workgroup timed out
     int* light;              // (LDS) every worker is on one XCD: barriers without L2 maintenance
 };
+constexpr int BOTTOM_STALL = 25;   // PC: iterations without halving the best residual before stopping
 constexpr unsigned BARRIER_SPIN_MAX = 1u << 18;   // polls before a grid barrier gives up (~0.2 s)
 
 template <int D, int NB>
@@ -522,6 +523,13 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
     const double rnorm0 = mx;
     double rho = s0v, rho1 = 0.0, alpha = 0.0, omega = 0.0;
     int iter = 0, conv = 0, stat = 0;
+    // PC (the mixed cycle's coarse solve, a preconditioner): the residual after each full
+    // iteration, the best one, and where it last halved.  BiCGSTAB can stagnate on a rare
+    // right-hand side; the solve then stops once BOTTOM_STALL iterations have not halved the
+    // best residual and returns its last iterate when that improved on the zero guess (the
+    // FP64 modes keep the reference's rule: solution written on convergence only).
+    double last_r = rnorm0, best_r = rnorm0;
+    int best_it = 0, stall = 0;
 
     // operand tile of the workgroup's blocks from own registers + ghost registers
     auto fill_tile = [&](const double (&own)[CPT], const double (&gh)[GPT]) {
@@ -702,12 +710,23 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
             load_ghosts(rG, A.rpub);
             SY_PH_MARK(pht, PH_R_GHOST);
             if (mx < rnorm0 * A.tol_rel + A.tol_abs) { conv = 1; break; }
+            if constexpr (PC) {
+                last_r = mx;
+                if (mx < 0.5 * best_r) {
+                    best_r = mx;
+                    best_it = iter;
+                } else if (iter - best_it >= BOTTOM_STALL) {
+                    stall = 1;
+                    break;
+                }
+            }
             if (omega == 0.0) { stat = 1; break; }
             rho1 = rho;
             rho = s0v;
         }
         if (iter > A.max_iter) iter = A.max_iter;
     }
+    if (PC && !conv && !stat && last_r < rnorm0) conv = 1;   // stalled or out of iterations: last iterate
     if (conv) {
 #pragma unroll
         for (int q = 0; q < CPT; q++)
@@ -727,6 +746,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
         A.stats[1] += iter;
         A.stats[2] = iter;
         A.stats[3] = stat;
+        A.stats[7] += stall;
     }
 }
 

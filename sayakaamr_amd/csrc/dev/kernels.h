@@ -374,13 +374,13 @@ __device__ inline T cf_sum(int f, int t1, int t2, const BlockInfo& bi, const T* 
 // SayakaTreeData.h:41-44).  Finer neighbour on tower m (a coarse/fine face seen from the
 // coarse side): the sum of the fine w*phi over the face cell (cf_sum), which is all the
 // operator needs from that side.  Walls: 0 (walls enter through the weights).
-template <int D, int NB, class T>
+template <int D, int NB, class T, int NTH = Geo<D, NB>::THREADS>
 __device__ inline void load_halo(T* tile, const T* __restrict__ x, const T* __restrict__ w, const BlockInfo& bi,
                                  int m) {
     using G = Geo<D, NB>;
     // stride by the compile-time thread count: blockDim is read from the dispatch packet,
     // one scalar load per wave that measured ~14 us per launch on MI355X
-    for (int g = threadIdx.x; g < G::NGHOST; g += G::THREADS) {
+    for (int g = threadIdx.x; g < G::NGHOST; g += NTH) {
         const int f = g / G::NSLAB, p = g % G::NSLAB;
         const int d = f >> 1, s = f & 1;
         int a1, a2;

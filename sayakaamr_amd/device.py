@@ -426,10 +426,11 @@ class MacSolverGPU:
         """Two-level preconditioner of the mixed solve's coarsest BiCGSTAB on / off."""
         _chk(self.lib.sayaka_dev_set_mixed_coarse(self.h, int(bool(two_level))), "set_mixed_coarse")
 
-    def set_mixed_smoother(self, merged: bool):
-        """Mixed V-cycle smoother: one launch per colour over the whole tower (double-buffered)
-        or one launch per sub-level and colour in place."""
-        _chk(self.lib.sayaka_dev_set_mixed_smoother(self.h, int(bool(merged))), "set_mixed_smoother")
+    def set_mixed_smoother(self, mode: int):
+        """Mixed V-cycle smoother: 2 = hybrid sweeps (one launch per sweep, the default),
+        1 = exact red-black with one launch per colour over the whole tower (double-buffered),
+        0 = exact red-black with one launch per sub-level and colour in place."""
+        _chk(self.lib.sayaka_dev_set_mixed_smoother(self.h, int(mode)), "set_mixed_smoother")
 
     # --- learned initial guess (include/sayaka_amd.h, sayaka_cnn_*) -------------------
     CNN_NP = 4 * (27 * 16 * 16 + 16)
