@@ -789,7 +789,10 @@ struct Impl {
                                c->d_binfo, ms.vf32, ms.u[l], ms.u[l - 1], l);
             HIPCHK(hipGetLastError());
         }
-        if ((e = smooth_level(c, l, post, false, out64, &wrote64))) return e;
+        // hybrid sweeps: one post-sweep more than asked (the stale block boundaries; on the moving
+        // 16^3-block cycle V(2,3) hybrid takes the exact V(2,2)'s outer iterations, V(2,2) one more)
+        const int npost = post + (ms.merged_smooth == 2 && post > 0 ? 1 : 0);
+        if ((e = smooth_level(c, l, npost, false, out64, &wrote64))) return e;
         if (out64 && !wrote64) return mix(c, T, M_F32_TO_F64, nullptr, ms.u[l], out64, nullptr);
         return 0;
     }
