@@ -527,11 +527,23 @@ constexpr int BI_WORDS = sizeof(BlockInfo) / sizeof(int);
 // k_smooth's).  Dependent memory round trips per workgroup: block id -> {table, own cells,
 // weights, vf, rhs in parallel} -> halo (needs the table) -> store.  FP64 modes with a fused
 // dot product finish with the grid reduction (reference order when rb.exact).
+// Threads of a k_stencil / k_smooth workgroup.  On the 139 M-cell 16^3-block snapshot 1 024
+// threads ran the FP64 operator at 4.38 against 4.64 TB/s with 512 (80 VGPRs, 6 waves per
+// SIMD), and the FP64 colour pass at 2.47 against 2.36 TB/s.
+template <int D, int NB>
+struct StencilCfg {
+    static constexpr int NT = Geo<D, NB>::THREADS;
+};
+template <int D, int NB>
+struct SmoothCfg {
+    static constexpr int NT = NB >= 16 ? 1024 : Geo<D, NB>::THREADS;
+};
+
 template <int D, int NB, int MODE, class T>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+__global__ void __launch_bounds__((StencilCfg<D, NB>::NT))
 k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
     using G = Geo<D, NB>;
-    constexpr int NT = G::THREADS;
+    constexpr int NT = StencilCfg<D, NB>::NT;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;
     constexpr bool REDUCE = MODE == ST_APPLY_DOT || MODE == ST_APPLY_DOT2 || MODE == ST_RESID_INIT;
     static_assert(MODE != ST_SMOOTH && MODE != ST_SMOOTH_DB, "the colour passes are k_smooth's");
@@ -573,7 +585,7 @@ k_stencil(StencilArgsT<T> a, RedBuf rb, int fin) {
         G::cell_ijk(c, i, j, k);
         tile[G::tidx(i, j, k)] = xo[q];
     }
-    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
+    load_halo<D, NB, T, NT>(tile, a.x, a.w, bi, a.m);
     if (PRE) {
         const int c = threadIdx.x;
         int i, j, k;
@@ -636,11 +648,11 @@ __device__ inline void colour_cell(int a, int colour, int& i, int& j, int& k) {
 }
 
 template <int D, int NB, int MODE, class T>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_smooth(StencilArgsT<T> a) {
+__global__ void __launch_bounds__((SmoothCfg<D, NB>::NT)) k_smooth(StencilArgsT<T> a) {
     using G = Geo<D, NB>;
     static_assert(MODE == ST_SMOOTH || MODE == ST_SMOOTH_DB, "smoothing modes only");
     static_assert(NB % 2 == 0, "even block edge");
-    constexpr int NT = G::THREADS;
+    constexpr int NT = SmoothCfg<D, NB>::NT;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;   // cells per thread (tile fill, copies)
     constexpr int NACT = G::NCELL / 2;               // cells of one colour
     constexpr int APT = (NACT + NT - 1) / NT;        // updates per thread
@@ -676,7 +688,7 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_smooth(StencilArgsT<T
         tile[G::tidx(i, j, k)] = xo[q];
         if (MODE == ST_SMOOTH_DB && ((i + j + k) & 1) != a.colour) a.out[b * G::NCELL + c] = xo[q];
     }
-    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
+    load_halo<D, NB, T, NT>(tile, a.x, a.w, bi, a.m);
     if (PRE) {
         const int ia = threadIdx.x;
         if (ia < NACT) {

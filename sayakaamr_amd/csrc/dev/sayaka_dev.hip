@@ -341,7 +341,7 @@ template <int D, int NB>
 int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
     using G = Geo<D, NB>;
     if (a.nlist <= 0) return 0;
-    dim3 grid(a.nlist), block(G::THREADS);
+    dim3 grid(a.nlist), block(StencilCfg<D, NB>::NT);
     RedBuf rb = redbuf(c);
     ProfScope ps(c, mode == ST_SMOOTH ? SAYAKA_KCLASS_SMOOTH : SAYAKA_KCLASS_APPLY, (long long)a.nlist * G::NCELL,
                  a.nlist * AlgBytes<D, NB, double>::stencil(mode), mode == ST_SMOOTH);
@@ -349,8 +349,8 @@ int launch_stencil(sayaka_dev_ctx* c, int mode, const StencilArgs& a, int fin) {
         case ST_APPLY: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_RESID: hipLaunchKernelGGL((k_stencil<D, NB, ST_RESID, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_SMOOTH:
-            hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, double>), grid, block, 0, c->stream, ps.start(),
-                                  ps.stop(), 0, a);
+            hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, double>), grid, dim3(SmoothCfg<D, NB>::NT), 0,
+                                  c->stream, ps.start(), ps.stop(), 0, a);
             break;
         case ST_APPLY_DOT: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT, double>), grid, block, 0, c->stream, a, rb, fin); break;
         case ST_APPLY_DOT2: hipLaunchKernelGGL((k_stencil<D, NB, ST_APPLY_DOT2, double>), grid, block, 0, c->stream, a, rb, fin); break;
@@ -692,7 +692,7 @@ struct Impl {
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL,
                              T.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
-                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(G::THREADS), 0,
+                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH_DB, float>), dim3(T.n), dim3(SmoothCfg<D, NB>::NT), 0,
                                       c->stream, ps.start(), ps.stop(), 0, a);
             }
             HIPCHK(hipGetLastError());
@@ -708,7 +708,7 @@ struct Impl {
                 a.colour = rb;
                 ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)sub.n * G::NCELL,
                              sub.n * AlgBytes<D, NB, float>::stencil(ST_SMOOTH), true);
-                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(G::THREADS), 0,
+                hipExtLaunchKernelGGL((k_smooth<D, NB, ST_SMOOTH, float>), dim3(sub.n), dim3(SmoothCfg<D, NB>::NT), 0,
                                       c->stream, ps.start(), ps.stop(), 0, a);
             }
         }
