@@ -281,7 +281,7 @@ struct Bottom {
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         mxl = wave_max(mxl);
         if (lane == 0) wl[wv] = mxl;
-        if (!A.exact) {
+        if (PC || !A.exact) {
             a0 = wave_sum(a0);
             a1 = wave_sum(a1);
             if (lane == 0) {
@@ -292,10 +292,10 @@ struct Bottom {
         __syncthreads();
         SY_PH_MARK(pht, PH_WAIT + (ph - 1) / 4);
         if (wv == 0) {
-            if (A.exact) {
+            if (!PC && A.exact) {   // (the two-level variant always sums in tree order)
                 // ordered per-block chains, one lane per block and sum
                 if (lane < BPW * nsum) {
-                    const double sv = lane_chain<G::NCELL>(red, lane);
+                    const double sv = lane_chain<G::NCELL, 200>(red, lane);
                     const int blk = lane % BPW;
                     if (blk < nvalid) part[PS * (w0 + blk) + lane / BPW] = sv;
                 }
@@ -318,10 +318,26 @@ struct Bottom {
         SY_PH_MARK(pht, ph);
         grid_barrier(A, bs, (nl + BPW - 1) / BPW);
         SY_PH_MARK(pht, ph + 1);
-        // totals (wave 0): the partials staged in LDS by lane in chunks of 64, the max reduced
-        // across the wave, the sums chained in block order in lanes 0 / 1 -- every workgroup
-        // forms the same totals in the same order
-        if (wv == 0) {
+        // totals (wave 0).  Ordered sums: the partials staged in LDS by lane in chunks of 64 and
+        // chained in block order in lanes 0 / 1.  Two-level variant (order-free): each lane adds
+        // its blocks' partials, then a fixed shuffle tree.  The max is reduced across the wave.
+        // Every workgroup forms the same totals in the same order.
+        if (wv == 0 && PC) {
+            double t0 = 0.0, t1 = 0.0, tm = 0.0;
+            for (int b = lane; b < nl; b += 64) {
+                t0 += load_published(part + PS * b);
+                if (nsum > 1) t1 += load_published(part + PS * b + 1);
+                tm = fmax(tm, load_published(part + PS * b + 2));
+            }
+            t0 = wave_sum(t0);
+            t1 = wave_sum(t1);
+            tm = wave_max(tm);
+            if (lane == 0) {
+                wl[48] = t0;
+                wl[49] = t1;
+                wl[50] = tm;
+            }
+        } else if (wv == 0) {
             double acc = 0.0, tm = 0.0;
             for (int base = 0; base < nl; base += 64) {
                 const int cnt = nl - base < 64 ? nl - base : 64;
@@ -333,7 +349,7 @@ struct Bottom {
                     if (ok) tm = fmax(tm, load_published(part + PS * (base + lane) + 2));
                 }
                 wave_lds_barrier();
-                if (lane < (nsum > 1 ? 2 : 1)) acc = chain_n(red + lane * 64, cnt16, acc);
+                if (lane < (nsum > 1 ? 2 : 1)) acc = chain_n<200>(red + lane * 64, cnt16, acc);
                 wave_lds_barrier();
             }
             tm = wave_max(tm);

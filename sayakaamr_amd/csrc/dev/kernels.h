@@ -5,6 +5,7 @@
 // the CPU oracle; only reductions (dot products) sum in a different, fixed order.
 #pragma once
 
+#include "chain_asm.h"
 #include "dev_common.h"
 
 namespace sydev {
@@ -85,72 +86,41 @@ struct RedBuf {
                         //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
 };
 
-// Lane l of the calling wave adds the N values staged for chain l onto s, in index order.  Chain
-// l occupies red[l*(N+2) .. l*(N+2)+N): the stride keeps every chain 16-byte aligned for
-// 128-bit LDS reads and puts concurrent chains on distinct LDS banks.  A sequential FP64 chain
-// is bound by the dependent-add latency (~10 cycles on MI355X); the chain is unrolled with two
-// windows of 2*W values in flight so that the LDS reads hide behind the adds (measured 11-13
-// cycles per add with W = 8, against 22 for a rolled loop with register rotation).  W = 4
-// keeps the register footprint of kernels that chain only at their end small.
-template <int N, int W = 8>
-__device__ inline double lane_chain(const double* red, int l, double s = 0.0) {
-    static_assert(N % (4 * W) == 0, "chain length must be a multiple of 4W");
-    const double2* p = reinterpret_cast<const double2*>(red + l * (N + 2));
-    double2 A[W], B[W];
-#pragma unroll
-    for (int u = 0; u < W; u++) A[u] = p[u];
-    // W = 8: full unroll up to 512 values (the 8^3 block); otherwise a rolled outer loop (no
-    // register rotation either way: A and B are reloaded in place)
-#pragma unroll((W == 8 && N <= 512) ? N / (4 * W) : 1)
-    for (int c = 0; c < N / 2; c += 2 * W) {
-#pragma unroll
-        for (int u = 0; u < W; u++) B[u] = p[c + W + u];
-#pragma unroll
-        for (int u = 0; u < W; u++) {
-            s += A[u].x;
-            s += A[u].y;
-        }
-        if (c + 2 * W < N / 2) {
-#pragma unroll
-            for (int u = 0; u < W; u++) A[u] = p[c + 2 * W + u];
-        }
-#pragma unroll
-        for (int u = 0; u < W; u++) {
-            s += B[u].x;
-            s += B[u].y;
-        }
-    }
+// LDS byte address of a pointer into __shared__ memory.
+__device__ inline unsigned lds_addr(const double* p) {
+    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) double*)p;
+}
+
+// Sequential sum of n (a multiple of 16) contiguous 16-byte aligned LDS values onto s, in index
+// order, as the assembly blocks of chain_asm.h (blocks of 512, 64 and 16 values; BASE = the
+// first of the 36 registers a block uses, 16 or 200).  A sequential FP64 sum is bound by its
+// dependent adds and by the LDS reads that feed them: 10.9 clocks per add on MI355X against 16.3
+// for the compiler's own code, which kept two or three reads ahead in register-heavy kernels
+// (3.7 us per 512 values inside k_bottom) and none in one build (7.6 us).
+template <int BASE = 16>
+__device__ inline double lds_chain(const double* p, int n, double s) {
+    const unsigned a = lds_addr(p);
+    int k = 0;
+    for (; k + 512 <= n; k += 512) s = lds_chain_asm<512, BASE>(a + 8u * k, s);
+    for (; k + 64 <= n; k += 64) s = lds_chain_asm<64, BASE>(a + 8u * k, s);
+    for (; k + 16 <= n; k += 16) s = lds_chain_asm<16, BASE>(a + 8u * k, s);
     return s;
 }
 
-// Sequential sum of n (a multiple of 16) contiguous 16-byte aligned LDS values onto acc, two
-// windows of 8 values in flight (the same scheme as lane_chain, for a run-time length).
+// Lane l of the calling wave adds the N values staged for chain l onto s, in index order.  Chain
+// l occupies red[l*(N+2) .. l*(N+2)+N): the stride keeps every chain 16-byte aligned and puts
+// concurrent chains on distinct LDS banks.
+template <int N, int BASE = 16>
+__device__ inline double lane_chain(const double* red, int l, double s = 0.0) {
+    static_assert(N % 16 == 0, "chain length must be a multiple of 16");
+    return lds_chain<BASE>(red + l * (N + 2), N, s);
+}
+
+// Sequential sum of n (a multiple of 16) contiguous 16-byte aligned LDS values onto acc.
+template <int BASE = 16>
 __device__ inline double chain_n(const double* p, int n, double acc) {
     if (n <= 0) return acc;
-    const double2* q = reinterpret_cast<const double2*>(p);
-    double2 A[4], B[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) A[u] = q[u];
-#pragma unroll 1
-    for (int c = 0; c < n / 2; c += 8) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) B[u] = q[c + 4 + u];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            acc += A[u].x;
-            acc += A[u].y;
-        }
-        if (c + 8 < n / 2) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) A[u] = q[c + 8 + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            acc += B[u].x;
-            acc += B[u].y;
-        }
-    }
-    return acc;
+    return lds_chain<BASE>(p, n, acc);
 }
 
 // LDS written by lanes of this wave becomes visible to the wave's other lanes (no workgroup
@@ -182,7 +152,7 @@ __device__ inline void seq_block_sums(const double (&v0)[CPT], const double (&v1
             if (nsum > 1) stage_chain(scratch, RUN, 1, threadIdx.x, v1[q]);
         }
         __syncthreads();
-        if ((int)threadIdx.x < nsum) r = lane_chain<RUN, 4>(scratch, threadIdx.x, r);
+        if ((int)threadIdx.x < nsum) r = lane_chain<RUN>(scratch, threadIdx.x, r);
     }
     __syncthreads();
     if (threadIdx.x == 1) scratch[0] = r;
