@@ -272,10 +272,16 @@ struct Bottom {
     // partials and forms the totals (in block order when exact).  a0/a1 are the thread's
     // running sums in fast mode.  Both sides of the barrier run in wave 0 alone after one
     // workgroup barrier, so the other waves only wait once on each side.
+    struct NoWork {
+        __device__ void operator()() const {}
+    };
+    // after_barrier(): run by every thread right after the grid barrier, beside wave 0's totals
+    // (loads of data the other workers published before the barrier, which need no totals)
+    template <class AfterBarrier = NoWork>
     __device__ static void reduce(unsigned long long* pht, int ph, BarState& bs,
                                   const BottomArgs& A, double a0, double a1,
                                   int nsum, double mxl, double* part, double* red, double* wl, int nvalid,
-                                  double& s0, double& s1, double& mx) {
+                                  double& s0, double& s1, double& mx, const AfterBarrier& after_barrier = AfterBarrier()) {
         const int w0 = bs.wid * BPW;
         const int nl = A.st.nlist;
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -317,6 +323,7 @@ struct Bottom {
         }
         SY_PH_MARK(pht, ph);
         grid_barrier(A, bs, (nl + BPW - 1) / BPW);
+        after_barrier();
         SY_PH_MARK(pht, ph + 1);
         // totals (wave 0).  Ordered sums: the partials staged in LDS by lane in chunks of 64 and
         // chained in block order in lanes 0 / 1.  Two-level variant (order-free): each lane adds
@@ -660,9 +667,9 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 B::publish_block_sums(bq, partA, wq, w0, nvalid);
             }
             SY_PH_MARK(pht, PH_OP_V);
-            B::reduce(pht, PH_A, bs, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx);
+            B::reduce(pht, PH_A, bs, A, a0, 0.0, 1, 0.0, partA, sh, wl, nvalid, s0v, s1, mx,
+                      [&]() { load_ghosts(vG, A.vpub); });
             if constexpr (PC) B::read_block_sums(partA, sv_, nl);
-            load_ghosts(vG, A.vpub);
             if (s0v == 0.0) { stat = 1; break; }
             alpha = rho / s0v;
             // s = r - alpha v ; x += alpha phat ; t = A shat ; |s|, t.s, t.t  (barrier B)
@@ -721,9 +728,9 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
                 B::publish_block_sums(bq, partC, wq, w0, nvalid);
             }
             SY_PH_MARK(pht, PH_UPD_R);
-            B::reduce(pht, PH_C, bs, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx);
+            B::reduce(pht, PH_C, bs, A, a0, 0.0, 1, mxl, partC, sh, wl, nvalid, s0v, s1, mx,
+                      [&]() { load_ghosts(rG, A.rpub); });
             if constexpr (PC) B::read_block_sums(partC, sr_, nl);
-            load_ghosts(rG, A.rpub);
             SY_PH_MARK(pht, PH_R_GHOST);
             if (mx < rnorm0 * A.tol_rel + A.tol_abs) { conv = 1; break; }
             if constexpr (PC) {
