@@ -467,29 +467,32 @@ def pmc_workload_large():
                           fill_write_bytes=n * spec.nb ** 3 * 8)), flush=True)
 
 
-def _read_pmc(dirs):
-    vals = {}   # (kernel, counter) -> [(dispatch order, grid size, KiB)]
+def _read_pmc(dirs, with_wg=False):
+    vals = {}   # (kernel, counter) -> [(dispatch order, grid size, KiB[, workgroup size])]
     for d in dirs:
         for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(path) as f:
                 for i, row in enumerate(csv.DictReader(f)):
                     k = (row.get("Kernel_Name", ""), row.get("Counter_Name", ""))
                     order = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or i)
-                    vals.setdefault(k, []).append((order, int(row.get("Grid_Size", 0) or 0),
-                                                   float(row["Counter_Value"])))
+                    rec = (order, int(row.get("Grid_Size", 0) or 0), float(row["Counter_Value"]))
+                    if with_wg:
+                        rec = rec + (int(row.get("Workgroup_Size", 0) or 0),)
+                    vals.setdefault(k, []).append(rec)
     return vals
 
 
 def summarize_pmc_large(dirs):
-    """Counter bytes against algorithmic bytes of the snapshot's FP64 operator and smoother:
-    per dispatch, blocks = grid size / 512 work-items, algorithmic bytes per block as coded in
-    AlgBytes (apply 3C + Fw, one colour pass (4C + Fw) / 2; C = 4096 x 8 B, Fw = 3 x 17 x 16 x 16
-    x 8 B), counter bytes scaled by the calibration kernels."""
+    """Counter bytes against algorithmic bytes of the snapshot's FP64 operator, FP64 colour pass
+    and FP32 hybrid sweep: per dispatch, blocks = grid size / workgroup size, algorithmic bytes
+    per block as coded in AlgBytes (apply 3C + Fw, one colour pass (4C + Fw) / 2, one FP32 sweep
+    4C + Fw (3C + Fw from zero); C = 4096 cells x 8 or 4 B, Fw = 3 x 17 x 16 x 16 faces x 8 or
+    4 B), counter bytes scaled by the calibration kernels."""
     C_ = 4096 * 8.0
     Fw = 3 * 17 * 16 * 16 * 8.0
-    vals = _read_pmc(dirs)
-    dot = [v for (k, c), lst in vals.items() if c == "FETCH_SIZE" and "k_vec<" in k for v in lst]
-    fill = [v for (k, c), lst in vals.items() if c == "WRITE_SIZE" and "k_vec<" in k for v in lst]
+    vals = _read_pmc(dirs, with_wg=True)
+    dot = [v[:3] for (k, c), lst in vals.items() if c == "FETCH_SIZE" and "k_vec<" in k for v in lst]
+    fill = [v[:3] for (k, c), lst in vals.items() if c == "WRITE_SIZE" and "k_vec<" in k for v in lst]
     dot.sort()
     fill.sort()
     # the last k_vec dispatch is the dot; the fill is the k_vec just before it
@@ -502,20 +505,23 @@ def summarize_pmc_large(dirs):
         kw = (gfill / 512) * 4096 * 8 / (fill[-2][2] * 1024 if len(fill) > 1 else fill[-1][2] * 1024)
         lines.append(f"calibration: dot over {gdot // 512} blocks -> FETCH factor {kf:.4f}; "
                      f"fill of {gfill // 512} blocks -> WRITE factor {kw:.4f}")
-    for name, mode, per_block in (("apply", 0, 3 * C_ + Fw), ("smoother colour pass", 2, 0.5 * (4 * C_ + Fw))):
-        kre = f"k_stencil<3, 16, {mode}, double>"
+    for name, kre, per_block in (("apply", "k_stencil<3, 16, 0, double>", 3 * C_ + Fw),
+                                 ("FP64 colour pass", "k_smooth<3, 16, 2, double>", 0.5 * (4 * C_ + Fw)),
+                                 ("FP32 hybrid sweep", "k_sweep32<3, 16, false>", 0.5 * (4 * C_ + Fw)),
+                                 ("FP32 hybrid sweep from zero", "k_sweep32<3, 16, true>", 0.5 * (3 * C_ + Fw))):
         fe = [v for (k, c), lst in vals.items() if c == "FETCH_SIZE" and kre in k for v in lst]
         wr = [v for (k, c), lst in vals.items() if c == "WRITE_SIZE" and kre in k for v in lst]
         if not fe or not wr or kf is None:
             continue
-        alg = sum(g / 512 * per_block for _, g, _ in fe)
-        fb = sum(v for _, _, v in fe) * 1024 * kf
-        wb = sum(v for _, _, v in wr) * 1024 * kw
-        cells = sum(g / 512 * 4096 for _, g, _ in fe)
+        nblk = lambda g, wg: g / (wg or 512)
+        alg = sum(nblk(g, wg) * per_block for _, g, _, wg in fe)
+        fb = sum(v[2] for v in fe) * 1024 * kf
+        wb = sum(v[2] for v in wr) * 1024 * kw
+        cells = sum(nblk(g, wg) * 4096 for _, g, _, wg in fe)
         lines.append(f"{name}: {len(fe)} dispatches, algorithmic {alg / cells:.2f} B/cell, counter read "
                      f"{fb / cells:.2f} B/cell + write {wb / cells:.2f} B/cell = {(fb + wb) / alg:.3f} x algorithmic")
     for (k, c), lst in sorted(vals.items()):
-        lines.append(f"  {c:10s} {np.mean([v for _, _, v in lst]):14.1f} KiB x{len(lst):4d}  {k[:90]}")
+        lines.append(f"  {c:10s} {np.mean([v[2] for v in lst]):14.1f} KiB x{len(lst):4d}  {k[:90]}")
     print("\n".join(lines))
 
 
@@ -539,7 +545,7 @@ def summarize_pmc(dirs):
         if not xs:
             return None, 0
         return (xs[-1][1], 1) if last else (float(np.mean([v for _, v in xs])), len(xs))
-    smooth_re = r"k_stencil<3, 8, 2, double>"
+    smooth_re = r"k_smooth<3, 8, 2, double>"
     fetch_s, nf = pick(smooth_re, "FETCH_SIZE")
     write_s, nw = pick(smooth_re, "WRITE_SIZE")
     fetch_dot, _ = pick(r"k_vec<", "FETCH_SIZE", grid=tw * 512, last=True)     # the final dot
@@ -728,7 +734,7 @@ def main():
                      "dominant_class": dom_name,
                      "note": "latency-bound: three ordered 512-term block sums and three grid barriers per "
                              "iteration (the reference's summation order); vectors stay in registers / LDS"},
-        "roofline_smoother": {"bound": "hbm", "kernel": "k_stencil<3, 8, 2, double> (FP64 red-black colour pass)",
+        "roofline_smoother": {"bound": "hbm", "kernel": "k_smooth<3, 8, 2, double> (FP64 red-black colour pass)",
                               "achieved": round(sm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(sm_gbs / HBM_PEAK_GBS, 4),
                               "traffic": None if traffic is None else traffic.get("smooth"),

@@ -139,9 +139,11 @@ k_restrict32(const int* list, const BlockInfo* binfo, const float* vf, const flo
 // between blocks).  A sweep is one launch that reads every operand once, against two
 // colour launches that each read every cache line, and the result does not depend on the
 // order in which workgroups run.  ZERO: the sweep starts from x = 0 (the first sweep of a
-// level's correction), so neither the block nor its halo is read.  out64 (optional): the
-// result also in FP64 (the last sweep of the cycle writes the preconditioned vector).
-// Per cell the arithmetic is k_smooth's (cell_op).
+// level's correction), so neither the block nor its halo is read.  RHS64 (with ZERO, the first
+// sweep of the cycle): the right-hand side is read in FP64 from rhs64 and stored in FP32 to
+// a.out2 for the rest of the cycle.  out64 (optional): the result also in FP64 (the last sweep
+// of the cycle writes the preconditioned vector).  Per cell the arithmetic is k_smooth's
+// (cell_op).
 // Threads of a k_sweep32 workgroup: 16^3 blocks take 1 024 (two updates per thread and colour,
 // 62 VGPRs, 8 waves per SIMD); 512 threads held four updates per colour in flight at 112 VGPRs
 // (4 waves per SIMD) and ran the 139 M-cell sweep at 3.04 against 3.37 TB/s.
@@ -149,10 +151,21 @@ template <int D, int NB>
 struct SweepCfg {
     static constexpr int NT = NB >= 16 ? 1024 : Geo<D, NB>::THREADS;
 };
-template <int D, int NB, bool ZERO>
-__global__ void __launch_bounds__((SweepCfg<D, NB>::NT)) k_sweep32(StencilArgsT<float> a, double* out64) {
+template <int D, int NB, bool ZERO, bool RHS64 = false>
+__global__ void __launch_bounds__((SweepCfg<D, NB>::NT))
+k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64) {
     using G = Geo<D, NB>;
     using T = float;
+    static_assert(ZERO || !RHS64, "the FP64 right-hand side is read by the first sweep");
+    // cell inputs without the right-hand side when it comes from rhs64
+    constexpr int LMODE = RHS64 ? ST_APPLY : ST_SMOOTH;
+    auto load_in = [&](CellIn<T>& ci, int b, int c) {
+        load_cell<D, NB, LMODE, T>(ci, a, b, c, true);
+        if (RHS64) {
+            ci.rhs = (T)rhs64[b * G::NCELL + c];
+            a.out2[b * G::NCELL + c] = ci.rhs;
+        }
+    };
     static_assert(NB % 2 == 0, "even block edge");
     constexpr int NT = SweepCfg<D, NB>::NT;
     constexpr int CPT = (G::NCELL + NT - 1) / NT;
@@ -177,7 +190,7 @@ __global__ void __launch_bounds__((SweepCfg<D, NB>::NT)) k_sweep32(StencilArgsT<
             if (ia < NACT) {
                 int i, j, k;
                 colour_cell<NB>(ia, col, i, j, k);
-                load_cell<D, NB, ST_SMOOTH, T>(in[col], a, b, G::cidx(i, j, k), true);
+                load_in(in[col], b, G::cidx(i, j, k));
             }
         }
     }
@@ -222,7 +235,7 @@ __global__ void __launch_bounds__((SweepCfg<D, NB>::NT)) k_sweep32(StencilArgsT<
             CellIn<T> tmp;
             CellIn<T>& ci = PRE ? in[col] : tmp;
             if (!PRE) {
-                load_cell<D, NB, ST_SMOOTH, T>(ci, a, b, G::cidx(i, j, k), true);
+                load_in(ci, b, G::cidx(i, j, k));
                 load_wcf<D, NB, T>(ci, a, b, G::cidx(i, j, k), bi);
             }
             const T u = cell_op<D, NB, true, T>(tile, ci.w, ci.wcf, i, j, k, bi, a.m, a.alpha, dv, ci.vf, ci.rhs);
@@ -328,13 +341,14 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_resid_restrict32(Sten
     }
 }
 
-// The coarse correction on tower T(l): u_l += u_{l-1} on the blocks below level l (the leaves
-// T(l-1) shares), += the injection of the parent's u_{l-1} (zero where vf = 0) on the blocks
-// of level l: k_mix's M_ADD and M_PROLONG_ADD in one launch over T(l).
+// The coarse correction on tower T(l): out = u_l + u_{l-1} on the blocks below level l (the
+// leaves T(l-1) shares), u_l + the injection of the parent's u_{l-1} (nothing where vf = 0) on
+// the blocks of level l: k_mix's M_ADD and M_PROLONG_ADD in one launch over T(l).  out may be
+// u (in place) or the other buffer of the double-buffered sweeps that follow.
 template <int D, int NB>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
-k_correct32(const int* list, const BlockInfo* __restrict__ binfo, const float* __restrict__ vf, float* u,
-            const float* __restrict__ uc, int l) {
+k_correct32(const int* list, const BlockInfo* __restrict__ binfo, const float* __restrict__ vf, const float* u,
+            const float* __restrict__ uc, float* out, int l) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
     const BlockInfo& bi = binfo[b];
@@ -343,15 +357,17 @@ k_correct32(const int* list, const BlockInfo* __restrict__ binfo, const float* _
     const int p = bi.parent, wc = bi.which_child;
     for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
         const int gi = b * G::NCELL + c;
+        float v = u[gi];
         if (below) {
-            u[gi] += uc[gi];
+            v += uc[gi];
         } else {
             int i, j, k;
             G::cell_ijk(c, i, j, k);
             const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
                                    D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
-            if (vf[gi] > 0.0f) u[gi] += uc[p * G::NCELL + pc];
+            if (vf[gi] > 0.0f) v += uc[p * G::NCELL + pc];
         }
+        out[gi] = v;
     }
 }
 

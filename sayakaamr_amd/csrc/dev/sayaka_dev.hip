@@ -715,51 +715,63 @@ struct Impl {
         HIPCHK(hipGetLastError());
         return 0;
     }
-    // n hybrid sweeps (k_sweep32) on T(m), alternating between u and t32: u holds the initial
-    // value (not read when from_zero) and receives the result.  out64 (may be null) receives
-    // the result in FP64 too; returns 1 when the last sweep wrote it, 0 when the caller must.
-    static int sweeps32(sayaka_dev_ctx* c, int m, float* u, const float* f, int n, bool from_zero, double* out64,
-                        int* wrote64) {
+    // n hybrid sweeps (k_sweep32) on T(m), alternating between u and t32 so that the last one
+    // writes u: the initial value (not read when from_zero) is in sweep_start(u, n).  out64
+    // (may be null) receives the result in FP64 too; *wrote64 = 1 when the last sweep wrote it.
+    static float* sweep_start(sayaka_dev_ctx* c, float* u, int n) { return (n & 1) ? c->mixed.t32 : u; }
+    // rhs64 (from_zero only): the first sweep reads the right-hand side in FP64 and stores f.
+    static int sweeps32(sayaka_dev_ctx* c, int m, float* u, float* f, int n, bool from_zero, double* out64,
+                        int* wrote64, const double* rhs64 = nullptr) {
         *wrote64 = 0;
         ListRef T = c->tw[m];
         if (T.n <= 0 || n <= 0) return 0;
-        float* cur = u;
-        float* nxt = c->mixed.t32;
+        float* cur = sweep_start(c, u, n);
+        float* nxt = cur == u ? c->mixed.t32 : u;
         for (int s = 0; s < n; s++) {
             const bool zero = s == 0 && from_zero;
             StencilArgsT<float> a = args32(c, T, m);
             a.x = zero ? f : cur;   // not read from zero; any valid array
             a.out = nxt;
             a.rhs = f;
-            double* o64 = (s == n - 1 && nxt == u) ? out64 : nullptr;
-            const double bytes = 2 * AlgBytes<D, NB, float>::stencil(ST_SMOOTH) - (zero ? AlgBytes<D, NB, float>::C : 0.0);
+            a.out2 = f;
+            const double* r64 = zero ? rhs64 : nullptr;
+            double* o64 = s == n - 1 ? out64 : nullptr;
+            using AB = AlgBytes<D, NB, float>;
+            // a sweep reads x, rhs, vf, w and writes x; from zero x is not read; the FP64
+            // right-hand side adds 4 B read and 4 B written per cell
+            const double bytes = 2 * AB::stencil(ST_SMOOTH) - (zero ? AB::C : 0.0) + (r64 ? 2 * AB::C : 0.0);
             ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL, T.n * bytes, true);
             constexpr int SNT = SweepCfg<D, NB>::NT;
-            if (zero)
+            if (zero && r64)
+                hipExtLaunchKernelGGL((k_sweep32<D, NB, true, true>), dim3(T.n), dim3(SNT), 0, c->stream,
+                                      ps.start(), ps.stop(), 0, a, o64, r64);
+            else if (zero)
                 hipExtLaunchKernelGGL((k_sweep32<D, NB, true>), dim3(T.n), dim3(SNT), 0, c->stream, ps.start(),
-                                      ps.stop(), 0, a, o64);
+                                      ps.stop(), 0, a, o64, nullptr);
             else
                 hipExtLaunchKernelGGL((k_sweep32<D, NB, false>), dim3(T.n), dim3(SNT), 0, c->stream, ps.start(),
-                                      ps.stop(), 0, a, o64);
+                                      ps.stop(), 0, a, o64, nullptr);
             if (o64) *wrote64 = 1;
             std::swap(cur, nxt);
         }
         HIPCHK(hipGetLastError());
-        if (cur != u) return mix(c, T, M_COPY, u, cur, nullptr, nullptr);
         return 0;
     }
     // nu sweeps of the configured smoother on T(l) for u_l (from zero when from_zero)
-    static int smooth_level(sayaka_dev_ctx* c, int l, int nu, bool from_zero, double* out64, int* wrote64) {
+    static int smooth_level(sayaka_dev_ctx* c, int l, int nu, bool from_zero, double* out64, int* wrote64,
+                            const double* rhs64 = nullptr) {
         MixedState& ms = c->mixed;
         *wrote64 = 0;
-        if (ms.merged_smooth == 2 && nu > 0) return sweeps32(c, l, ms.u[l], ms.f[l], nu, from_zero, out64, wrote64);
+        if (ms.merged_smooth == 2 && nu > 0)
+            return sweeps32(c, l, ms.u[l], ms.f[l], nu, from_zero, out64, wrote64, rhs64);
         int e;
         if (from_zero && (e = mix(c, c->tw[l], M_ZERO, ms.u[l], nullptr, nullptr, nullptr))) return e;
         for (int s = 0; s < nu; s++)
             if ((e = smooth32(c, l, ms.u[l], ms.f[l]))) return e;
         return 0;
     }
-    static int vrec(sayaka_dev_ctx* c, int l, int pre, int post, double* out64) {
+    // rhs64 (the top level with hybrid pre-sweeps): f_l still in FP64 there, read by the first sweep
+    static int vrec(sayaka_dev_ctx* c, int l, int pre, int post, double* out64, const double* rhs64 = nullptr) {
         MixedState& ms = c->mixed;
         int e, wrote64 = 0;
         if (l == 1) {
@@ -772,7 +784,7 @@ struct Impl {
             return out64 ? mix(c, c->tw[1], M_F32_TO_F64, nullptr, ms.u[1], out64, nullptr) : 0;
         }
         ListRef T = c->tw[l];
-        if ((e = smooth_level(c, l, pre, true, nullptr, &wrote64))) return e;
+        if ((e = smooth_level(c, l, pre, true, nullptr, &wrote64, rhs64))) return e;
         if (T.n > 0) {
             // f_{l-1} from r = f_l - A_l u_l: copied on the shared leaves, restricted onto the
             // non-leaf blocks of level l-1 (every child of which is a block of T(l))
@@ -784,14 +796,17 @@ struct Impl {
             HIPCHK(hipGetLastError());
         }
         if ((e = vrec(c, l - 1, pre, post, nullptr))) return e;
-        if (T.n > 0) {
-            hipLaunchKernelGGL((k_correct32<D, NB>), dim3(T.n), dim3(G::THREADS), 0, c->stream, dlist(c, T),
-                               c->d_binfo, ms.vf32, ms.u[l], ms.u[l - 1], l);
-            HIPCHK(hipGetLastError());
-        }
         // hybrid sweeps: one post-sweep more than asked (the stale block boundaries; on the moving
         // 16^3-block cycle V(2,3) hybrid takes the exact V(2,2)'s outer iterations, V(2,2) one more)
-        const int npost = post + (ms.merged_smooth == 2 && post > 0 ? 1 : 0);
+        const bool hybrid = ms.merged_smooth == 2 && post > 0;
+        const int npost = post + (hybrid ? 1 : 0);
+        if (T.n > 0) {
+            // the corrected u goes where the sweeps start (the other buffer for an odd count)
+            float* dst = hybrid ? sweep_start(c, ms.u[l], npost) : ms.u[l];
+            hipLaunchKernelGGL((k_correct32<D, NB>), dim3(T.n), dim3(G::THREADS), 0, c->stream, dlist(c, T),
+                               c->d_binfo, ms.vf32, ms.u[l], ms.u[l - 1], dst, l);
+            HIPCHK(hipGetLastError());
+        }
         if ((e = smooth_level(c, l, npost, false, out64, &wrote64))) return e;
         if (out64 && !wrote64) return mix(c, T, M_F32_TO_F64, nullptr, ms.u[l], out64, nullptr);
         return 0;
@@ -799,8 +814,10 @@ struct Impl {
     static int mixed_vcycle(sayaka_dev_ctx* c, int L, const double* resid64, double* phat64, int pre, int post) {
         MixedState& ms = c->mixed;
         int e;
-        if ((e = mix(c, c->tw[L], M_F64_TO_F32, ms.f[L], nullptr, nullptr, resid64))) return e;
-        return vrec(c, L, pre, post, phat64);
+        // hybrid pre-sweeps read the FP64 residual themselves (and store f_L); otherwise convert
+        const bool fused = ms.merged_smooth == 2 && pre > 0 && L >= 2;
+        if (!fused && (e = mix(c, c->tw[L], M_F64_TO_F32, ms.f[L], nullptr, nullptr, resid64))) return e;
+        return vrec(c, L, pre, post, phat64, fused ? resid64 : nullptr);
     }
 
     // Coarse/fine synchronisation of one or two face arrays [blk][D][nface] (fd1 may be null)
