@@ -312,7 +312,7 @@ __device__ inline int face_type(const BlockInfo& bi, int f, int m) {
 // comes from the table (BlockInfo::nfc), so no dependent table load is needed.
 template <int D, int NB, class T>
 __device__ inline T cf_sum(int f, int t1, int t2, const BlockInfo& bi, const T* __restrict__ x,
-                           const T* __restrict__ w) {
+                           const T* __restrict__ w, const T* __restrict__ xslab = nullptr) {
     using G = Geo<D, NB>;
     const int d = f >> 1, s = f & 1;
     int a1, a2;
@@ -331,7 +331,8 @@ __device__ inline T cf_sum(int f, int t1, int t2, const BlockInfo& bi, const T* 
             c[a1] = fq[a1] = (t1 - s1 * half) * 2 + q1;
             if (D == 3) c[a2] = fq[a2] = (t2 - s2 * half) * 2 + q2;
             const T wv = w[(fb * D + d) * G::NFACE + G::fidx(d, fq[0], fq[1], fq[2])];
-            const T ph = x[fb * G::NCELL + G::cidx(c[0], c[1], c[2])];
+            const T ph = (d == 0 && xslab) ? xslab[(fb * 2 + (s ? 0 : 1)) * G::NSLAB + c[1] + NB * c[2]]
+                                           : x[fb * G::NCELL + G::cidx(c[0], c[1], c[2])];
             acc += wv * ph;
         }
     }
@@ -344,9 +345,12 @@ __device__ inline T cf_sum(int f, int t1, int t2, const BlockInfo& bi, const T* 
 // SayakaTreeData.h:41-44).  Finer neighbour on tower m (a coarse/fine face seen from the
 // coarse side): the sum of the fine w*phi over the face cell (cf_sum), which is all the
 // operator needs from that side.  Walls: 0 (walls enter through the weights).
+// xslab (optional): every block's x-boundary columns of x, [block][side i = 0 / NB-1][j + NB k]
+// (written by the kernel that wrote x), read for the x faces instead of x itself: a column of
+// x is one cache line per cell, the slab one contiguous run.
 template <int D, int NB, class T, int NTH = Geo<D, NB>::THREADS>
 __device__ inline void load_halo(T* tile, const T* __restrict__ x, const T* __restrict__ w, const BlockInfo& bi,
-                                 int m) {
+                                 int m, const T* __restrict__ xslab = nullptr) {
     using G = Geo<D, NB>;
     // stride by the compile-time thread count: blockDim is read from the dispatch packet,
     // one scalar load per wave that measured ~14 us per launch on MI355X
@@ -363,12 +367,13 @@ __device__ inline void load_halo(T* tile, const T* __restrict__ x, const T* __re
         gq[d] = s ? ext : -1;
         const int n = bi.nbr[f];
         T v = T(0);
+        const bool sl = d == 0 && xslab;
         if (n >= 0) {
             if (bi.level < m && !bi.nbr_leaf[f]) {
-                v = cf_sum<D, NB, T>(f, q[a1], D == 3 ? q[a2] : 0, bi, x, w);
+                v = cf_sum<D, NB, T>(f, q[a1], D == 3 ? q[a2] : 0, bi, x, w, xslab);
             } else {
                 q[d] = s ? 0 : ext - 1;
-                v = x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
+                v = sl ? xslab[(n * 2 + (s ? 0 : 1)) * G::NSLAB + p] : x[n * G::NCELL + G::cidx(q[0], q[1], q[2])];
             }
         } else if (n > PHYS_BNDRY) {
             const int cb = bi.cnbr[f];
@@ -377,7 +382,8 @@ __device__ inline void load_halo(T* tile, const T* __restrict__ x, const T* __re
             r[d] = s ? 0 : ext - 1;
             r[a1] = (q[a1] >> 1) + side_of(wc, a1) * (NB / 2);
             if (D == 3) r[a2] = (q[a2] >> 1) + side_of(wc, a2) * (NB / 2);
-            v = x[cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
+            v = sl ? xslab[(cb * 2 + (s ? 0 : 1)) * G::NSLAB + r[1] + NB * r[2]]
+                   : x[cb * G::NCELL + G::cidx(r[0], r[1], r[2])];
         }
         tile[G::tidx(gq[0], gq[1], gq[2])] = v;
     }

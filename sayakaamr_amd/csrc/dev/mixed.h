@@ -49,6 +49,9 @@ struct MixedState {
     int merged_smooth = 2;
     float* u[MIX_MAXLV + 1] = {};
     float* f[MIX_MAXLV + 1] = {};
+    // x-boundary columns of u[l] and of t32 (hybrid sweeps; load_halo's xslab)
+    float* xs[MIX_MAXLV + 1] = {};
+    float* xst = nullptr;
     // The reference's bottom test (1e-6 relative + 1e-6 absolute) stops the coarse solve
     // early once the outer residual is small, which leaves the cycle without a coarse
     // correction late in the solve and can stall the outer BiCGSTAB (20 iterations instead of
@@ -58,11 +61,14 @@ struct MixedState {
 };
 
 inline void mixed_free(MixedState& ms) {
-    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32, ms.t32};
+    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32, ms.t32, ms.xst};
     for (float* p : ptrs) if (p) (void)hipFree(p);
+    ms.xst = nullptr;
     for (int l = 0; l <= MIX_MAXLV; l++) {
         if (ms.u[l]) (void)hipFree(ms.u[l]);
         if (ms.f[l]) (void)hipFree(ms.f[l]);
+        if (ms.xs[l]) (void)hipFree(ms.xs[l]);
+        ms.xs[l] = nullptr;
         ms.u[l] = ms.f[l] = nullptr;
     }
     ms.w32 = ms.wcf32 = ms.vf32 = ms.r32 = ms.t32 = nullptr;
@@ -133,6 +139,17 @@ k_restrict32(const int* list, const BlockInfo* binfo, const float* vf, const flo
 }
 
 
+// The block's x-boundary columns (i = 0, i = NB-1) from its LDS tile into the slab array
+// [block][side][j + NB k] that load_halo reads for x faces.
+template <int D, int NB, int NT>
+__device__ inline void store_xslab(const float* tile, float* xs, int b) {
+    using G = Geo<D, NB>;
+    for (int t = threadIdx.x; t < 2 * G::NSLAB; t += NT) {
+        const int side = t / G::NSLAB, p = t % G::NSLAB;
+        xs[(b * 2 + side) * G::NSLAB + p] = tile[G::tidx(side ? NB - 1 : 0, p % NB, p / NB)];
+    }
+}
+
 // One hybrid Gauss-Seidel sweep of one block (FP32, x -> out, double-buffered): both colours
 // of the block in LDS, red first, then black from the updated red; the one-cell halo holds the
 // neighbours' values from before the launch for both colours (block Gauss-Seidel, Jacobi
@@ -153,7 +170,8 @@ struct SweepCfg {
 };
 template <int D, int NB, bool ZERO, bool RHS64 = false>
 __global__ void __launch_bounds__((SweepCfg<D, NB>::NT))
-k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64) {
+k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64, const float* __restrict__ xsin,
+          float* xsout) {
     using G = Geo<D, NB>;
     using T = float;
     static_assert(ZERO || !RHS64, "the FP64 right-hand side is read by the first sweep");
@@ -208,7 +226,7 @@ k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64
             G::cell_ijk(c, i, j, k);
             tile[G::tidx(i, j, k)] = xo[q];
         }
-        load_halo<D, NB, T, NT>(tile, a.x, a.w, bi, a.m);
+        load_halo<D, NB, T, NT>(tile, a.x, a.w, bi, a.m, xsin);
     }
     if (PRE) {
         const int ia = threadIdx.x;
@@ -253,6 +271,7 @@ k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64
         a.out[b * G::NCELL + c] = v;
         if (out64) out64[b * G::NCELL + c] = (double)v;
     }
+    if (xsout) store_xslab<D, NB, NT>(tile, xsout, b);
 }
 
 // r = f - A u on tower T(l) (FP32), handed straight to level l-1's right-hand side: a block
@@ -261,7 +280,8 @@ k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64
 // the same order.  Replaces the residual launch, the copy and the restriction (and their
 // re-reads of r).  a.x = u, a.rhs = f; fc = f_{l-1}.
 template <int D, int NB>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_resid_restrict32(StencilArgsT<float> a, float* fc, int l) {
+__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
+k_resid_restrict32(StencilArgsT<float> a, float* fc, int l, const float* __restrict__ xsin) {
     using G = Geo<D, NB>;
     using T = float;
     constexpr int NT = G::THREADS;
@@ -295,7 +315,7 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_resid_restrict32(Sten
         G::cell_ijk(c, i, j, k);
         tile[G::tidx(i, j, k)] = xo[q];
     }
-    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m);
+    load_halo<D, NB, T>(tile, a.x, a.w, bi, a.m, xsin);
     if (PRE && (int)threadIdx.x < G::NCELL) load_wcf<D, NB, T>(in[0], a, b, threadIdx.x, bi);
     __syncthreads();
     const bool below = bi.level < l;   // uniform over the workgroup
@@ -348,7 +368,7 @@ __global__ void __launch_bounds__((Geo<D, NB>::THREADS)) k_resid_restrict32(Sten
 template <int D, int NB>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
 k_correct32(const int* list, const BlockInfo* __restrict__ binfo, const float* __restrict__ vf, const float* u,
-            const float* __restrict__ uc, float* out, int l) {
+            const float* __restrict__ uc, float* out, int l, float* xsout) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
     const BlockInfo& bi = binfo[b];
@@ -368,6 +388,8 @@ k_correct32(const int* list, const BlockInfo* __restrict__ binfo, const float* _
             if (vf[gi] > 0.0f) v += uc[p * G::NCELL + pc];
         }
         out[gi] = v;
+        const int i = c % NB;
+        if (xsout && (i == 0 || i == NB - 1)) xsout[(b * 2 + (i ? 1 : 0)) * G::NSLAB + c / NB] = v;
     }
 }
 

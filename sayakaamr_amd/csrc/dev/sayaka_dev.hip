@@ -653,9 +653,12 @@ struct Impl {
             HIPCHK(hipMalloc(&ms.vf32, cells * sizeof(float)));
             HIPCHK(hipMalloc(&ms.r32, cells * sizeof(float)));
             HIPCHK(hipMalloc(&ms.t32, cells * sizeof(float)));
+            const size_t slabs = (size_t)c->cap * 2 * G::NSLAB;
+            HIPCHK(hipMalloc(&ms.xst, slabs * sizeof(float)));
             for (int l = 1; l <= c->maxlevel; l++) {
                 HIPCHK(hipMalloc(&ms.u[l], cells * sizeof(float)));
                 HIPCHK(hipMalloc(&ms.f[l], cells * sizeof(float)));
+                HIPCHK(hipMalloc(&ms.xs[l], slabs * sizeof(float)));
             }
             ms.cap_cells = cells;
             ms.cap_faces = faces;
@@ -727,6 +730,8 @@ struct Impl {
         if (T.n <= 0 || n <= 0) return 0;
         float* cur = sweep_start(c, u, n);
         float* nxt = cur == u ? c->mixed.t32 : u;
+        // the x-boundary columns of a buffer: t32's, or those of u = u[m]
+        auto slab = [&](const float* buf) { return buf == c->mixed.t32 ? c->mixed.xst : c->mixed.xs[m]; };
         for (int s = 0; s < n; s++) {
             const bool zero = s == 0 && from_zero;
             StencilArgsT<float> a = args32(c, T, m);
@@ -742,15 +747,17 @@ struct Impl {
             const double bytes = 2 * AB::stencil(ST_SMOOTH) - (zero ? AB::C : 0.0) + (r64 ? 2 * AB::C : 0.0);
             ProfScope ps(c, SAYAKA_KCLASS_SMOOTH32, (long long)T.n * G::NCELL, T.n * bytes, true);
             constexpr int SNT = SweepCfg<D, NB>::NT;
+            const float* xsin = zero ? nullptr : slab(cur);
+            float* xsout = slab(nxt);
             if (zero && r64)
                 hipExtLaunchKernelGGL((k_sweep32<D, NB, true, true>), dim3(T.n), dim3(SNT), 0, c->stream,
-                                      ps.start(), ps.stop(), 0, a, o64, r64);
+                                      ps.start(), ps.stop(), 0, a, o64, r64, xsin, xsout);
             else if (zero)
                 hipExtLaunchKernelGGL((k_sweep32<D, NB, true>), dim3(T.n), dim3(SNT), 0, c->stream, ps.start(),
-                                      ps.stop(), 0, a, o64, nullptr);
+                                      ps.stop(), 0, a, o64, nullptr, xsin, xsout);
             else
                 hipExtLaunchKernelGGL((k_sweep32<D, NB, false>), dim3(T.n), dim3(SNT), 0, c->stream, ps.start(),
-                                      ps.stop(), 0, a, o64, nullptr);
+                                      ps.stop(), 0, a, o64, nullptr, xsin, xsout);
             if (o64) *wrote64 = 1;
             std::swap(cur, nxt);
         }
@@ -791,8 +798,10 @@ struct Impl {
             StencilArgsT<float> a = args32(c, T, l);
             a.x = ms.u[l];
             a.rhs = ms.f[l];
+            // the slab of u_l is current only after hybrid pre-sweeps
+            const float* xsin = ms.merged_smooth == 2 && pre > 0 ? ms.xs[l] : nullptr;
             hipLaunchKernelGGL((k_resid_restrict32<D, NB>), dim3(T.n), dim3(G::THREADS), 0, c->stream, a,
-                               ms.f[l - 1], l);
+                               ms.f[l - 1], l, xsin);
             HIPCHK(hipGetLastError());
         }
         if ((e = vrec(c, l - 1, pre, post, nullptr))) return e;
@@ -803,8 +812,9 @@ struct Impl {
         if (T.n > 0) {
             // the corrected u goes where the sweeps start (the other buffer for an odd count)
             float* dst = hybrid ? sweep_start(c, ms.u[l], npost) : ms.u[l];
+            float* xsout = hybrid ? (dst == ms.t32 ? ms.xst : ms.xs[l]) : nullptr;
             hipLaunchKernelGGL((k_correct32<D, NB>), dim3(T.n), dim3(G::THREADS), 0, c->stream, dlist(c, T),
-                               c->d_binfo, ms.vf32, ms.u[l], ms.u[l - 1], dst, l);
+                               c->d_binfo, ms.vf32, ms.u[l], ms.u[l - 1], dst, l, xsout);
             HIPCHK(hipGetLastError());
         }
         if ((e = smooth_level(c, l, npost, false, out64, &wrote64))) return e;
