@@ -256,9 +256,10 @@ int sayaka_dev_set_mixed_bottom(sayaka_dev_ctx* ctx, double tol_rel, double tol_
 int sayaka_dev_set_mixed_coarse(sayaka_dev_ctx* ctx, int two_level);
 
 /* Smoother of the mixed solve's V-cycle: 2 = hybrid sweeps, one launch per sweep over the
- * whole tower, each block red then black in LDS with its halo as it was before the launch
- * (the default for 16^3 blocks); 1 = exact red-black, each colour in one launch over every
- * sub-level of the tower, double-buffered (the default for smaller blocks); 0 = exact
+ * whole tower, each block red then black in LDS with its halo as it was before the launch,
+ * with post-sweeps beyond sayaka_solve_params.post_smooth (one more on 16^3 blocks, three more
+ * on smaller ones) for the stale block boundaries (the default); 1 = exact red-black, each
+ * colour in one launch over every sub-level of the tower, double-buffered; 0 = exact
  * red-black, one launch per sub-level and colour in place, in the reference's sub-level
  * order.  No reference counterpart. */
 int sayaka_dev_set_mixed_smoother(sayaka_dev_ctx* ctx, int merged);

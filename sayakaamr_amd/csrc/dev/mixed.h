@@ -20,9 +20,10 @@
 // straight to f_{l-1} (copy on the shared leaves, restriction onto the parents), and one
 // launch adds the coarse correction.  Each level keeps its own u and f (the towers share
 // their coarse leaves, so one in-place array as in the reference cannot hold both
-// corrections).  Exact red-black colour passes (two launches per sweep) are the default for
-// blocks smaller than 16^3: there the hybrid sweep's stale block boundaries (more than half the
-// cells of an 8^3 block) cost the solve an outer iteration.
+// corrections).  The stale block boundaries take away post-smoothing, so the hybrid cycle runs
+// more post-sweeps than asked (one more on 16^3 blocks, three more on 8^3 blocks, where more than
+// half the cells sit on a boundary).  Exact red-black colour passes (two launches per sweep)
+// remain selectable.
 #pragma once
 
 #include "bottom.h"
@@ -42,10 +43,9 @@ struct MixedState {
     float* vf32 = nullptr;
     float* r32 = nullptr;
     float* t32 = nullptr;       // second buffer of the double-buffered smoother
-    // 2: hybrid sweeps (k_sweep32, one launch per sweep over the tower; the default for 16^3
-    // blocks); 1: exact red-black, one launch per colour over the tower, double-buffered (the
-    // default for smaller blocks); 0: exact red-black, one launch per sub-level and colour in
-    // place (the reference's order)
+    // 2: hybrid sweeps (k_sweep32, one launch per sweep over the tower; the default);
+    // 1: exact red-black, one launch per colour over the tower, double-buffered;
+    // 0: exact red-black, one launch per sub-level and colour in place (the reference's order)
     int merged_smooth = 2;
     float* u[MIX_MAXLV + 1] = {};
     float* f[MIX_MAXLV + 1] = {};

@@ -805,10 +805,12 @@ struct Impl {
             HIPCHK(hipGetLastError());
         }
         if ((e = vrec(c, l - 1, pre, post, nullptr))) return e;
-        // hybrid sweeps: one post-sweep more than asked (the stale block boundaries; on the moving
-        // 16^3-block cycle V(2,3) hybrid takes the exact V(2,2)'s outer iterations, V(2,2) one more)
+        // hybrid sweeps: post-sweeps beyond those asked for make up for the stale block boundaries
+        // (measured, mixed solve from zero: 16^3 blocks, 13 steps of the moving cycle: hybrid V(2,2)
+        // 149 ms, V(2,3) 134-139, V(2,4) 132-134, V(2,5) 142, exact V(2,2) 170; 8^3 blocks, the 3D
+        // benchmark: V(2,3) 5.19 ms, V(2,4) 4.95, V(2,5) 4.75, V(2,6) 4.90, exact V(2,2) 5.80)
         const bool hybrid = ms.merged_smooth == 2 && post > 0;
-        const int npost = post + (hybrid ? 1 : 0);
+        const int npost = post + (hybrid ? (NB >= 16 ? 1 : 3) : 0);
         if (T.n > 0) {
             // the corrected u goes where the sweeps start (the other buffer for an odd count)
             float* dst = hybrid ? sweep_start(c, ms.u[l], npost) : ms.u[l];
@@ -1287,10 +1289,6 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     c->device = device;
     c->ndim = ndim;
     c->nb = nb;
-    // hybrid sweeps where blocks are large (16^3: same outer iterations, a third less smoother
-    // time); exact red-black colour passes on 8^3 blocks, where the hybrid sweep's stale block
-    // boundaries cost an outer iteration
-    c->mixed.merged_smooth = nb >= 16 ? 2 : 1;
     c->cap = max_blocks;
     const int nbz = ndim == 3 ? nb : 1;
     c->ncell = nb * nb * nbz;
