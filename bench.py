@@ -13,7 +13,7 @@ Reported (one JSON line on stdout):
   value            plain FP64 ms/solve -- the reference's algorithm and summation order, bit-
                    identical to the reference's CPU solve;
   fp64_fast        the same algorithm with order-free sums (rounding differs);
-  mixed            FP32 V(2,2)-cycle inside the FP64 BiCGSTAB, 5 seeded right-hand sides,
+  mixed            FP32 V-cycle (hybrid sweeps) inside the FP64 BiCGSTAB, 5 seeded right-hand sides,
                    error against the FP64 solution per seed;
   learned          the learned initial guess on held-out random-sphere / blob problems
                    (zero guess vs guess-with-fallback, mixed solver; FP64 beside it);
@@ -715,7 +715,7 @@ def main():
         "mixed": None if ms_mixed is None else {
                   "ms_per_solve": round(ms_mixed, 3),
                   "speedup_vs_fp64": round(ms64 / ms_mixed, 3),
-                  "dtype": "f32 V(2,2)-cycle, f64 outer BiCGSTAB and coarsest solve",
+                  "dtype": "f32 V-cycle (hybrid block Gauss-Seidel sweeps), f64 outer BiCGSTAB and coarsest solve",
                   "coarse_iters_per_call": round(rmx["bottom_iters"] / max(rmx["bottom_calls"], 1), 1),
                   "seeds": mixed_rows, "kernels": classes_mixed},
         "learned": learned,
