@@ -486,8 +486,9 @@ def summarize_pmc_large(dirs):
     """Counter bytes against algorithmic bytes of the snapshot's FP64 operator, FP64 colour pass
     and FP32 hybrid sweep: per dispatch, blocks = grid size / workgroup size, algorithmic bytes
     per block as coded in AlgBytes (apply 3C + Fw, one colour pass (4C + Fw) / 2, one FP32 sweep
-    4C + Fw (3C + Fw from zero); C = 4096 cells x 8 or 4 B, Fw = 3 x 17 x 16 x 16 faces x 8 or
-    4 B), counter bytes scaled by the calibration kernels."""
+    4C + Fw (3C + Fw from zero, 5C + Fw from zero with the FP64 rhs read and the FP32 one
+    stored); C = 4096 cells x 8 or 4 B, Fw = 3 x 17 x 16 x 16 faces x 8 or 4 B), counter bytes
+    scaled by the calibration kernels."""
     C_ = 4096 * 8.0
     Fw = 3 * 17 * 16 * 16 * 8.0
     vals = _read_pmc(dirs, with_wg=True)
@@ -507,8 +508,10 @@ def summarize_pmc_large(dirs):
                      f"fill of {gfill // 512} blocks -> WRITE factor {kw:.4f}")
     for name, kre, per_block in (("apply", "k_stencil<3, 16, 0, double>", 3 * C_ + Fw),
                                  ("FP64 colour pass", "k_smooth<3, 16, 2, double>", 0.5 * (4 * C_ + Fw)),
-                                 ("FP32 hybrid sweep", "k_sweep32<3, 16, false>", 0.5 * (4 * C_ + Fw)),
-                                 ("FP32 hybrid sweep from zero", "k_sweep32<3, 16, true>", 0.5 * (3 * C_ + Fw))):
+                                 ("FP32 hybrid sweep", "k_sweep32<3, 16, false, false>", 0.5 * (4 * C_ + Fw)),
+                                 ("FP32 hybrid sweep from zero", "k_sweep32<3, 16, true, false>", 0.5 * (3 * C_ + Fw)),
+                                 ("FP32 hybrid sweep from zero, FP64 rhs", "k_sweep32<3, 16, true, true>",
+                                  0.5 * (5 * C_ + Fw))):
         fe = [v for (k, c), lst in vals.items() if c == "FETCH_SIZE" and kre in k for v in lst]
         wr = [v for (k, c), lst in vals.items() if c == "WRITE_SIZE" and kre in k for v in lst]
         if not fe or not wr or kf is None:
