@@ -41,7 +41,6 @@ struct MixedState {
     float* w32 = nullptr;
     float* wcf32 = nullptr;
     float* vf32 = nullptr;
-    float* r32 = nullptr;
     float* t32 = nullptr;       // second buffer of the double-buffered smoother
     // 2: hybrid sweeps (k_sweep32, one launch per sweep over the tower; the default);
     // 1: exact red-black, one launch per colour over the tower, double-buffered;
@@ -61,7 +60,7 @@ struct MixedState {
 };
 
 inline void mixed_free(MixedState& ms) {
-    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.r32, ms.t32, ms.xst};
+    float* ptrs[] = {ms.w32, ms.wcf32, ms.vf32, ms.t32, ms.xst};
     for (float* p : ptrs) if (p) (void)hipFree(p);
     ms.xst = nullptr;
     for (int l = 0; l <= MIX_MAXLV; l++) {
@@ -71,7 +70,7 @@ inline void mixed_free(MixedState& ms) {
         ms.xs[l] = nullptr;
         ms.u[l] = ms.f[l] = nullptr;
     }
-    ms.w32 = ms.wcf32 = ms.vf32 = ms.r32 = ms.t32 = nullptr;
+    ms.w32 = ms.wcf32 = ms.vf32 = ms.t32 = nullptr;
     ms.cap_cells = ms.cap_faces = ms.cap_cf = 0;
     ms.coef_version = -1;
 }
@@ -81,7 +80,7 @@ __global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst
         dst[q] = (float)src[q];
 }
 
-enum MixOp { M_ZERO = 0, M_COPY, M_ADD, M_PROLONG_ADD, M_F64_TO_F32, M_F32_TO_F64 };
+enum MixOp { M_ZERO = 0, M_F64_TO_F32, M_F32_TO_F64 };
 
 template <int D, int NB>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
@@ -89,55 +88,16 @@ k_mix(const int* list, int op, const BlockInfo* __restrict__ binfo, const float*
       float* dst, const float* src, double* dst64, const double* src64) {
     using G = Geo<D, NB>;
     const int b = list[blockIdx.x];
-    const int h = NB / 2;
     for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
         const int gi = b * G::NCELL + c;
         switch (op) {
             case M_ZERO: dst[gi] = 0.0f; break;
-            case M_COPY: dst[gi] = src[gi]; break;
-            case M_ADD: dst[gi] += src[gi]; break;
-            case M_PROLONG_ADD: {
-                int i, j, k;
-                G::cell_ijk(c, i, j, k);
-                const int p = binfo[b].parent, wc = binfo[b].which_child;
-                const int pc = G::cidx((i >> 1) + side_of(wc, 0) * h, (j >> 1) + side_of(wc, 1) * h,
-                                       D == 3 ? (k >> 1) + side_of(wc, 2) * h : 0);
-                if (vf[gi] > 0.0f) dst[gi] += src[p * G::NCELL + pc];
-                break;
-            }
             case M_F64_TO_F32: dst[gi] = (float)src64[gi]; break;
             case M_F32_TO_F64: dst64[gi] = (double)src[gi]; break;
             default: break;
         }
     }
 }
-
-// vf-weighted average of the children's residual onto the non-leaf blocks of one level.
-template <int D, int NB>
-__global__ void __launch_bounds__((Geo<D, NB>::THREADS))
-k_restrict32(const int* list, const BlockInfo* binfo, const float* vf, const float* src, float* dst) {
-    using G = Geo<D, NB>;
-    const int b = list[blockIdx.x];
-    const float wgt = 1.0f / G::NCHILD;
-    const int h = NB / 2;
-    for (int c = threadIdx.x; c < G::NCELL; c += G::THREADS) {
-        int i, j, k;
-        G::cell_ijk(c, i, j, k);
-        const int si = i >= h, sj = j >= h, sk = (D == 3 && k >= h);
-        const int fb = binfo[b].child[si | (sj << 1) | (sk << 2)];
-        const int fi0 = (i - si * h) * 2, fj0 = (j - sj * h) * 2, fk0 = D == 3 ? (k - sk * h) * 2 : 0;
-        float sum = 0.0f;
-        for (int kk = 0; kk <= (D == 3 ? 1 : 0); kk++)
-            for (int jj = 0; jj <= 1; jj++)
-                for (int ii = 0; ii <= 1; ii++) {
-                    const int fi = fb * G::NCELL + G::cidx(fi0 + ii, fj0 + jj, fk0 + kk);
-                    sum += src[fi] * vf[fi];
-                }
-        const float v = vf[b * G::NCELL + c];
-        dst[b * G::NCELL + c] = v > 0.0f ? (sum * wgt) / v : 0.0f;
-    }
-}
-
 
 // The block's x-boundary columns (i = 0, i = NB-1) from its LDS tile into the slab array
 // [block][side][j + NB k] that load_halo reads for x faces.
@@ -276,8 +236,8 @@ k_sweep32(StencilArgsT<float> a, double* out64, const double* __restrict__ rhs64
 
 // r = f - A u on tower T(l) (FP32), handed straight to level l-1's right-hand side: a block
 // below level l (a leaf that T(l-1) shares) passes r on as it is; a block of level l restricts
-// r onto its parent's octant, the vf-weighted average of k_restrict32 with the same sums in
-// the same order.  Replaces the residual launch, the copy and the restriction (and their
+// r onto its parent's octant: the vf-weighted average of the 2^d children's r (sum of r * vf
+// in kk, jj, ii order, / 2^d, / the parent's vf; 0 where that is 0).  Replaces the residual launch, the copy and the restriction (and their
 // re-reads of r).  a.x = u, a.rhs = f; fc = f_{l-1}.
 template <int D, int NB>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))
@@ -363,7 +323,7 @@ k_resid_restrict32(StencilArgsT<float> a, float* fc, int l, const float* __restr
 
 // The coarse correction on tower T(l): out = u_l + u_{l-1} on the blocks below level l (the
 // leaves T(l-1) shares), u_l + the injection of the parent's u_{l-1} (nothing where vf = 0) on
-// the blocks of level l: k_mix's M_ADD and M_PROLONG_ADD in one launch over T(l).  out may be
+// the blocks of level l, in one launch over T(l).  out may be
 // u (in place) or the other buffer of the double-buffered sweeps that follow.
 template <int D, int NB>
 __global__ void __launch_bounds__((Geo<D, NB>::THREADS))

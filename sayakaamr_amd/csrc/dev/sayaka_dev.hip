@@ -651,7 +651,6 @@ struct Impl {
             HIPCHK(hipMalloc(&ms.w32, faces * sizeof(float)));
             HIPCHK(hipMalloc(&ms.wcf32, cf * sizeof(float)));
             HIPCHK(hipMalloc(&ms.vf32, cells * sizeof(float)));
-            HIPCHK(hipMalloc(&ms.r32, cells * sizeof(float)));
             HIPCHK(hipMalloc(&ms.t32, cells * sizeof(float)));
             const size_t slabs = (size_t)c->cap * 2 * G::NSLAB;
             HIPCHK(hipMalloc(&ms.xst, slabs * sizeof(float)));
