@@ -297,10 +297,24 @@ struct Bottom {
         }
         __syncthreads();
         SY_PH_MARK(pht, PH_WAIT + (ph - 1) / 4);
+        // ordered per-block chains (the two-level variant always sums in tree order); chain
+        // l = which * BPW + blk is the sum `which` of block blk
+        constexpr bool SEGCH = G::NCELL == 512;
+        if constexpr (SEGCH) {
+            // two chains per wave with every value in registers (seg_chain512_asm): chain l in
+            // half l % 2 of wave l / 2, its result in that half's last lane
+            const int nch = BPW * nsum;
+            if (!PC && A.exact && 2 * wv < nch) {
+                const int l = 2 * wv + (lane >> 5);
+                const int lc = l < nch ? l : nch - 1;
+                const double sv = seg_chain512_asm<200>(lds_addr(red + lc * (G::NCELL + 2) + 16 * (lane & 31)));
+                if ((lane & 31) == 31 && l < nch && l % BPW < nvalid) part[PS * (w0 + l % BPW) + l / BPW] = sv;
+            }
+        }
         if (wv == 0) {
-            if (!PC && A.exact) {   // (the two-level variant always sums in tree order)
-                // ordered per-block chains, one lane per block and sum
-                if (lane < BPW * nsum) {
+            if (!PC && A.exact) {
+                // one lane per chain
+                if (!SEGCH && lane < BPW * nsum) {
                     const double sv = lane_chain<G::NCELL, 200>(red, lane);
                     const int blk = lane % BPW;
                     if (blk < nvalid) part[PS * (w0 + blk) + lane / BPW] = sv;

@@ -16,6 +16,15 @@ inside the block (the last step waits for all), so no load is in flight when the
 BASE is a template argument: 16 for kernels that chain at the end of a short body, 200 in
 k_bottom, whose Krylov state occupies the low registers.
 
+A second form, seg_chain512_asm<BASE>(addr), runs two 512-term sums in one wave with every
+value in registers: lane 32 h + j (h = 0, 1) reads values 16 j .. 16 j + 15 of sum h (eight
+128-bit reads, all waited for before the first add), and the running sum moves one lane to the
+right after each lane's 16 adds (DPP wave_shr:1 on both halves of the double; every lane adds
+its own values at every step, so no EXEC changes are needed, and lane 32 h + s holds sum h up
+to its segment after step s).  Measured on MI355X, including the reads: 6.3 clocks per add
+against 7.85 for the 128-bit ring (5.68 is a register-only chain).  s_nop 1 covers the two
+wait states a DPP read needs after a VALU write of the same register.
+
     python sayakaamr_amd/csrc/dev/gen_chain_asm.py   # rewrites chain_asm.h beside this script
 """
 import os
@@ -52,6 +61,27 @@ def block(n, base):
     return lines
 
 
+SEG = 16           # values per lane in seg_chain512_asm (512 = 32 lanes x 16)
+SEG_BASES = (200,)
+
+
+def seg_block(base):
+    acc = base + 2 * SEG
+    lines = ["s_waitcnt lgkmcnt(0)"]
+    for k in range(SEG // 2):
+        lines.append(f"ds_read_b128 v[{base + 4 * k}:{base + 4 * k + 3}], %[a] offset:{16 * k}")
+    lines += [f"v_mov_b32 v{acc}, 0", f"v_mov_b32 v{acc + 1}, 0", "s_waitcnt lgkmcnt(0)"]
+    for step in range(512 // SEG):
+        if step:
+            lines.append("s_nop 1")
+            lines.append(f"v_mov_b32_dpp v{acc}, v{acc} wave_shr:1 row_mask:0xf bank_mask:0xf")
+            lines.append(f"v_mov_b32_dpp v{acc + 1}, v{acc + 1} wave_shr:1 row_mask:0xf bank_mask:0xf")
+        for k in range(SEG):
+            lines.append(f"v_add_f64 v[{acc}:{acc + 1}], v[{acc}:{acc + 1}], v[{base + 2 * k}:{base + 2 * k + 1}]")
+    lines += [f"v_mov_b32 %[lo], v{acc}", f"v_mov_b32 %[hi], v{acc + 1}"]
+    return lines
+
+
 def main():
     here = os.path.dirname(os.path.abspath(__file__))
     out = [
@@ -84,6 +114,29 @@ def main():
             out.append("    return s;")
             out.append("}")
             out.append("")
+    out += [
+        "// seg_chain512_asm<BASE>(addr): two sequential 512-term sums per wave (whole wave active),",
+        "// lane 32 h + j holding values 16 j .. 16 j + 15 of sum h at LDS byte address addr; the sum",
+        f"// moves one lane right (DPP wave_shr:1) after each lane's {SEG} adds.  Result: sum 0 in lane",
+        f"// 31, sum 1 in lane 63.  Registers v[BASE .. BASE + {2 * SEG + 1}] are used and clobbered.",
+        "template <int BASE>",
+        "__device__ inline double seg_chain512_asm(unsigned addr);",
+        "",
+    ]
+    for base in SEG_BASES:
+        clob = ", ".join(f'"v{r}"' for r in range(base, base + 2 * SEG + 2))
+        out.append("template <>")
+        out.append(f"__device__ inline double seg_chain512_asm<{base}>(unsigned addr) {{")
+        out.append("    unsigned lo, hi;")
+        out.append("    asm volatile(")
+        for ln in seg_block(base):
+            out.append(f'        "{ln}\\n"')
+        out.append('        : [lo] "=v"(lo), [hi] "=v"(hi)')
+        out.append('        : [a] "v"(addr)')
+        out.append(f'        : "memory", {clob});')
+        out.append("    return __hiloint2double((int)hi, (int)lo);")
+        out.append("}")
+        out.append("")
     out.append("}  // namespace sydev")
     with open(os.path.join(here, "chain_asm.h"), "w") as f:
         f.write("\n".join(out) + "\n")
