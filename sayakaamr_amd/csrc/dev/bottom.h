@@ -117,7 +117,16 @@ struct BotCfg {
     static constexpr int CPT = WCELL / NT;                             // cells per thread
     static constexpr int WGHOST = BPW * G::NGHOST;
     static constexpr int GPT = (WGHOST + NT - 1) / NT;                 // ghost slots per thread
-    static constexpr int NRED = 2 * BPW * (G::NCELL + 2);              // two sums staged
+    // staged terms of the ordered sums, chain l = which * BPW + blk.  512-term chains
+    // (seg_chain512_asm: lane j reads terms 16 j .. 16 j + 15) put two empty slots after every
+    // 16 terms, so that the lanes' 128-byte runs start 144 bytes apart and spread over the LDS
+    // banks; other lengths are contiguous (lane_chain)
+    static constexpr bool SEGCH = G::NCELL == 512;
+    static constexpr int CSTRIDE = SEGCH ? G::NCELL + G::NCELL / 8 + 2 : G::NCELL + 2;
+    __host__ __device__ static constexpr int chain_slot(int l, int c) {
+        return l * CSTRIDE + (SEGCH ? c + 2 * (c >> 4) : c);
+    }
+    static constexpr int NRED = 2 * BPW * CSTRIDE;                     // two sums staged
     static constexpr int NTIL = BPW * G::NTILE;
     // face weights of the workgroup's blocks staged in LDS when they fit beside the rest
     static constexpr bool WLDS = BPW * D * G::NFACE * 8 <= 64 * 1024;
@@ -202,7 +211,7 @@ struct Bottom {
     // it in LDS for the ordered chains; fast mode adds it to the thread's running sum.
     __device__ static void stage(double* red, int exact, int which, int blk, int c, double val, double& acc) {
         if (exact) {
-            stage_chain(red, G::NCELL, which * BPW + blk, c, val);
+            red[C::chain_slot(which * BPW + blk, c)] = val;
         } else {
             acc += val;
         }
@@ -299,7 +308,7 @@ struct Bottom {
         SY_PH_MARK(pht, PH_WAIT + (ph - 1) / 4);
         // ordered per-block chains (the two-level variant always sums in tree order); chain
         // l = which * BPW + blk is the sum `which` of block blk
-        constexpr bool SEGCH = G::NCELL == 512;
+        constexpr bool SEGCH = C::SEGCH;
         if constexpr (SEGCH) {
             // two chains per wave with every value in registers (seg_chain512_asm): chain l in
             // half l % 2 of wave l / 2, its result in that half's last lane
@@ -307,7 +316,7 @@ struct Bottom {
             if (!PC && A.exact && 2 * wv < nch) {
                 const int l = 2 * wv + (lane >> 5);
                 const int lc = l < nch ? l : nch - 1;
-                const double sv = seg_chain512_asm<200>(lds_addr(red + lc * (G::NCELL + 2) + 16 * (lane & 31)));
+                const double sv = seg_chain512_asm<200>(lds_addr(red + C::chain_slot(lc, 16 * (lane & 31))));
                 if ((lane & 31) == 31 && l < nch && l % BPW < nvalid) part[PS * (w0 + l % BPW) + l / BPW] = sv;
             }
         }
