@@ -316,7 +316,7 @@ struct Bottom {
             if (!PC && A.exact && 2 * wv < nch) {
                 const int l = 2 * wv + (lane >> 5);
                 const int lc = l < nch ? l : nch - 1;
-                const double sv = seg_chain512_asm<200>(lds_addr(red + C::chain_slot(lc, 16 * (lane & 31))));
+                const double sv = seg_chain512_asm<200>(lds_addr(red + C::chain_slot(lc, 16 * (lane & 31))), 0.0);
                 if ((lane & 31) == 31 && l < nch && l % BPW < nvalid) part[PS * (w0 + l % BPW) + l / BPW] = sv;
             }
         }

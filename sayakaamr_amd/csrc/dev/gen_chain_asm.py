@@ -16,7 +16,7 @@ inside the block (the last step waits for all), so no load is in flight when the
 BASE is a template argument: 16 for kernels that chain at the end of a short body, 200 in
 k_bottom, whose Krylov state occupies the low registers.
 
-A second form, seg_chain512_asm<BASE>(addr), runs two 512-term sums in one wave with every
+A second form, seg_chain512_asm<BASE>(addr, init), runs two 512-term sums in one wave with every
 value in registers: lane 32 h + j (h = 0, 1) reads values 16 j .. 16 j + 15 of sum h (eight
 128-bit reads, all waited for before the first add), and the running sum moves one lane to the
 right after each lane's 16 adds (DPP wave_shr:1 on both halves of the double; every lane adds
@@ -62,7 +62,7 @@ def block(n, base):
 
 
 SEG = 16           # values per lane in seg_chain512_asm (512 = 32 lanes x 16)
-SEG_BASES = (200,)
+SEG_BASES = (16, 200)
 
 
 def seg_block(base):
@@ -70,7 +70,7 @@ def seg_block(base):
     lines = ["s_waitcnt lgkmcnt(0)"]
     for k in range(SEG // 2):
         lines.append(f"ds_read_b128 v[{base + 4 * k}:{base + 4 * k + 3}], %[a] offset:{16 * k}")
-    lines += [f"v_mov_b32 v{acc}, 0", f"v_mov_b32 v{acc + 1}, 0", "s_waitcnt lgkmcnt(0)"]
+    lines += [f"v_mov_b32 v{acc}, %[ilo]", f"v_mov_b32 v{acc + 1}, %[ihi]", "s_waitcnt lgkmcnt(0)"]
     for step in range(512 // SEG):
         if step:
             lines.append("s_nop 1")
@@ -115,24 +115,26 @@ def main():
             out.append("}")
             out.append("")
     out += [
-        "// seg_chain512_asm<BASE>(addr): two sequential 512-term sums per wave (whole wave active),",
-        "// lane 32 h + j holding values 16 j .. 16 j + 15 of sum h at LDS byte address addr; the sum",
-        f"// moves one lane right (DPP wave_shr:1) after each lane's {SEG} adds.  Result: sum 0 in lane",
-        f"// 31, sum 1 in lane 63.  Registers v[BASE .. BASE + {2 * SEG + 1}] are used and clobbered.",
+        "// seg_chain512_asm<BASE>(addr, init): two sequential 512-term sums per wave (whole wave",
+        "// active), lane 32 h + j holding values 16 j .. 16 j + 15 of sum h at LDS byte address addr,",
+        "// sum h starting from lane 32 h's init; the sum moves one lane right (DPP wave_shr:1) after",
+        f"// each lane's {SEG} adds.  Result: sum 0 in lane 31, sum 1 in lane 63.  Registers",
+        f"// v[BASE .. BASE + {2 * SEG + 1}] are used and clobbered.",
         "template <int BASE>",
-        "__device__ inline double seg_chain512_asm(unsigned addr);",
+        "__device__ inline double seg_chain512_asm(unsigned addr, double init);",
         "",
     ]
     for base in SEG_BASES:
         clob = ", ".join(f'"v{r}"' for r in range(base, base + 2 * SEG + 2))
         out.append("template <>")
-        out.append(f"__device__ inline double seg_chain512_asm<{base}>(unsigned addr) {{")
+        out.append(f"__device__ inline double seg_chain512_asm<{base}>(unsigned addr, double init) {{")
         out.append("    unsigned lo, hi;")
+        out.append("    const unsigned ilo = (unsigned)__double2loint(init), ihi = (unsigned)__double2hiint(init);")
         out.append("    asm volatile(")
         for ln in seg_block(base):
             out.append(f'        "{ln}\\n"')
         out.append('        : [lo] "=v"(lo), [hi] "=v"(hi)')
-        out.append('        : [a] "v"(addr)')
+        out.append('        : [a] "v"(addr), [ilo] "v"(ilo), [ihi] "v"(ihi)')
         out.append(f'        : "memory", {clob});')
         out.append("    return __hiloint2double((int)hi, (int)lo);")
         out.append("}")

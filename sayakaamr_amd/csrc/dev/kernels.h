@@ -167,37 +167,61 @@ __device__ inline double load_published(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Staged layout of the two 512-term sums of seg_chain512_asm: term c of sum h at
+// h * SEG_STRIDE + c + 2 (c / 16), so that the lanes' 128-byte runs start 144 bytes apart and
+// spread over the LDS banks.
+constexpr int SEG_STRIDE = 512 + 512 / 8 + 2;
+__device__ inline int seg_slot(int h, int c) { return h * SEG_STRIDE + c + 2 * (c >> 4); }
+
 // Sum of n (sum, sum, max) partials written by an earlier launch: exact mode sums in index
-// order (chunks staged in LDS by all threads, chained in lane 0 / lane 1), fast mode by a
-// strided tree.  The max is order-free.  Results valid in thread 0.
+// order, fast mode by a strided tree.  The max is order-free.  Results valid in thread 0.
+// Exact mode (NT = 512): chunks of 512 partials, wave 0 chaining chunk k (both sums at once,
+// seg_chain512_asm, the running sums carried from chunk to chunk) while the other waves load
+// chunk k + 1 into the other of two LDS buffers; zero padding after the last partial (the
+// running sums start at +0 and never become -0, so adding +0 changes nothing).
 template <int NT>
 __device__ inline void sum_published(const double* part, unsigned n, bool exact, int nsum, double& a, double& b,
-                                     double& m, double* chunk /* 2*NT doubles of LDS */, double* lds) {
-    static_assert(NT % 16 == 0, "chunk must be a multiple of 16");
+                                     double& m, double* chunk /* 4 * SEG_STRIDE doubles of LDS, 16-byte aligned */,
+                                     double* lds) {
+    static_assert(NT == 512, "one partial per thread and chunk");
     a = 0.0;
     b = 0.0;
     m = 0.0;
     if (exact) {
-        double ml = 0.0, acc = 0.0;
-        for (unsigned base = 0; base < n; base += NT) {
-            const unsigned cnt = n - base < (unsigned)NT ? n - base : (unsigned)NT;
-            const unsigned cnt8 = (cnt + 15u) & ~15u;   // zero padding (acc starts at +0)
-            const unsigned g = threadIdx.x;
-            if (g < cnt8) {
-                const bool ok = g < cnt;
-                chunk[g] = ok ? part[3 * (base + g)] : 0.0;
-                chunk[NT + g] = ok && nsum > 1 ? part[3 * (base + g) + 1] : 0.0;
-                if (ok) ml = fmax(ml, part[3 * (base + g) + 2]);
+        double ml = 0.0;
+        const unsigned nchunk = (n + 511u) / 512u;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        // stage chunk k (partials 512 k ..) into buffer k % 2 with threads t0 .. t0 + nt - 1
+        auto stage = [&](unsigned k, int t, int nt) {
+            double* buf = chunk + (k & 1u) * 2 * SEG_STRIDE;
+            for (int g = t; g < 512; g += nt) {
+                const unsigned i = 512u * k + (unsigned)g;
+                const bool ok = i < n;
+                buf[seg_slot(0, g)] = ok ? part[3 * i] : 0.0;
+                buf[seg_slot(1, g)] = ok && nsum > 1 ? part[3 * i + 1] : 0.0;
+                if (ok) ml = fmax(ml, part[3 * i + 2]);
+            }
+        };
+        if (nchunk > 0) stage(0, threadIdx.x, NT);
+        __syncthreads();
+        double acc = 0.0;   // wave 0: sum 0 in lanes 0-31, sum 1 in lanes 32-63 (carried in lanes 31, 63)
+        for (unsigned k = 0; k < nchunk; k++) {
+            if (wv == 0) {
+                const double* buf = chunk + (k & 1u) * 2 * SEG_STRIDE;
+                const double r = seg_chain512_asm<16>(lds_addr(buf + seg_slot(lane >> 5, 16 * (lane & 31))), acc);
+                acc = __shfl(r, lane < 32 ? 31 : 63, 64);
+            } else if (k + 1 < nchunk) {
+                stage(k + 1, threadIdx.x - 64, NT - 64);
             }
             __syncthreads();
-            if ((int)threadIdx.x < nsum) acc = chain_n(chunk + threadIdx.x * NT, (int)cnt8, acc);
-            __syncthreads();
         }
-        if (threadIdx.x == 1) chunk[0] = acc;
+        if (threadIdx.x == 32) chunk[0] = acc;   // sum 1 (the buffers are no longer read)
+        __syncthreads();
+        const double sb = chunk[0];
         double z0 = 0.0, z1 = 0.0;
         block_reduce3<NT>(z0, z1, ml, lds);   // contains barriers
         a = acc;
-        b = nsum > 1 ? chunk[0] : 0.0;
+        b = nsum > 1 ? sb : 0.0;
         m = ml;
     } else {
         for (unsigned g = threadIdx.x; g < n; g += NT) {
@@ -223,7 +247,7 @@ __device__ inline void grid_finish(double s0, double s1, double mx, const RedBuf
 // previous launch summed (in list order when exact) and the Krylov scalars updated.
 constexpr int RED_NT = 512;
 __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsum, int fin) {
-    __shared__ __attribute__((aligned(16))) double chunk[2 * RED_NT];
+    __shared__ __attribute__((aligned(16))) double chunk[4 * SEG_STRIDE];
     __shared__ double lds[3 * (RED_NT / 64)];
     double a, b, m;
     sum_published<RED_NT>(rb.partials, (unsigned)n, rb.exact != 0, nsum, a, b, m, chunk, lds);
