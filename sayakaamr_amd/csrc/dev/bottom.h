@@ -91,6 +91,8 @@ struct BottomArgs {
     const float* s0inv;    // coarse inverse, row-major nlist x nlist
     const double* jac;     // [level-1 list position][cell]: 1 / diag(A), 0 on solid cells
     const int* l1pos;      // block id -> position in the level-1 list, -1 for other blocks
+    const int* skip;       // when non-null and *skip != 0: the launch does nothing (a solve that
+                           // stopped at the half step queued this V-cycle before knowing it)
 };
 
 constexpr int PC_MAXBLK = 64;   // level-1 blocks the two-level preconditioner supports
@@ -402,6 +404,7 @@ __global__ void __launch_bounds__((BotCfg<D, NB>::NT)) k_bottom(BottomArgs A) {
     using B = Bottom<D, NB, PC>;
     constexpr int BPW = C::BPW, NT = C::NT, CPT = C::CPT, GPT = C::GPT;
     constexpr int PS = B::PS;
+    if (A.skip && *A.skip) return;                 // every workgroup reads the same flag
     if (A.confine && (blockIdx.x & 7u)) return;   // one XCD: not a worker
     __shared__ int s_broken, s_light;     // a grid barrier timed out; one-XCD barriers (grid_barrier)
     if (threadIdx.x == 0) s_broken = s_light = 0;   // visible after the first __syncthreads below

@@ -84,6 +84,9 @@ struct RedBuf {
     int exact;          // 1: sums in the reference's order (cells k,j,i in a block, then
                         //    blocks in list order: DotProd / mg_dotprod_level,
                         //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
+    int* stop = nullptr;          // FIN_SNORM: 1 when the half step stops the solve (alpha_den == 0,
+    double tol_rel = 0.0;         // or |s| < rnorm0 * tol_rel or |s| < tol_abs: the host's test),
+    double tol_abs = 0.0;         // else 0
 };
 
 // LDS byte address of a pointer into __shared__ memory.
@@ -256,7 +259,10 @@ __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsu
         switch (fin) {
             case FIN_INIT: S[S_RNORM0] = m; S[S_RHO] = a; break;
             case FIN_ALPHA: S[S_ALPHA_DEN] = a; S[S_ALPHA] = a != 0.0 ? S[S_RHO] / a : 0.0; break;
-            case FIN_SNORM: S[S_SNORM] = m; break;
+            case FIN_SNORM:
+                S[S_SNORM] = m;
+                if (rb.stop) *rb.stop = (S[S_ALPHA_DEN] == 0.0 || m < S[S_RNORM0] * rb.tol_rel || m < rb.tol_abs) ? 1 : 0;
+                break;
             case FIN_OMEGA: S[S_TS] = a; S[S_TT] = b; S[S_OMEGA] = b != 0.0 ? a / b : 0.0; break;
             case FIN_RUPDATE: S[S_RNORM] = m; S[S_RHO1] = S[S_RHO]; S[S_RHO] = a; break;
             case FIN_DOT: S[S_DOT] = a; break;
@@ -840,9 +846,9 @@ __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fi
                 mx = fmax(mx, fabs(sv));
                 break;
             }
-            case V_RUPDATE: {
+            case V_RUPDATE: {   // x (into dst when given) += omega shat ; r = s - omega t
                 const double om = S[S_OMEGA];
-                a.x[gi] = 1.0 * a.x[gi] + om * a.src[gi];
+                (a.dst ? a.dst : a.x)[gi] = 1.0 * a.x[gi] + om * a.src[gi];
                 const double rv = 1.0 * a.s[gi] + -om * a.t[gi];
                 const_cast<double*>(a.r)[gi] = rv;
                 mx = fmax(mx, fabs(rv));

@@ -110,6 +110,13 @@ struct sayaka_dev_ctx {
     double* d_bot_rpub = nullptr;
     double* d_bot_part = nullptr;
     int* d_bot_stats = nullptr;
+    // Half-step test on the device: the solve queues the second half of an iteration before it
+    // reads |s|; the FIN_SNORM finish writes d_stop (1: the solve stops there), and the
+    // coarsest-level solves launched while bottom_skip is set do nothing when it is 1.
+    int* d_stop = nullptr;
+    bool half_stop = false, bottom_skip = false;
+    double half_tol_rel = 0.0, half_tol_abs = 0.0;
+    double defer_factor = 100.0;   // sayaka_dev_set_half_step_defer
     unsigned long long* d_bot_flags = nullptr;   // grid-barrier flags of the bottom solve, one per workgroup
     unsigned long long bot_launches = 0;
     int bot_fit[2] = {0, 0};                     // workgroups of k_bottom<.., PC> resident at once
@@ -372,9 +379,16 @@ int launch_vec(sayaka_dev_ctx* c, ListRef lst, int op, const VecArgs& a, int fin
     ProfScope ps(c, SAYAKA_KCLASS_VEC, (long long)lst.n * G::NCELL, lst.n * AlgBytes<D, NB, double>::vec(op, extra));
     hipLaunchKernelGGL((k_vec<G::NCELL, G::THREADS>), dim3(lst.n), dim3(G::THREADS), 0, c->stream, a, op,
                        redbuf(c), fin);
-    if (fin != FIN_NONE)
-        hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, redbuf(c), lst.n,
+    if (fin != FIN_NONE) {
+        RedBuf rb = redbuf(c);
+        if (fin == FIN_SNORM && c->half_stop) {
+            rb.stop = c->d_stop;
+            rb.tol_rel = c->half_tol_rel;
+            rb.tol_abs = c->half_tol_abs;
+        }
+        hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, rb, lst.n,
                            (op == V_RUPDATE || op == V_DOT) ? 1 : 0, fin);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -466,6 +480,7 @@ struct Impl {
         A.s0inv = c->d_pc_s0inv;
         A.jac = c->d_pc_jac;
         A.l1pos = c->d_l1pos;
+        A.skip = c->bottom_skip ? c->d_stop : nullptr;
         using BC = BotCfg<D, NB>;
         const int nwg = (lst.n + BC::BPW - 1) / BC::BPW;
         const void* kern = two_level ? (const void*)k_bottom<D, NB, true> : (const void*)k_bottom<D, NB, false>;
@@ -1131,9 +1146,48 @@ struct Impl {
         int stat = 0, iter;
         double* phat = precond ? F[SAYAKA_PHAT] : F[SAYAKA_P];
         double* shat = precond ? F[SAYAKA_SHAT] : F[SAYAKA_S];
+        // The iterate x - x0 alternates between X and X0: the full step writes the buffer the
+        // half step did not, so the half step's x survives the second half of its iteration.
+        // With a preconditioner, and while the last tested norm is at least defer_factor x the
+        // stopping threshold (a half step that far from convergence practically never stops),
+        // the second half is queued before the host reads |s| (one host round trip per
+        // iteration instead of two): the FIN_SNORM finish evaluates the half-step test on the
+        // device, and the second half's coarsest-level solve does nothing when it holds.  The
+        // host then applies the same test to the same |s|, so every decision, the history and
+        // the bottom-solve counts are those of testing before the second half.
+        double* xc = F[SAYAKA_X];
+        double* xo = F[SAYAKA_X0];
+        double* xsol = xc;
+        const double thr = std::max(rnorm0 * P->tol_rel, P->tol_abs);
+        double last_tested = rnorm0;
+        struct DeferScope {
+            sayaka_dev_ctx* c;
+            ~DeferScope() { c->half_stop = c->bottom_skip = false; }
+        } defer_scope{c};
+        c->half_tol_rel = P->tol_rel;
+        c->half_tol_abs = P->tol_abs;
+        // the half-step test (SayakaMacSolver.cpp:683-920): breakdown, then |s| against the tolerances
+        auto half_step_test = [&]() -> bool {
+            if (c->h_scal[S_ALPHA_DEN] == 0.0) {
+                stat = 1;
+                return true;
+            }
+            const double snorm = c->h_scal[S_SNORM];
+            push(-snorm);
+            st->rnorm = snorm;
+            if (snorm < rnorm0 * P->tol_rel || snorm < P->tol_abs) {
+                conv = true;
+                st->half_step = 1;
+                xsol = xc;
+                return true;
+            }
+            return false;
+        };
         for (iter = 1; iter <= P->max_iter; iter++) {
             st->iters = iter;
             if (c->h_scal[S_RHO] == 0.0) { stat = 1; break; }
+            const bool defer = precond && c->defer_factor >= 0.0 && last_tested >= c->defer_factor * thr;
+            c->half_stop = defer;
             {
                 VecArgs a = vec_args(c, T);
                 a.r = F[SAYAKA_R];
@@ -1159,24 +1213,20 @@ struct Impl {
                 a.r = F[SAYAKA_R];
                 a.v = F[SAYAKA_V];
                 a.s = F[SAYAKA_S];
-                a.x = F[SAYAKA_X];
+                a.x = xc;
                 a.phat = phat;
                 a.resid = F[SAYAKA_RESID];
                 if ((e = launch_vec<D, NB>(c, T, V_SUPDATE, a, FIN_SNORM))) return e;
             }
-            if ((e = read_scalars(c))) return e;
-            if (c->h_scal[S_ALPHA_DEN] == 0.0) { stat = 1; break; }
-            const double snorm = c->h_scal[S_SNORM];
-            push(-snorm);
-            st->rnorm = snorm;
-            if (snorm < rnorm0 * P->tol_rel || snorm < P->tol_abs) {
-                conv = true;
-                st->half_step = 1;
-                break;
+            if (!defer) {
+                if ((e = read_scalars(c))) return e;
+                if (half_step_test()) break;
             }
             if (precond) {
+                c->bottom_skip = defer;
                 if (mixed) e = mixed_vcycle(c, L, F[SAYAKA_RESID], shat, P->pre_smooth, P->post_smooth);
                 else e = vcycle(c, L, shat);
+                c->bottom_skip = false;
                 if (e) return e;
             }
             {
@@ -1191,22 +1241,30 @@ struct Impl {
                 a.r = F[SAYAKA_R];
                 a.s = F[SAYAKA_S];
                 a.t = F[SAYAKA_T];
-                a.x = F[SAYAKA_X];
+                a.x = xc;
+                a.dst = xo;
                 a.src = shat;
                 a.rhat = F[SAYAKA_RHAT];
                 if ((e = launch_vec<D, NB>(c, T, V_RUPDATE, a, FIN_RUPDATE))) return e;
             }
             if ((e = read_scalars(c))) return e;
+            if (defer && half_step_test()) break;
             if (c->h_scal[S_TT] == 0.0) { stat = 1; break; }
             const double rnorm = c->h_scal[S_RNORM];
             push(rnorm);
             st->rnorm = rnorm;
-            if (rnorm < rnorm0 * P->tol_rel + P->tol_abs) { conv = true; break; }
+            last_tested = rnorm;
+            if (rnorm < rnorm0 * P->tol_rel + P->tol_abs) {
+                conv = true;
+                xsol = xo;
+                break;
+            }
             if (c->h_scal[S_OMEGA] == 0.0) { stat = 1; break; }
+            std::swap(xc, xo);
         }
         if (conv) {
             VecArgs a = vec_args(c, T);
-            a.x = F[SAYAKA_X];
+            a.x = xsol;
             a.dst = F[SAYAKA_SOL];
             if ((e = launch_vec<D, NB>(c, T, V_FINAL, a, FIN_NONE))) return e;
         }
@@ -1315,6 +1373,8 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_fsum, (size_t)PC_MAXBLK * 6 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_s0inv, (size_t)PC_MAXBLK * PC_MAXBLK * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 8 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->d_stop, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_stop, 0, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_flags, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->d_bot_flags, 0, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 8 * sizeof(int), hipHostMallocDefault);
@@ -1349,7 +1409,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
     if (c->d_remap) hipFree(c->d_remap);
     if (c->d_fill_val) hipFree(c->d_fill_val);
     void* ptrs[] = {c->d_vf, c->d_af, c->d_b, c->d_w, c->d_wcf, c->d_partials, c->d_scal,
-                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_bot_flags, c->d_binfo, c->d_met, c->d_lists,
+                    c->d_bot_vpub, c->d_bot_rpub, c->d_bot_part, c->d_bot_stats, c->d_stop, c->d_bot_flags, c->d_binfo, c->d_met, c->d_lists,
                     c->d_l1pos, c->d_pc_jac, c->d_pc_fsum, c->d_pc_s0inv};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < SAYAKA_KCLASS_N; k++)
@@ -1678,6 +1738,12 @@ int sayaka_dev_memcpy(sayaka_dev_ctx* c, void* dst, const void* src, size_t byte
     HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->xfer[to_device ? 0 : 1] += (long long)bytes;
+    return 0;
+}
+
+int sayaka_dev_set_half_step_defer(sayaka_dev_ctx* c, double factor) {
+    if (!c || factor != factor) return fail(-1, "bad argument");
+    c->defer_factor = factor;
     return 0;
 }
 

@@ -87,6 +87,7 @@ _SIG = {
     "sayaka_dev_get_velocity": ([vp, vp], i32),
     "sayaka_dev_profile": ([vp, i32, i32], i32),
     "sayaka_dev_set_bottom_placement": ([vp, i32], i32),
+    "sayaka_dev_set_half_step_defer": ([vp, f64], i32),
     "sayaka_dev_bottom_placement": ([vp, vp], i32),
     "sayaka_dev_profile_read": ([vp, i32, C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                  C.POINTER(f64)], i32),
@@ -532,6 +533,11 @@ class MacSolverGPU:
 
     def profile(self, kclass, enable=True):
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")
+
+    def set_half_step_defer(self, factor=100.0):
+        """Queue an iteration's second half before reading |s| while the last tested norm is at
+        least factor x the stopping threshold (0: always, negative: never; results identical)."""
+        _chk(self.lib.sayaka_dev_set_half_step_defer(self.h, float(factor)), "set_half_step_defer")
 
     def set_bottom_placement(self, one_xcd=True):
         """The coarsest solve on one XCD when it fits (True, the default) or spread over the device."""

@@ -145,6 +145,49 @@ def test_fp64_solve_bit_exact(dcase):
     assert res2["iters"] == res["iters"] and np.array_equal(res2["hist"], res["hist"])
 
 
+def test_fp64_solve_stopping_at_a_half_step(dcase):
+    """A solve that stops at the half step of an iteration (MacSolver::cg_solve_level: x +=
+    alpha phat, then |s| under the tolerance, SayakaMacSolver.cpp:683-920).  The device queues the
+    second half of each iteration before the host reads |s|, and tests |s| on the device to skip
+    the coarsest-level solve of a V-cycle whose result is not wanted; the absolute tolerance is set
+    between the chosen |s| and every value tested before it.  Iterations, history, coarsest-level
+    solve counts and the solution equal the C oracle's."""
+    _, spec, g, dev, tree, inp, bc = dcase
+    h = np.asarray(g["solve_hist"])
+    tested = np.abs(h)
+    half = [i for i in range(1, len(h)) if h[i] < 0 and tested[i] < tested[:i].min()]
+    assert half, "no half step below every earlier test value"
+    i = half[-1]
+    tol = float(np.sqrt(tested[i] * tested[:i].min()))
+    orc = O.Oracle(tree, spec)
+    U.setup_oracle(orc, dict(inp, bc=[bc]))
+    orc.set_rhs(inp["rhs"])
+    orc.set_zero_sol()
+    orc.set_params(0.0, tol, 2000, 1)
+    orc.bottom_stats(reset=True)
+    ro = orc.cg_solve()
+    calls, its = orc.bottom_stats()
+    dev.upload(D.RHS, inp["rhs"])
+    leaf = tree["node_type"] == 1
+    sol_o = orc.get_sol()[leaf]
+    try:
+        # 0: every second half queued before the host reads |s| (the stopping one included:
+        # its coarsest solve skipped on the device); the default 100; negative: none
+        for factor in (0.0, 100.0, -1.0):
+            dev.set_half_step_defer(factor)
+            dev.zero(D.SOL)
+            res = dev.solve(tol_rel=0.0, tol_abs=tol, max_iter=2000, precond=1, precision=D.FP64)
+            print(f"[gpu] half-step stop, defer factor {factor}: iters {res['iters']} (oracle {ro['iters']}), "
+                  f"bottom calls {res['bottom_calls']} / {calls}, its {res['bottom_iters']} / {its}", flush=True)
+            assert res["status"] == 0 and res["half_step"] and ro["converged"]
+            assert res["iters"] == ro["iters"] == i // 2 + 1
+            assert np.array_equal(res["hist"], ro["hist"])
+            assert res["bottom_calls"] == calls == 2 * res["iters"] - 1 and res["bottom_iters"] == its
+            assert np.array_equal(dev.download(D.SOL)[leaf], sol_o)
+    finally:
+        dev.set_half_step_defer(100.0)
+
+
 def test_zero_rhs_returns_immediately(dcase):
     _, spec, g, dev, tree, inp, bc = dcase
     dev.zero(D.RHS)

@@ -132,6 +132,41 @@ def test_c1_mixed_two_level_coarse_solve(c1):
     assert out[True] < 0.6 * out[False]
 
 
+def test_c1_half_step_defer_changes_nothing(c1):
+    """Queuing an iteration's second half before the host reads |s| (defer factor 0: always;
+    negative: never; default 100) changes no iteration, history value, coarsest-solve count or
+    solution bit, in FP64 and in mixed precision, with the stopping test at the full step (the
+    default tolerance) and at a half step (an absolute tolerance between the last half step's
+    |s| and every norm tested before it)."""
+    g, spec, tree, inp, dev = c1
+    try:
+        for prec in (D.FP64, D.MIXED):
+            dev.set_half_step_defer(-1.0)
+            dev.zero(D.SOL)
+            r0 = dev.solve(precision=prec)
+            h = np.abs(r0["hist"])
+            i = max(k for k in range(1, len(h)) if r0["hist"][k] < 0 and h[k] < h[:k].min())
+            tol_half = float(np.sqrt(h[i] * h[:i].min()))
+            for tol_rel, tol_abs in ((1e-10, 0.0), (0.0, tol_half)):
+                out = []
+                for factor in (-1.0, 0.0, 100.0):
+                    dev.set_half_step_defer(factor)
+                    dev.zero(D.SOL)
+                    r = dev.solve(tol_rel=tol_rel, tol_abs=tol_abs, precision=prec)
+                    assert r["status"] == 0
+                    out.append((r, dev.download(D.SOL)))
+                print(f"[gpu] defer {prec} tol ({tol_rel}, {tol_abs:.3e}): iters {out[0][0]['iters']} "
+                      f"half {out[0][0]['half_step']} bottom calls {out[0][0]['bottom_calls']}", flush=True)
+                assert out[0][0]["half_step"] == (tol_abs > 0)
+                for r, sol in out[1:]:
+                    for k in ("iters", "half_step", "bottom_calls", "bottom_iters"):
+                        assert r[k] == out[0][0][k], k
+                    assert np.array_equal(r["hist"], out[0][0]["hist"])
+                    assert np.array_equal(sol, out[0][1])
+    finally:
+        dev.set_half_step_defer(100.0)
+
+
 def test_c1_fp64_fast_within_rounding(c1):
     """Order-free sums: same algorithm, so the reference's tolerance is reached with the
     iteration count within one; the solution agrees to the accuracy a 1e-10 relative
