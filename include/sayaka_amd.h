@@ -175,14 +175,16 @@ int sayaka_dev_transfer_stats(sayaka_dev_ctx* ctx, long long* out4);
  * The results are identical either way. */
 int sayaka_dev_set_bottom_placement(sayaka_dev_ctx* ctx, int one_xcd);
 
-/* When sayaka_dev_solve queues the second half of an iteration before it reads the half-step
- * norm |s| (no reference counterpart; the reference tests |s| on its one thread before going on,
- * SayakaMacSolver.cpp:683-920): with a preconditioner, whenever the last tested norm is at
- * least factor x the stopping threshold max(rnorm0 * tol_rel, tol_abs).  The device then applies
- * the half-step test itself and the queued V-cycle's coarsest-level solve does nothing when it
- * holds; the other kernels of that half run and are discarded.  factor 0 queues every second
- * half, a negative factor none.  Default 100.  Iterations, history, solution and coarsest-solve
- * counts are identical for every factor; only the time differs. */
+/* When sayaka_dev_solve queues a half-iteration before it has read whether the solve stops
+ * (no reference counterpart; the reference tests on its one thread before going on,
+ * SayakaMacSolver.cpp:683-920): with a preconditioner, whenever the last tested norm is at least
+ * factor x the stopping threshold max(rnorm0 * tol_rel, tol_abs), the second half of an
+ * iteration goes in before the host reads |s|, and (FP64 modes) the next iteration's first half
+ * before it reads |r|.  The device applies the stopping tests itself; a queued V-cycle's
+ * coarsest-level solve does nothing when they hold, a queued half step leaves x as it is, and
+ * the other kernels of that half run and are discarded.  factor 0 queues every half, a negative
+ * factor none.  Default 100.  Iterations, history, solution and coarsest-solve counts are
+ * identical for every factor; only the time differs. */
 int sayaka_dev_set_half_step_defer(sayaka_dev_ctx* ctx, double factor);
 
 /* Coarsest-solve launches since the context's creation: out3[0] all, out3[1] placed on one XCD,

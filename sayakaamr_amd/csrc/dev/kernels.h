@@ -84,9 +84,18 @@ struct RedBuf {
     int exact;          // 1: sums in the reference's order (cells k,j,i in a block, then
                         //    blocks in list order: DotProd / mg_dotprod_level,
                         //    SayakaBlockData.cpp:185-208, SayakaMacUtil.cpp:79-106)
-    int* stop = nullptr;          // FIN_SNORM: 1 when the half step stops the solve (alpha_den == 0,
-    double tol_rel = 0.0;         // or |s| < rnorm0 * tol_rel or |s| < tol_abs: the host's test),
-    double tol_abs = 0.0;         // else 0
+    // Stopping tests of a solve's host, evaluated on the device (the same expressions on the
+    // same scalars) for the launches it queues before reading them: stop[0] (FIN_SNORM) = 1 when
+    // the half step stops the solve (alpha_den == 0, |s| < rnorm0 tol_rel, |s| < tol_abs);
+    // stop[1] (FIN_RUPDATE) = 1 when the half step or the full step does (t.t == 0,
+    // |r| < rnorm0 tol_rel + tol_abs, omega == 0, or the next iteration's rho == 0).
+    int* stop = nullptr;
+    double tol_rel = 0.0;
+    double tol_abs = 0.0;
+    // FIN_RUPDATE: a copy of the scalars in host memory (device address of coherent pinned
+    // memory), then seq in its slot S_NSCALAR, for a host that does not wait for the stream
+    double* host = nullptr;
+    unsigned long long seq = 0;
 };
 
 // LDS byte address of a pointer into __shared__ memory.
@@ -261,14 +270,30 @@ __global__ void __launch_bounds__(RED_NT) k_red_finish(RedBuf rb, int n, int nsu
             case FIN_ALPHA: S[S_ALPHA_DEN] = a; S[S_ALPHA] = a != 0.0 ? S[S_RHO] / a : 0.0; break;
             case FIN_SNORM:
                 S[S_SNORM] = m;
-                if (rb.stop) *rb.stop = (S[S_ALPHA_DEN] == 0.0 || m < S[S_RNORM0] * rb.tol_rel || m < rb.tol_abs) ? 1 : 0;
+                if (rb.stop) rb.stop[0] = (S[S_ALPHA_DEN] == 0.0 || m < S[S_RNORM0] * rb.tol_rel || m < rb.tol_abs) ? 1 : 0;
                 break;
             case FIN_OMEGA: S[S_TS] = a; S[S_TT] = b; S[S_OMEGA] = b != 0.0 ? a / b : 0.0; break;
-            case FIN_RUPDATE: S[S_RNORM] = m; S[S_RHO1] = S[S_RHO]; S[S_RHO] = a; break;
+            case FIN_RUPDATE:
+                S[S_RNORM] = m;
+                S[S_RHO1] = S[S_RHO];
+                S[S_RHO] = a;
+                if (rb.stop)
+                    rb.stop[1] = (rb.stop[0] || S[S_TT] == 0.0 || m < S[S_RNORM0] * rb.tol_rel + rb.tol_abs ||
+                                  S[S_OMEGA] == 0.0 || a == 0.0) ? 1 : 0;
+                break;
             case FIN_DOT: S[S_DOT] = a; break;
             case FIN_NORM: S[S_NORM] = m; break;
             case FIN_DOTNORM: S[S_DOT] = a; S[S_AUX0] = b; S[S_NORM] = m; break;
             default: break;
+        }
+        if (rb.host) {   // the values first, then seq with a system-scope release
+            double v[S_NSCALAR];   // every load issued before the first store
+#pragma unroll
+            for (int k = 0; k < S_NSCALAR; k++) v[k] = S[k];
+#pragma unroll
+            for (int k = 0; k < S_NSCALAR; k++) rb.host[k] = v[k];
+            __hip_atomic_store(reinterpret_cast<unsigned long long*>(rb.host + S_NSCALAR), rb.seq, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -801,6 +826,8 @@ struct VecArgs {
     const double* src;
     double* dst;
     const double* scal;
+    const int* xskip;   // V_SUPDATE: x is left as it is when *xskip != 0 (a queued half step of a
+                        // solve that has stopped)
 };
 
 template <int NCELL, int NT>
@@ -840,7 +867,7 @@ __global__ void __launch_bounds__(NT) k_vec(VecArgs a, int op, RedBuf rb, int fi
                 const double al = S[S_ALPHA];
                 const double sv = 1.0 * a.r[gi] + -al * a.v[gi];
                 a.s[gi] = sv;
-                a.x[gi] = 1.0 * a.x[gi] + al * a.phat[gi];
+                if (!a.xskip || !*a.xskip) a.x[gi] = 1.0 * a.x[gi] + al * a.phat[gi];
                 if (a.resid) a.resid[gi] = a.vf[gi] > 0 ? sv : 0.0;
                 if (a.dst) a.dst[gi] = sv;
                 mx = fmax(mx, fabs(sv));

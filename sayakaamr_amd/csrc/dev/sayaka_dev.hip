@@ -10,6 +10,8 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -110,13 +112,22 @@ struct sayaka_dev_ctx {
     double* d_bot_rpub = nullptr;
     double* d_bot_part = nullptr;
     int* d_bot_stats = nullptr;
-    // Half-step test on the device: the solve queues the second half of an iteration before it
-    // reads |s|; the FIN_SNORM finish writes d_stop (1: the solve stops there), and the
-    // coarsest-level solves launched while bottom_skip is set do nothing when it is 1.
+    // Stopping tests on the device (RedBuf::stop), for the half-iterations a solve queues before
+    // it has read whether the solve stops: while stop_flags is set, the FIN_SNORM / FIN_RUPDATE
+    // finishes write d_stop[0] / d_stop[1]; the coarsest-level solves launched while skip_ptr is
+    // set do nothing when *skip_ptr is 1.
     int* d_stop = nullptr;
-    bool half_stop = false, bottom_skip = false;
-    double half_tol_rel = 0.0, half_tol_abs = 0.0;
+    bool stop_flags = false;
+    const int* skip_ptr = nullptr;
+    double stop_tol_rel = 0.0, stop_tol_abs = 0.0;
     double defer_factor = 100.0;   // sayaka_dev_set_half_step_defer
+    // FIN_RUPDATE finishes launched while publish is set copy the scalars into coherent pinned
+    // memory: h_mirror[0 .. S_NSCALAR) and the sequence number in h_mirror[S_NSCALAR] (d_mirror =
+    // its device address); the host reads them without waiting for the rest of the stream.
+    double* h_mirror = nullptr;
+    double* d_mirror = nullptr;
+    bool publish = false;
+    unsigned long long pub_seq = 0;
     unsigned long long* d_bot_flags = nullptr;   // grid-barrier flags of the bottom solve, one per workgroup
     unsigned long long bot_launches = 0;
     int bot_fit[2] = {0, 0};                     // workgroups of k_bottom<.., PC> resident at once
@@ -381,10 +392,14 @@ int launch_vec(sayaka_dev_ctx* c, ListRef lst, int op, const VecArgs& a, int fin
                        redbuf(c), fin);
     if (fin != FIN_NONE) {
         RedBuf rb = redbuf(c);
-        if (fin == FIN_SNORM && c->half_stop) {
+        if ((fin == FIN_SNORM || fin == FIN_RUPDATE) && c->stop_flags) {
             rb.stop = c->d_stop;
-            rb.tol_rel = c->half_tol_rel;
-            rb.tol_abs = c->half_tol_abs;
+            rb.tol_rel = c->stop_tol_rel;
+            rb.tol_abs = c->stop_tol_abs;
+        }
+        if (fin == FIN_RUPDATE && c->publish && c->d_mirror) {
+            rb.host = c->d_mirror;
+            rb.seq = ++c->pub_seq;
         }
         hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(RED_NT), 0, c->stream, rb, lst.n,
                            (op == V_RUPDATE || op == V_DOT) ? 1 : 0, fin);
@@ -480,7 +495,7 @@ struct Impl {
         A.s0inv = c->d_pc_s0inv;
         A.jac = c->d_pc_jac;
         A.l1pos = c->d_l1pos;
-        A.skip = c->bottom_skip ? c->d_stop : nullptr;
+        A.skip = c->skip_ptr;
         using BC = BotCfg<D, NB>;
         const int nwg = (lst.n + BC::BPW - 1) / BC::BPW;
         const void* kern = two_level ? (const void*)k_bottom<D, NB, true> : (const void*)k_bottom<D, NB, false>;
@@ -895,6 +910,30 @@ struct Impl {
         return 0;
     }
 
+    // The scalars a FIN_RUPDATE finish published (sequence number seq) into h_scal, without
+    // waiting for the launches queued after it: spin on the sequence number for 1 ms, then poll
+    // it every ~20 us, asking the stream for errors (a failed launch never publishes).
+    static int read_published(sayaka_dev_ctx* c, unsigned long long seq) {
+        unsigned long long* sq = reinterpret_cast<unsigned long long*>(c->h_mirror + S_NSCALAR);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned n = 1;; n++) {
+            if (__atomic_load_n(sq, __ATOMIC_ACQUIRE) == seq) break;
+            if ((n & 255u) == 0) {
+                const auto dt = std::chrono::steady_clock::now() - t0;
+                if (dt > std::chrono::milliseconds(1)) {
+                    const hipError_t q = hipStreamQuery(c->stream);
+                    if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
+                    if (q == hipSuccess && __atomic_load_n(sq, __ATOMIC_ACQUIRE) != seq)
+                        return fail(-5, "solve: published scalars missing after the stream finished");
+                    std::this_thread::sleep_for(std::chrono::microseconds(20));
+                }
+            }
+            __builtin_ia32_pause();
+        }
+        std::memcpy(c->h_scal, c->h_mirror, S_NSCALAR * sizeof(double));
+        return 0;
+    }
+
     static int read_scalars(sayaka_dev_ctx* c) {
         HIPCHK(hipMemcpyAsync(c->h_scal, c->d_scal, S_NSCALAR * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1146,15 +1185,19 @@ struct Impl {
         int stat = 0, iter;
         double* phat = precond ? F[SAYAKA_PHAT] : F[SAYAKA_P];
         double* shat = precond ? F[SAYAKA_SHAT] : F[SAYAKA_S];
-        // The iterate x - x0 alternates between X and X0: the full step writes the buffer the
-        // half step did not, so the half step's x survives the second half of its iteration.
-        // With a preconditioner, and while the last tested norm is at least defer_factor x the
-        // stopping threshold (a half step that far from convergence practically never stops),
-        // the second half is queued before the host reads |s| (one host round trip per
-        // iteration instead of two): the FIN_SNORM finish evaluates the half-step test on the
-        // device, and the second half's coarsest-level solve does nothing when it holds.  The
-        // host then applies the same test to the same |s|, so every decision, the history and
-        // the bottom-solve counts are those of testing before the second half.
+        // The iterate x - x0 alternates between X and X0: the half step updates it in place, the
+        // full step writes the other field, so the half step's x survives the second half.
+        //
+        // With a preconditioner, and while the last norm the host has tested is at least
+        // defer_factor x the stopping threshold (a half-iteration that far from convergence
+        // practically never stops), the solve queues a half-iteration before it has read whether
+        // the previous one stopped: the second half before it reads |s|, the next iteration's
+        // first half before it reads |r|.  The finishes of the two reductions then evaluate the
+        // host's stopping tests on the device (RedBuf::stop), the queued V-cycle's coarsest-level
+        // solve does nothing when they hold, and the queued half step leaves x as it is; the
+        // host reads |r| from the copy the finish publishes (read_published) while the queued
+        // work runs.  The host applies the same tests to the same values, so every decision, the
+        // history, the coarsest-solve counts and the solution are those of testing first.
         double* xc = F[SAYAKA_X];
         double* xo = F[SAYAKA_X0];
         double* xsol = xc;
@@ -1162,10 +1205,15 @@ struct Impl {
         double last_tested = rnorm0;
         struct DeferScope {
             sayaka_dev_ctx* c;
-            ~DeferScope() { c->half_stop = c->bottom_skip = false; }
+            ~DeferScope() {
+                c->stop_flags = c->publish = false;
+                c->skip_ptr = nullptr;
+            }
         } defer_scope{c};
-        c->half_tol_rel = P->tol_rel;
-        c->half_tol_abs = P->tol_abs;
+        c->stop_flags = precond != 0;
+        c->stop_tol_rel = P->tol_rel;
+        c->stop_tol_abs = P->tol_abs;
+        auto far = [&]() { return precond && c->defer_factor >= 0.0 && last_tested >= c->defer_factor * thr; };
         // the half-step test (SayakaMacSolver.cpp:683-920): breakdown, then |s| against the tolerances
         auto half_step_test = [&]() -> bool {
             if (c->h_scal[S_ALPHA_DEN] == 0.0) {
@@ -1175,6 +1223,7 @@ struct Impl {
             const double snorm = c->h_scal[S_SNORM];
             push(-snorm);
             st->rnorm = snorm;
+            last_tested = snorm;
             if (snorm < rnorm0 * P->tol_rel || snorm < P->tol_abs) {
                 conv = true;
                 st->half_step = 1;
@@ -1183,22 +1232,22 @@ struct Impl {
             }
             return false;
         };
-        for (iter = 1; iter <= P->max_iter; iter++) {
-            st->iters = iter;
-            if (c->h_scal[S_RHO] == 0.0) { stat = 1; break; }
-            const bool defer = precond && c->defer_factor >= 0.0 && last_tested >= c->defer_factor * thr;
-            c->half_stop = defer;
+        // first half of an iteration: p, phat = M^-1 p, v = A phat, alpha, s, x += alpha phat, |s|;
+        // skip: the stop flag this half is queued behind (its coarsest solve and x update)
+        auto first_half = [&](int it, const int* skip) -> int {
             {
                 VecArgs a = vec_args(c, T);
                 a.r = F[SAYAKA_R];
                 a.p = F[SAYAKA_P];
                 a.v = F[SAYAKA_V];
                 a.resid = F[SAYAKA_RESID];
-                if ((e = launch_vec<D, NB>(c, T, iter == 1 ? V_PINIT : V_PUPDATE, a, FIN_NONE))) return e;
+                if ((e = launch_vec<D, NB>(c, T, it == 1 ? V_PINIT : V_PUPDATE, a, FIN_NONE))) return e;
             }
             if (precond) {
+                c->skip_ptr = skip;
                 if (mixed) e = mixed_vcycle(c, L, F[SAYAKA_RESID], phat, P->pre_smooth, P->post_smooth);
                 else e = vcycle(c, L, phat);
+                c->skip_ptr = nullptr;
                 if (e) return e;
             }
             {
@@ -1208,25 +1257,23 @@ struct Impl {
                 a.dotv = F[SAYAKA_RHAT];
                 if ((e = launch_stencil<D, NB>(c, ST_APPLY_DOT, a, FIN_ALPHA))) return e;
             }
-            {
-                VecArgs a = vec_args(c, T);
-                a.r = F[SAYAKA_R];
-                a.v = F[SAYAKA_V];
-                a.s = F[SAYAKA_S];
-                a.x = xc;
-                a.phat = phat;
-                a.resid = F[SAYAKA_RESID];
-                if ((e = launch_vec<D, NB>(c, T, V_SUPDATE, a, FIN_SNORM))) return e;
-            }
-            if (!defer) {
-                if ((e = read_scalars(c))) return e;
-                if (half_step_test()) break;
-            }
+            VecArgs a = vec_args(c, T);
+            a.r = F[SAYAKA_R];
+            a.v = F[SAYAKA_V];
+            a.s = F[SAYAKA_S];
+            a.x = xc;
+            a.phat = phat;
+            a.resid = F[SAYAKA_RESID];
+            a.xskip = skip;
+            return launch_vec<D, NB>(c, T, V_SUPDATE, a, FIN_SNORM);
+        };
+        // second half: shat = M^-1 s, t = A shat, omega, x (into xo) += omega shat, r, |r|, rho
+        auto second_half = [&](const int* skip, bool pub) -> int {
             if (precond) {
-                c->bottom_skip = defer;
+                c->skip_ptr = skip;
                 if (mixed) e = mixed_vcycle(c, L, F[SAYAKA_RESID], shat, P->pre_smooth, P->post_smooth);
                 else e = vcycle(c, L, shat);
-                c->bottom_skip = false;
+                c->skip_ptr = nullptr;
                 if (e) return e;
             }
             {
@@ -1236,19 +1283,51 @@ struct Impl {
                 a.dotv = F[SAYAKA_S];
                 if ((e = launch_stencil<D, NB>(c, ST_APPLY_DOT2, a, FIN_OMEGA))) return e;
             }
-            {
-                VecArgs a = vec_args(c, T);
-                a.r = F[SAYAKA_R];
-                a.s = F[SAYAKA_S];
-                a.t = F[SAYAKA_T];
-                a.x = xc;
-                a.dst = xo;
-                a.src = shat;
-                a.rhat = F[SAYAKA_RHAT];
-                if ((e = launch_vec<D, NB>(c, T, V_RUPDATE, a, FIN_RUPDATE))) return e;
+            VecArgs a = vec_args(c, T);
+            a.r = F[SAYAKA_R];
+            a.s = F[SAYAKA_S];
+            a.t = F[SAYAKA_T];
+            a.x = xc;
+            a.dst = xo;
+            a.src = shat;
+            a.rhat = F[SAYAKA_RHAT];
+            c->publish = pub;
+            e = launch_vec<D, NB>(c, T, V_RUPDATE, a, FIN_RUPDATE);
+            c->publish = false;
+            return e;
+        };
+        bool queued = false;   // the first half of iteration `iter` is queued already
+        for (iter = 1; iter <= P->max_iter; iter++) {
+            st->iters = iter;
+            if (!queued) {
+                if (c->h_scal[S_RHO] == 0.0) { stat = 1; break; }
+                if ((e = first_half(iter, nullptr))) return e;
             }
-            if ((e = read_scalars(c))) return e;
-            if (defer && half_step_test()) break;
+            queued = false;
+            const bool defer_half = far();
+            if (!defer_half) {
+                if ((e = read_scalars(c))) return e;
+                if (half_step_test()) break;
+            }
+            // the next iteration's first half goes in before |r| is read when still far from the
+            // threshold (and the iteration limit leaves one); FP64 modes only: the mixed cycle
+            // cuts the residual by 1e2-1e4 per iteration, so the factor does not keep its last
+            // full step from paying for a queued first half (~0.45 ms of FP32 V-cycle kernels)
+            const bool defer_full = !mixed && iter < P->max_iter && far();
+            if ((e = second_half(defer_half ? c->d_stop : nullptr, defer_full))) return e;
+            const unsigned long long seq = c->pub_seq;
+            if (defer_full) {
+                // in place on xo, the x the full step writes (left as it is when the solve stops)
+                std::swap(xc, xo);
+                e = first_half(iter + 1, c->d_stop + 1);
+                std::swap(xc, xo);
+                if (e) return e;
+                if ((e = read_published(c, seq))) return e;
+                queued = true;
+            } else if ((e = read_scalars(c))) {
+                return e;
+            }
+            if (defer_half && half_step_test()) break;
             if (c->h_scal[S_TT] == 0.0) { stat = 1; break; }
             const double rnorm = c->h_scal[S_RNORM];
             push(rnorm);
@@ -1261,6 +1340,11 @@ struct Impl {
             }
             if (c->h_scal[S_OMEGA] == 0.0) { stat = 1; break; }
             std::swap(xc, xo);
+            if (queued && c->h_scal[S_RHO] == 0.0) {   // the next iteration's opening test
+                st->iters = iter + 1;
+                stat = 1;
+                break;
+            }
         }
         if (conv) {
             VecArgs a = vec_args(c, T);
@@ -1373,8 +1457,13 @@ int sayaka_dev_create(int device, int ndim, int nb, int max_blocks, sayaka_dev_c
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_fsum, (size_t)PC_MAXBLK * 6 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_pc_s0inv, (size_t)PC_MAXBLK * PC_MAXBLK * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_stats, 8 * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&c->d_stop, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(c->d_stop, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->d_stop, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_stop, 0, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_mirror, (S_NSCALAR + 1) * sizeof(double), hipHostMallocCoherent);
+    if (e == hipSuccess) {
+        std::memset(c->h_mirror, 0, (S_NSCALAR + 1) * sizeof(double));
+        e = hipHostGetDevicePointer((void**)&c->d_mirror, c->h_mirror, 0);
+    }
     if (e == hipSuccess) e = hipMalloc(&c->d_bot_flags, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->d_bot_flags, 0, (size_t)(max_blocks + 64) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_bot_stats, 8 * sizeof(int), hipHostMallocDefault);
@@ -1419,6 +1508,7 @@ int sayaka_dev_destroy(sayaka_dev_ctx* c) {
         }
     for (auto e : c->ev_pool) hipEventDestroy(e);
     if (c->h_scal) hipHostFree(c->h_scal);
+    if (c->h_mirror) hipHostFree(c->h_mirror);
     if (c->h_bot_stats) hipHostFree(c->h_bot_stats);
     mixed_free(c->mixed);
     cnn_free(c->cnn);

@@ -535,7 +535,8 @@ class MacSolverGPU:
         _chk(self.lib.sayaka_dev_profile(self.h, kclass, int(enable)), "profile")
 
     def set_half_step_defer(self, factor=100.0):
-        """Queue an iteration's second half before reading |s| while the last tested norm is at
+        """Queue a half-iteration before reading whether the solve stops (the second half before
+        |s|; in FP64 modes also the next first half before |r|) while the last tested norm is at
         least factor x the stopping threshold (0: always, negative: never; results identical)."""
         _chk(self.lib.sayaka_dev_set_half_step_defer(self.h, float(factor)), "set_half_step_defer")
 
